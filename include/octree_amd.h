@@ -1,0 +1,152 @@
+/*
+ * octree_amd.h -- C ABI of the MI355X (gfx950) SFC-encode / reorder / linear-octree / neighbour-
+ * search library. Plain C, extern "C", plain pointers and sizes; no C++ or torch types cross it.
+ *
+ * Each entry point replaces one piece of the reference's header-only C++ API
+ * (derivada/octrees-benchmark; paths relative to its root). The drop-in C++ facade
+ * (octrees-benchmark_amd/facade/octree/ headers) keeps the reference's class names and signatures and
+ * forwards to these calls.
+ *
+ * Conventions
+ *  - Every call returns OCT_OK (0) or a negative OCT_ERR_* code; oct_last_error() gives the text.
+ *    Nothing throws across the ABI and nothing calls exit() (the reference exit()s on overflow at
+ *    linear_octree.hpp:387-390; here that is OCT_ERR_STATE).
+ *  - A context owns one HIP stream on one device and every device buffer; calls on one context are
+ *    not re-entrant (the facade serialises them with a mutex).
+ *  - Host buffers are copied in/out; "_dev" variants take/return device pointers on the context's
+ *    device and enqueue on the context's stream.
+ *  - Indices are positions in the SFC-sorted cloud, as in the reference (searches run after
+ *    sortPoints, so the reference returns sorted-cloud indices).
+ */
+#ifndef OCTREE_AMD_H
+#define OCTREE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oct_ctx oct_ctx;
+
+enum oct_status {
+    OCT_OK = 0,
+    OCT_ERR_ARG = -1,         /* invalid argument (null pointer, unknown id, size limit) */
+    OCT_ERR_HIP = -2,         /* HIP runtime error */
+    OCT_ERR_STATE = -3,       /* call out of order (e.g. search before build) or device check failed */
+    OCT_ERR_NOMEM = -4,       /* device allocation failed */
+    OCT_ERR_UNSUPPORTED = -5  /* encoder/kernel combination the reference does not support */
+};
+
+/* Encoder ids = the reference's EncoderType values (inc/main_options.hpp:73-83). Only the two 3-D
+ * encoders can build a LinearOctree (inc/benchmarking/neighbor_benchmarks.hpp:782-786). */
+enum oct_encoder { OCT_ENC_MORTON3D = 1, OCT_ENC_HILBERT3D = 5 };
+/* Kernel ids = the reference's Kernel_t values (inc/main_options.hpp:97-103). */
+enum oct_kernel { OCT_KERNEL_CIRCLE = 0, OCT_KERNEL_SPHERE = 1, OCT_KERNEL_SQUARE = 2, OCT_KERNEL_CUBE = 3 };
+/* Search algorithms = the reference's SearchAlgo values (inc/main_options.hpp:29-35). */
+enum oct_algo { OCT_ALGO_NEIGHBORS = 1, OCT_ALGO_PRUNE = 2, OCT_ALGO_STRUCT = 3 };
+/* Arithmetic of the per-point / per-node tests. FP64 evaluates the reference's expressions as
+ * written. FP32_RECHECK evaluates them in FP32 on leaf-recentred coordinates and re-evaluates in
+ * FP64 every decision that falls inside a proven rounding band; results are identical. */
+enum oct_precision { OCT_PREC_FP64 = 0, OCT_PREC_FP32_RECHECK = 1 };
+
+typedef struct oct_tree_info {
+    uint32_t n_leaf;          /* leaves (cornerstone nLeaf) */
+    uint32_t n_internal;      /* internal nodes, (nLeaf - 1) / 7 */
+    uint32_t n_total;         /* nLeaf + nInternal */
+    uint32_t max_depth_seen;  /* deepest level created by a split */
+    uint32_t max_points_leaf; /* build parameter */
+    uint32_t n_levels;        /* levels holding nodes */
+    double build_ms;          /* device time of the build kernels */
+} oct_tree_info;
+
+typedef struct oct_search_stats {
+    uint64_t total_neighbors;  /* sum over queries of |N(q)| */
+    uint64_t points_tested;    /* per-point predicate evaluations */
+    uint64_t nodes_classified; /* node predicate evaluations */
+    uint64_t rechecked;        /* FP32 decisions re-evaluated in FP64 (points + nodes) */
+    double kernel_ms;          /* device time of the search kernel(s) */
+} oct_search_stats;
+
+/* ---- context -------------------------------------------------------------------------------- */
+int oct_create(oct_ctx **ctx, int device);
+int oct_destroy(oct_ctx *ctx);
+const char *oct_last_error(const oct_ctx *ctx);
+int oct_synchronize(oct_ctx *ctx);
+const char *oct_version(void);
+
+/* ---- H1 points + bounding box ----------------------------------------------------------------
+ * oct_set_points: copies n points; stride_bytes is the byte distance between consecutive points
+ * whose first three doubles are x, y, z (24 for packed xyz, 32 for the reference's Point struct,
+ * inc/geometry/point.hpp:23-29). Replaces the container hand-off of sortPoints.
+ * oct_bbox: mbb (inc/geometry/box.hpp:94-137) -> center = min + (max - min) / 2,
+ * radii = (max - min) / 2; the encode/octree Box is Box(center, radii) (box.hpp:17). */
+int oct_set_points(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_bytes);
+int oct_set_points_dev(oct_ctx *ctx, const double *xyz_dev, size_t n, size_t stride_bytes);
+int oct_bbox(oct_ctx *ctx, double center[3], double radii[3]);
+
+/* ---- H2 encode: PointEncoder::encodePoints (point_encoder.hpp:54-63) with getAnchorCoords
+ * (:37-49) and MortonEncoder3D::encode (morton_encoder_3d.hpp:35-37) or HilbertEncoder3D::encode
+ * (hilbert_encoder_3d.hpp:29-66). keys_out (nullable): n keys in input order. */
+int oct_encode(oct_ctx *ctx, int encoder, const double center[3], const double radii[3], uint64_t *keys_out);
+
+/* ---- H3 reorder: the radix passes of sortPoints (point_encoder.hpp:285-383) -- a stable sort by
+ * key. perm_out[i] (nullable) = input index of the point now at sorted position i; keys_out
+ * (nullable) = sorted keys. The device keeps the sorted cloud for the build and searches. */
+int oct_sort(oct_ctx *ctx, uint32_t *perm_out, uint64_t *keys_out);
+
+/* One call for sortPoints(points, meta) (point_encoder.hpp:484-503): bbox + encode + sort. */
+int oct_sort_points(oct_ctx *ctx, int encoder, double center_out[3], double radii_out[3], uint32_t *perm_out,
+                    uint64_t *keys_out);
+
+/* Upload an already sorted cloud with its keys and Box (LinearOctree ctor inputs,
+ * linear_octree.hpp:614-618), replacing what oct_sort left on the device. */
+int oct_set_sorted(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_bytes, const uint64_t *codes,
+                   int encoder, const double center[3], const double radii[3]);
+
+/* ---- H4 build: LinearOctree(points, codes, box, enc) -> buildOctree (linear_octree.hpp:567-623);
+ * maxPointsLeaf is the reference's global mainOptions.maxPointsLeaf (main_options.hpp:210). */
+int oct_build(oct_ctx *ctx, uint32_t max_points_leaf, oct_tree_info *info);
+
+/* Tree dump for parity checks, in the reference's node order (level-major, key-minor).
+ * Any pointer may be NULL. prefixes: nTotal placeholder-bit keys (inter.prefixes); offsets: nTotal
+ * first-child indices, 0 = leaf (offsets); ranges: nTotal x 2 [first, last) point ranges
+ * (internalRanges); centers: nTotal x 3 (centers); radii: 22 x 3 (precomputedRadii). */
+int oct_get_tree(oct_ctx *ctx, uint64_t *prefixes, uint32_t *offsets, uint64_t *ranges, double *centers,
+                 double *radii);
+
+/* ---- H5 radius searches: neighbors / neighborsPrune / neighborsStruct over the four kernels
+ * (linear_octree.hpp:695-816, 1038-1087; kernel_factory.hpp:23-41).
+ * Queries: either centres (centers != NULL, nq x 3 doubles) or indices into the sorted cloud
+ * (center_idx != NULL), or, if both are NULL, all points 0..nq-1 of the sorted cloud.
+ * radii: nq per-query radii, or NULL to use radius for every query (a double, as the kernels'
+ * constructors take, kernel_3d.hpp:123).
+ * counts_out (nq, nullable): |N(q)| (the harness keeps result.size(), neighbor_benchmarks.hpp:509).
+ * Then oct_radius_fetch copies the result lists of the last search:
+ *   neighbors/Prune -> indices (CSR, offsets nq+1), Struct -> [first,last) ranges (CSR of pairs),
+ *   each list in the reference's own emission order. */
+int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *centers, const uint32_t *center_idx,
+               const double *radii, double radius, size_t nq, int want_lists, uint32_t *counts_out,
+               oct_search_stats *stats);
+/* After oct_radius(want_lists=1): offsets (nq+1) and entries (offsets[nq] u32 indices, or 2 u32 per range).
+ * entries == NULL fetches the offsets only (size query). */
+int oct_radius_fetch(oct_ctx *ctx, uint64_t *offsets, uint32_t *entries, size_t entries_capacity);
+/* Device-resident variant for throughput runs: counts stay on the device (counts_dev may be NULL). */
+int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint32_t *center_idx_dev, double radius,
+                   size_t nq, uint32_t *counts_dev, oct_search_stats *stats);
+
+/* ---- H6 kNN: LinearOctree::knn (linear_octree.hpp:899-942), benchmark label KNNV2. Exact k
+ * nearest under the (squared FP64 distance, sorted index) order; returns min(n, k) per query,
+ * ascending. idx_out: nq x k int32 (-1 pads), d2_out: nq x k FP64 squared distances (NaN pads). */
+int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *center_idx, size_t nq, uint32_t k,
+            int32_t *idx_out, double *d2_out, oct_search_stats *stats);
+
+/* ---- timing: device milliseconds of the stages of the last calls (HIP events on the context's
+ * stream): [0] bbox, [1] encode, [2] sort, [3] gather, [4] build, [5] search, [6] knn. */
+int oct_stage_times(const oct_ctx *ctx, double ms[8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

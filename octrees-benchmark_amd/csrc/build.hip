@@ -1,0 +1,206 @@
+// H4: linear-octree build on gfx950.
+//
+// The reference grows a cornerstone leaf array by rounds of split decisions (linear_octree.hpp:
+// 151-333, 458-489), then derives the level-ordered node arrays (offsets, internalRanges, centers)
+// by prefix sort, linking and recursion (:336-565). Starting from the root, its merges never fire,
+// so its tree is the canonical one: a node splits iff it holds more than maxPointsLeaf points and
+// its level is below 21, and nodes are numbered level by level, in key order within a level.
+//
+// This build produces those arrays directly, top-down, one level per step:
+//   level l nodes -> (flag = count > M && l < 21) -> exclusive scan of flags -> children of the
+//   flagged nodes are the next level, in key order, at indices levelStart[l+1] + 8 * rank + octant.
+// A child's point range comes from two binary searches of the sorted keys inside its parent's
+// range (the reference's lower_bound counting, :317-321); its FP64 centre is getCenter
+// (point_encoder.hpp:517-539) evaluated in the reference's operation order.
+// A last pass writes the leaf-recentred FP32 copy of the cloud used by the FP32 searches.
+#include "device.hpp"
+
+namespace oct {
+
+struct BuildGeom {
+    double boxmin[3];
+    double hl[3];
+    double rad[22][3];
+    int encoder;
+};
+
+__device__ inline uint32_t lower_bound_keys(const uint64_t *__restrict__ keys, uint32_t lo, uint32_t hi, uint64_t v) {
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline void write_node(NodeRec *__restrict__ nodes, uint64_t *__restrict__ nkey, uint32_t u, uint64_t key,
+                                  uint32_t level, uint32_t first, uint32_t last, const BuildGeom &g) {
+    double c[3];
+    node_center(g.encoder, key, level, g.boxmin, g.hl, g.rad[level], c);
+    NodeRec r;
+    r.cx = c[0], r.cy = c[1], r.cz = c[2];
+    r.first = first, r.last = last;
+    nodes[u] = r;
+    nkey[u] = key;
+}
+
+__global__ void k_root(NodeRec *nodes, uint64_t *nkey, uint32_t *flags, uint32_t n, uint32_t max_leaf, BuildGeom g) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        write_node(nodes, nkey, 0, 0ull, 0, 0, n, g);
+        flags[0] = (n > max_leaf) ? 1u : 0u;
+    }
+}
+
+// One thread per node of level `level` (indices [ls, le)). Flagged nodes write their 8 children.
+__global__ __launch_bounds__(256) void k_children(const uint64_t *__restrict__ keys, NodeRec *__restrict__ nodes,
+                                                 uint64_t *__restrict__ nkey, uint32_t *__restrict__ child,
+                                                 const uint32_t *__restrict__ flags, const uint32_t *__restrict__ rank,
+                                                 uint32_t *__restrict__ next_flags, uint32_t ls, uint32_t le, uint32_t level,
+                                                 uint32_t max_leaf, BuildGeom g) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= le - ls) return;
+    uint32_t u = ls + i;
+    if (!flags[i]) {
+        child[u] = 0;
+        return;
+    }
+    const uint32_t cb = le + 8u * rank[i];
+    child[u] = cb;
+    const NodeRec pr = nodes[u];
+    const uint64_t pkey = nkey[u];
+    const uint32_t cl = level + 1;
+    const uint64_t step = node_range(cl);
+    uint32_t lo = pr.first;
+    for (uint32_t c = 0; c < 8; c++) {
+        uint32_t hi = (c == 7) ? pr.last : lower_bound_keys(keys, lo, pr.last, pkey + (uint64_t)(c + 1) * step);
+        write_node(nodes, nkey, cb + c, pkey + (uint64_t)c * step, cl, lo, hi, g);
+        next_flags[cb - le + c] = (hi - lo > max_leaf && cl < kMaxDepth) ? 1u : 0u;
+        lo = hi;
+    }
+}
+
+// Leaf-recentred FP32 coordinates: a = fl32(p - c_leaf) per axis (FP64 subtraction, one rounding to
+// FP32), w = fl32(|a|^2) computed in FP64 from the FP32 values. One wave per leaf; the wave also
+// records amax = max |a|^2 over the leaf (rounded up), which bounds the FP32 rounding band.
+__global__ __launch_bounds__(256) void k_recentre(const double4 *__restrict__ p64, const NodeRec *__restrict__ nodes,
+                                                 const uint32_t *__restrict__ child, uint32_t n_total, float4 *__restrict__ p32,
+                                                 float *__restrict__ amax) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    if (wave >= n_total) return;
+    if (child[wave] != 0) return;
+    const NodeRec r = nodes[wave];
+    double m = 0.0;
+    for (uint32_t i = r.first + lane; i < r.last; i += 64) {
+        double4 p = p64[i];
+        float ax = (float)(p.x - r.cx), ay = (float)(p.y - r.cy), az = (float)(p.z - r.cz);
+        double w = (double)ax * (double)ax + (double)ay * (double)ay + (double)az * (double)az;
+        p32[i] = make_float4(ax, ay, az, (float)w);
+        m = w > m ? w : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if (lane == 0) amax[wave] = __double2float_ru(m);
+}
+
+template <class T>
+static hipError_t grow_preserve(oct_ctx *ctx, DevBuf<T> &b, size_t need, size_t valid) {
+    if (need <= b.cap) return hipSuccess;
+    size_t cap = b.cap ? b.cap : 1024;
+    while (cap < need) cap *= 2;
+    T *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, cap * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (b.p && valid) {
+        e = hipMemcpyAsync(p, b.p, valid * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return e;
+    }
+    if (b.p) (void)hipFree(b.p);
+    b.p = p;
+    b.cap = cap;
+    return hipSuccess;
+}
+
+hipError_t launch_build(oct_ctx *ctx, uint32_t M, std::string &err) {
+    const uint32_t n = (uint32_t)ctx->n;
+    BuildGeom g;
+    g.encoder = ctx->encoder;
+    for (int a = 0; a < 3; a++) g.boxmin[a] = ctx->boxmin[a], g.hl[a] = ctx->half_lengths[a];
+    for (int l = 0; l < 22; l++)
+        for (int a = 0; a < 3; a++) g.rad[l][a] = ctx->level_radii[l][a];
+
+    hipError_t e;
+    size_t est = 64 + 4 * (size_t)n / (M ? M : 1);
+    if ((e = grow_preserve(ctx, ctx->nodes, est, 0)) != hipSuccess) return e;
+    if ((e = grow_preserve(ctx, ctx->nkey, est, 0)) != hipSuccess) return e;
+    if ((e = grow_preserve(ctx, ctx->child, est + 1, 0)) != hipSuccess) return e;
+    if ((e = grow_preserve(ctx, ctx->flags, est, 0)) != hipSuccess) return e;
+    if ((e = grow_preserve(ctx, ctx->nrank, est, 0)) != hipSuccess) return e;
+    if ((e = ctx->misc.reserve(12 * 1024 + 64)) != hipSuccess) return e;
+    uint32_t *total_dev = ctx->misc.p + 8;
+
+    stage_begin(ctx, ST_BUILD);
+    k_root<<<1, 64, 0, ctx->stream>>>(ctx->nodes.p, ctx->nkey.p, ctx->flags.p, n, M, g);
+    uint32_t ls = 0, le = 1, level = 0, n_internal = 0;
+    ctx->level_start[0] = 0;
+    ctx->level_start[1] = 1;
+    for (;; level++) {
+        const uint32_t cnt = le - ls;
+        if ((e = exclusive_scan_u32(ctx, ctx->flags.p + ls, ctx->nrank.p + ls, cnt, total_dev)) != hipSuccess) return e;
+        uint32_t nint = 0;
+        if ((e = hipMemcpyAsync(&nint, total_dev, sizeof(nint), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+        if (nint > cnt || (nint && level >= kMaxDepth)) {
+            err = "build: inconsistent split count";
+            return hipErrorUnknown;
+        }
+        const size_t newle = (size_t)le + 8ull * nint;
+        if (newle >= (1ull << 31)) {
+            err = "build: node count exceeds 2^31";
+            return hipErrorInvalidValue;
+        }
+        if ((e = grow_preserve(ctx, ctx->nodes, newle, le)) != hipSuccess) return e;
+        if ((e = grow_preserve(ctx, ctx->nkey, newle, le)) != hipSuccess) return e;
+        if ((e = grow_preserve(ctx, ctx->child, newle + 1, le)) != hipSuccess) return e;
+        if ((e = grow_preserve(ctx, ctx->flags, newle, le)) != hipSuccess) return e;
+        if ((e = grow_preserve(ctx, ctx->nrank, newle, le)) != hipSuccess) return e;
+        unsigned nb = (cnt + 255) / 256;
+        k_children<<<nb, 256, 0, ctx->stream>>>(ctx->sorted_keys, ctx->nodes.p, ctx->nkey.p, ctx->child.p, ctx->flags.p + ls,
+                                                ctx->nrank.p + ls, ctx->flags.p + le, ls, le, level, M, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        n_internal += nint;
+        ctx->level_start[level + 2] = (uint32_t)newle;
+        if (nint == 0) break;
+        ls = le;
+        le = (uint32_t)newle;
+    }
+    const uint32_t n_total = le;
+    // sentinel: offsets has nTotal + 1 entries in the reference (linear_octree.hpp:497)
+    if ((e = hipMemsetAsync(ctx->child.p + n_total, 0, sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
+    for (uint32_t l = level + 2; l < kMaxDepth + 3; l++) ctx->level_start[l] = n_total;
+    ctx->n_total = n_total;
+    ctx->n_internal = n_internal;
+    ctx->n_leaf = n_total - n_internal;
+    ctx->n_levels = level + 1;
+    ctx->max_depth_seen = level;
+    ctx->max_points_leaf = M;
+
+    // leaf-recentred FP32 cloud
+    if ((e = ctx->p32.reserve(ctx->n)) != hipSuccess) return e;
+    if ((e = ctx->amax.reserve(n_total)) != hipSuccess) return e;
+    if (n_total) {
+        size_t threads = (size_t)n_total * 64;
+        unsigned nb = (unsigned)((threads + 255) / 256);
+        k_recentre<<<nb, 256, 0, ctx->stream>>>(ctx->p64.p, ctx->nodes.p, ctx->child.p, n_total, ctx->p32.p, ctx->amax.p);
+    }
+    stage_end(ctx, ST_BUILD);
+    return hipGetLastError();
+}
+
+}  // namespace oct

@@ -1,0 +1,523 @@
+// C ABI (include/octree_amd.h): context management, argument checking, host<->device staging, and
+// the sequencing of the gfx950 kernels. Every HIP call is checked; errors become negative status
+// codes with a message in oct_last_error().
+#include <algorithm>
+#include <cfloat>
+#include <cstdio>
+#include <numeric>
+
+#include "device.hpp"
+
+using namespace oct;
+
+namespace oct {
+void stage_begin(oct_ctx *ctx, Stage s) { (void)hipEventRecord(ctx->ev0[s], ctx->stream); }
+void stage_end(oct_ctx *ctx, Stage s) {
+    (void)hipEventRecord(ctx->ev1[s], ctx->stream);
+    ctx->ev_used[s] = true;
+}
+}  // namespace oct
+
+namespace {
+
+int fail(oct_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hfail(oct_ctx *ctx, hipError_t e, const char *where, const std::string &extra = "") {
+    std::string m = std::string(where) + ": " + hipGetErrorString(e);
+    if (!extra.empty()) m += " (" + extra + ")";
+    return fail(ctx, e == hipErrorOutOfMemory ? OCT_ERR_NOMEM : OCT_ERR_HIP, m);
+}
+
+#define HCHK(expr, where)                                  \
+    do {                                                   \
+        hipError_t _e = (expr);                            \
+        if (_e != hipSuccess) return hfail(ctx, _e, where); \
+    } while (0)
+
+bool valid_encoder(int e) { return e == OCT_ENC_MORTON3D || e == OCT_ENC_HILBERT3D; }
+
+__global__ void k_pad(const double *__restrict__ xyz, double4 *__restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0);
+}
+
+__global__ void k_unstride(const double *__restrict__ src, size_t stride_d, double *__restrict__ xyz, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        xyz[3 * i] = src[stride_d * i];
+        xyz[3 * i + 1] = src[stride_d * i + 1];
+        xyz[3 * i + 2] = src[stride_d * i + 2];
+    }
+}
+
+// Copies n points (x, y, z leading doubles, stride bytes apart) into the packed device array.
+// Packed input is one flat copy; strided input (e.g. the 32-byte Point) is copied flat into a
+// staging buffer and repacked on the device.
+int upload_xyz(oct_ctx *ctx, const double *src, size_t n, size_t stride, hipMemcpyKind kind) {
+    if (stride < 24 || stride % 8) return fail(ctx, OCT_ERR_ARG, "stride_bytes must be >= 24 and a multiple of 8");
+    if (n >= (1ull << 30)) return fail(ctx, OCT_ERR_ARG, "at most 2^30 - 1 points");
+    HCHK(ctx->xyz.reserve(3 * n), "alloc points");
+    if (n) {
+        if (!src) return fail(ctx, OCT_ERR_ARG, "null point buffer");
+        if (stride == 24) {
+            HCHK(hipMemcpyAsync(ctx->xyz.p, src, n * 24, kind, ctx->stream), "copy points");
+        } else {
+            const size_t words = n * (stride / 8);
+            HCHK(ctx->qxyz.reserve(words), "alloc staging");
+            HCHK(hipMemcpyAsync(ctx->qxyz.p, src, n * stride, kind, ctx->stream), "copy points");
+            unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+            k_unstride<<<nb, 256, 0, ctx->stream>>>(ctx->qxyz.p, stride / 8, ctx->xyz.p, n);
+            HCHK(hipGetLastError(), "repack points");
+        }
+    }
+    ctx->n = n;
+    return OCT_OK;
+}
+
+// Box(center, radii): min' = center - radii, max' = center + radii (box.hpp:17); halfLengths and
+// precomputedRadii as setupBbox (linear_octree.hpp:439-455).
+void setup_geometry(oct_ctx *ctx) {
+    const double eps = (double)(1.0f / (float)(1 << kMaxDepth));
+    for (int a = 0; a < 3; a++) {
+        double mn = ctx->center[a] - ctx->radii[a], mx = ctx->center[a] + ctx->radii[a];
+        ctx->boxmin[a] = mn;
+        ctx->half_lengths[a] = (double)0.5f * eps * (mx - mn);
+    }
+    for (uint32_t l = 0; l <= kMaxDepth; l++) {
+        double side = (double)(1u << (kMaxDepth - l));
+        for (int a = 0; a < 3; a++) ctx->level_radii[l][a] = side * ctx->half_lengths[a];
+    }
+}
+
+double stage_ms(const oct_ctx *ctx, Stage s) {
+    if (!ctx->ev_used[s]) return 0.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0[s], ctx->ev1[s]) != hipSuccess) return 0.0;
+    return (double)ms;
+}
+
+// Slot order that visits queries in SFC order (coherent waves): indices into the sorted cloud are
+// already SFC positions; explicit centres are keyed with the cloud's encoder and box.
+void query_order(oct_ctx *ctx, const double *centers, const uint32_t *idx, size_t nq, std::vector<uint32_t> &order) {
+    order.resize(nq);
+    std::iota(order.begin(), order.end(), 0u);
+    if (idx) {
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return idx[a] < idx[b]; });
+    } else if (centers) {
+        std::vector<uint64_t> k(nq);
+        for (size_t i = 0; i < nq; i++) {
+            uint32_t x = quantize_axis(centers[3 * i], ctx->center[0], ctx->radii[0]);
+            uint32_t y = quantize_axis(centers[3 * i + 1], ctx->center[1], ctx->radii[1]);
+            uint32_t z = quantize_axis(centers[3 * i + 2], ctx->center[2], ctx->radii[2]);
+            k[i] = ctx->encoder == OCT_ENC_HILBERT3D ? hilbert_encode_tab(kHilbert.t, x, y, z) : morton_encode(x, y, z);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return k[a] < k[b]; });
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *oct_version(void) { return "octree-amd 0.1 (gfx950)"; }
+
+int oct_create(oct_ctx **out, int device) {
+    if (!out) return OCT_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return OCT_ERR_HIP;
+    if (device < 0 || device >= ndev) return OCT_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return OCT_ERR_HIP;
+    oct_ctx *ctx = new oct_ctx;
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return OCT_ERR_HIP;
+    }
+    for (int s = 0; s < ST_N; s++) {
+        if (hipEventCreate(&ctx->ev0[s]) != hipSuccess || hipEventCreate(&ctx->ev1[s]) != hipSuccess) {
+            delete ctx;
+            return OCT_ERR_HIP;
+        }
+    }
+    *out = ctx;
+    return OCT_OK;
+}
+
+int oct_destroy(oct_ctx *ctx) {
+    if (!ctx) return OCT_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->xyz.release();
+    for (auto &b : ctx->keys) b.release();
+    for (auto &b : ctx->vals) b.release();
+    ctx->hist.release(), ctx->lookback.release(), ctx->misc.release();
+    ctx->p64.release(), ctx->p32.release();
+    ctx->nodes.release(), ctx->child.release(), ctx->nkey.release(), ctx->flags.release(), ctx->nrank.release();
+    ctx->amax.release(), ctx->scan.release();
+    ctx->qidx.release(), ctx->qxyz.release(), ctx->qrad.release(), ctx->counts.release(), ctx->qoff.release();
+    ctx->entries.release(), ctx->stats.release(), ctx->kidx.release(), ctx->kd2.release();
+    for (int s = 0; s < ST_N; s++) {
+        if (ctx->ev0[s]) (void)hipEventDestroy(ctx->ev0[s]);
+        if (ctx->ev1[s]) (void)hipEventDestroy(ctx->ev1[s]);
+    }
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return OCT_OK;
+}
+
+const char *oct_last_error(const oct_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int oct_synchronize(oct_ctx *ctx) {
+    if (!ctx) return OCT_ERR_ARG;
+    HCHK(hipStreamSynchronize(ctx->stream), "synchronize");
+    return OCT_OK;
+}
+
+int oct_set_points(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_bytes) {
+    if (!ctx) return OCT_ERR_ARG;
+    HCHK(hipSetDevice(ctx->device), "set device");
+    int r = upload_xyz(ctx, xyz, n, stride_bytes, hipMemcpyHostToDevice);
+    if (r) return r;
+    HCHK(hipStreamSynchronize(ctx->stream), "copy points");
+    ctx->has_points = true;
+    ctx->has_keys = ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
+    return OCT_OK;
+}
+
+int oct_set_points_dev(oct_ctx *ctx, const double *xyz_dev, size_t n, size_t stride_bytes) {
+    if (!ctx) return OCT_ERR_ARG;
+    HCHK(hipSetDevice(ctx->device), "set device");
+    int r = upload_xyz(ctx, xyz_dev, n, stride_bytes, hipMemcpyDeviceToDevice);
+    if (r) return r;
+    ctx->has_points = true;
+    ctx->has_keys = ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
+    return OCT_OK;
+}
+
+int oct_bbox(oct_ctx *ctx, double center[3], double radii[3]) {
+    if (!ctx || !center || !radii) return fail(ctx, OCT_ERR_ARG, "null argument");
+    if (!ctx->has_points) return fail(ctx, OCT_ERR_STATE, "oct_bbox before oct_set_points");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    double mn[3], mx[3];
+    if (ctx->n == 0) {
+        for (int a = 0; a < 3; a++) mn[a] = DBL_MAX, mx[a] = -DBL_MAX;
+    } else {
+        HCHK(launch_bbox(ctx, mn, mx), "bbox");
+    }
+    // mbb: center = midpoint(min, max) = min + (max - min) / 2 (util.hpp:139-143); radii = (max - min) / 2
+    for (int a = 0; a < 3; a++) {
+        center[a] = mn[a] + (mx[a] - mn[a]) / 2;
+        radii[a] = (mx[a] - mn[a]) / 2;
+    }
+    return OCT_OK;
+}
+
+int oct_encode(oct_ctx *ctx, int encoder, const double center[3], const double radii[3], uint64_t *keys_out) {
+    if (!ctx || !center || !radii) return fail(ctx, OCT_ERR_ARG, "null argument");
+    if (!valid_encoder(encoder)) return fail(ctx, OCT_ERR_UNSUPPORTED, "only MortonEncoder3D (1) and HilbertEncoder3D (5) build octrees");
+    if (!ctx->has_points) return fail(ctx, OCT_ERR_STATE, "oct_encode before oct_set_points");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    HCHK(launch_encode(ctx, encoder, center, radii), "encode");
+    ctx->encoder = encoder;
+    for (int a = 0; a < 3; a++) ctx->center[a] = center[a], ctx->radii[a] = radii[a];
+    if (keys_out && ctx->n) HCHK(hipMemcpyAsync(keys_out, ctx->keys[0].p, ctx->n * 8, hipMemcpyDeviceToHost, ctx->stream), "copy keys");
+    HCHK(hipStreamSynchronize(ctx->stream), "encode");
+    ctx->has_keys = true;
+    ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
+    return OCT_OK;
+}
+
+int oct_sort(oct_ctx *ctx, uint32_t *perm_out, uint64_t *keys_out) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (!ctx->has_keys) return fail(ctx, OCT_ERR_STATE, "oct_sort before oct_encode");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    std::string err;
+    hipError_t e = launch_sort(ctx, err);
+    if (e != hipSuccess) return err.empty() ? hfail(ctx, e, "sort") : fail(ctx, OCT_ERR_STATE, err);
+    HCHK(launch_gather(ctx), "gather");
+    if (perm_out && ctx->n) HCHK(hipMemcpyAsync(perm_out, ctx->perm, ctx->n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy perm");
+    if (keys_out && ctx->n)
+        HCHK(hipMemcpyAsync(keys_out, ctx->sorted_keys, ctx->n * 8, hipMemcpyDeviceToHost, ctx->stream), "copy keys");
+    HCHK(hipStreamSynchronize(ctx->stream), "sort");
+    ctx->has_sorted = true;
+    ctx->has_keys = false;  // the unsorted key buffer was consumed
+    ctx->has_tree = ctx->has_lists = false;
+    return OCT_OK;
+}
+
+int oct_sort_points(oct_ctx *ctx, int encoder, double center_out[3], double radii_out[3], uint32_t *perm_out,
+                    uint64_t *keys_out) {
+    double c[3], r[3];
+    int s = oct_bbox(ctx, c, r);
+    if (s) return s;
+    if ((s = oct_encode(ctx, encoder, c, r, nullptr))) return s;
+    if ((s = oct_sort(ctx, perm_out, keys_out))) return s;
+    if (center_out)
+        for (int a = 0; a < 3; a++) center_out[a] = c[a];
+    if (radii_out)
+        for (int a = 0; a < 3; a++) radii_out[a] = r[a];
+    return OCT_OK;
+}
+
+int oct_set_sorted(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_bytes, const uint64_t *codes, int encoder,
+                   const double center[3], const double radii[3]) {
+    if (!ctx || !center || !radii || (n && !codes)) return fail(ctx, OCT_ERR_ARG, "null argument");
+    if (!valid_encoder(encoder)) return fail(ctx, OCT_ERR_UNSUPPORTED, "only the 3-D encoders build octrees");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    for (size_t i = 1; i < n; i++)
+        if (codes[i] < codes[i - 1]) return fail(ctx, OCT_ERR_ARG, "codes must be sorted ascending");
+    int r = upload_xyz(ctx, xyz, n, stride_bytes, hipMemcpyHostToDevice);
+    if (r) return r;
+    HCHK(ctx->keys[0].reserve(n), "alloc keys");
+    HCHK(ctx->p64.reserve(n), "alloc points");
+    if (n) {
+        HCHK(hipMemcpyAsync(ctx->keys[0].p, codes, n * 8, hipMemcpyHostToDevice, ctx->stream), "copy codes");
+        unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+        k_pad<<<nb, 256, 0, ctx->stream>>>(ctx->xyz.p, ctx->p64.p, n);
+        HCHK(hipGetLastError(), "pad points");
+    }
+    HCHK(hipStreamSynchronize(ctx->stream), "set sorted");
+    ctx->sorted_keys = ctx->keys[0].p;
+    ctx->perm = nullptr;
+    ctx->encoder = encoder;
+    for (int a = 0; a < 3; a++) ctx->center[a] = center[a], ctx->radii[a] = radii[a];
+    ctx->has_points = true;
+    ctx->has_keys = false;
+    ctx->has_sorted = true;
+    ctx->has_tree = ctx->has_lists = false;
+    return OCT_OK;
+}
+
+int oct_build(oct_ctx *ctx, uint32_t max_points_leaf, oct_tree_info *info) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (!ctx->has_sorted) return fail(ctx, OCT_ERR_STATE, "oct_build before the cloud is sorted");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    setup_geometry(ctx);
+    std::string err;
+    hipError_t e = launch_build(ctx, max_points_leaf, err);
+    if (e != hipSuccess) return err.empty() ? hfail(ctx, e, "build") : fail(ctx, OCT_ERR_STATE, err);
+    HCHK(hipStreamSynchronize(ctx->stream), "build");
+    ctx->has_tree = true;
+    ctx->has_lists = false;
+    if (info) {
+        info->n_leaf = ctx->n_leaf;
+        info->n_internal = ctx->n_internal;
+        info->n_total = ctx->n_total;
+        info->max_depth_seen = ctx->max_depth_seen;
+        info->max_points_leaf = max_points_leaf;
+        info->n_levels = ctx->n_levels;
+        info->build_ms = stage_ms(ctx, ST_BUILD);
+    }
+    return OCT_OK;
+}
+
+int oct_get_tree(oct_ctx *ctx, uint64_t *prefixes, uint32_t *offsets, uint64_t *ranges, double *centers, double *radii) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (!ctx->has_tree) return fail(ctx, OCT_ERR_STATE, "oct_get_tree before oct_build");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    const size_t nt = ctx->n_total;
+    std::vector<NodeRec> nodes(nt);
+    std::vector<uint64_t> keys(nt);
+    HCHK(hipMemcpyAsync(nodes.data(), ctx->nodes.p, nt * sizeof(NodeRec), hipMemcpyDeviceToHost, ctx->stream), "copy nodes");
+    HCHK(hipMemcpyAsync(keys.data(), ctx->nkey.p, nt * 8, hipMemcpyDeviceToHost, ctx->stream), "copy keys");
+    if (offsets) HCHK(hipMemcpyAsync(offsets, ctx->child.p, nt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy offsets");
+    HCHK(hipStreamSynchronize(ctx->stream), "get tree");
+    for (uint32_t l = 0; l <= kMaxDepth; l++)
+        for (uint32_t u = ctx->level_start[l]; u < ctx->level_start[l + 1] && u < nt; u++) {
+            if (prefixes) prefixes[u] = encode_placeholder(keys[u], l);
+        }
+    for (size_t u = 0; u < nt; u++) {
+        if (ranges) ranges[2 * u] = nodes[u].first, ranges[2 * u + 1] = nodes[u].last;
+        if (centers) centers[3 * u] = nodes[u].cx, centers[3 * u + 1] = nodes[u].cy, centers[3 * u + 2] = nodes[u].cz;
+    }
+    if (radii)
+        for (int l = 0; l < 22; l++)
+            for (int a = 0; a < 3; a++) radii[3 * l + a] = ctx->level_radii[l][a];
+    return OCT_OK;
+}
+
+static int check_search_args(oct_ctx *ctx, int kernel, int algo, int precision) {
+    if (kernel < OCT_KERNEL_CIRCLE || kernel > OCT_KERNEL_CUBE) return fail(ctx, OCT_ERR_ARG, "unknown kernel id");
+    if (algo < OCT_ALGO_NEIGHBORS || algo > OCT_ALGO_STRUCT) return fail(ctx, OCT_ERR_ARG, "unknown search algorithm id");
+    if (precision != OCT_PREC_FP64 && precision != OCT_PREC_FP32_RECHECK) return fail(ctx, OCT_ERR_ARG, "unknown precision");
+    if (!ctx->has_tree) return fail(ctx, OCT_ERR_STATE, "search before oct_build");
+    return OCT_OK;
+}
+
+static int read_stats(oct_ctx *ctx, oct_search_stats *stats, double ms) {
+    unsigned long long h[4];
+    HCHK(hipMemcpyAsync(h, ctx->stats.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "copy stats");
+    HCHK(hipStreamSynchronize(ctx->stream), "stats");
+    stats->nodes_classified = h[0];
+    stats->points_tested = h[1];
+    stats->rechecked = h[2];
+    stats->total_neighbors = h[3];
+    stats->kernel_ms = ms;
+    return OCT_OK;
+}
+
+int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *centers, const uint32_t *center_idx,
+               const double *radii, double radius, size_t nq, int want_lists, uint32_t *counts_out,
+               oct_search_stats *stats) {
+    if (!ctx) return OCT_ERR_ARG;
+    int s = check_search_args(ctx, kernel, algo, precision);
+    if (s) return s;
+    if (nq >= (1ull << 31)) return fail(ctx, OCT_ERR_ARG, "too many queries");
+    if (!centers && !center_idx && nq > ctx->n) return fail(ctx, OCT_ERR_ARG, "all-points search with nq > n");
+    if (center_idx)
+        for (size_t i = 0; i < nq; i++)
+            if (center_idx[i] >= ctx->n) return fail(ctx, OCT_ERR_ARG, "center index out of range");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    std::vector<uint32_t> order;
+    query_order(ctx, centers, center_idx, nq, order);
+    HCHK(ctx->qidx.reserve(2 * nq + 1), "alloc queries");
+    uint32_t *d_order = ctx->qidx.p, *d_idx = ctx->qidx.p + nq;
+    if (nq) HCHK(hipMemcpyAsync(d_order, order.data(), nq * 4, hipMemcpyHostToDevice, ctx->stream), "copy order");
+    if (center_idx && nq) HCHK(hipMemcpyAsync(d_idx, center_idx, nq * 4, hipMemcpyHostToDevice, ctx->stream), "copy idx");
+    if (centers) {
+        HCHK(ctx->qxyz.reserve(3 * nq + 1), "alloc centres");
+        if (nq) HCHK(hipMemcpyAsync(ctx->qxyz.p, centers, nq * 24, hipMemcpyHostToDevice, ctx->stream), "copy centres");
+    }
+    if (radii) {
+        HCHK(ctx->qrad.reserve(nq + 1), "alloc radii");
+        if (nq) HCHK(hipMemcpyAsync(ctx->qrad.p, radii, nq * 8, hipMemcpyHostToDevice, ctx->stream), "copy radii");
+    }
+    HCHK(ctx->counts.reserve(2 * nq + 1), "alloc counts");
+    HCHK(ctx->stats.reserve(8), "alloc stats");
+    HCHK(hipMemsetAsync(ctx->stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream), "clear stats");
+    RadiusLaunch L;
+    L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0;
+    L.order = d_order;
+    L.qidx = center_idx ? d_idx : nullptr;
+    L.qxyz = centers ? ctx->qxyz.p : nullptr;
+    L.qrad = radii ? ctx->qrad.p : nullptr;
+    L.radius = radius;
+    L.nq = nq;
+    L.counts = ctx->counts.p;
+    L.ecounts = ctx->counts.p + nq;
+    L.stats = ctx->stats.p;
+    HCHK(launch_radius(ctx, L), "radius count");
+    double ms = 0.0;
+    HCHK(hipStreamSynchronize(ctx->stream), "radius count");
+    ms += stage_ms(ctx, ST_SEARCH);
+    ctx->has_lists = false;
+    if (want_lists) {
+        HCHK(ctx->qoff.reserve(nq + 1), "alloc offsets");
+        HCHK(exclusive_scan_u32_to_u64(ctx, ctx->counts.p + nq, ctx->qoff.p, nq), "scan");
+        uint64_t total = 0;
+        HCHK(hipMemcpyAsync(&total, ctx->qoff.p + nq, 8, hipMemcpyDeviceToHost, ctx->stream), "copy total");
+        HCHK(hipStreamSynchronize(ctx->stream), "scan");
+        const size_t words = (size_t)total * (algo == OCT_ALGO_STRUCT ? 2 : 1);
+        HCHK(ctx->entries.reserve(words + 1), "alloc lists");
+        L.mode = 1;
+        L.offsets = ctx->qoff.p;
+        L.entries = ctx->entries.p;
+        L.stats = nullptr;
+        HCHK(launch_radius(ctx, L), "radius fill");
+        HCHK(hipStreamSynchronize(ctx->stream), "radius fill");
+        ms += stage_ms(ctx, ST_SEARCH);
+        ctx->has_lists = true;
+    }
+    ctx->last_nq = nq;
+    ctx->last_algo = algo;
+    if (counts_out && nq) HCHK(hipMemcpyAsync(counts_out, ctx->counts.p, nq * 4, hipMemcpyDeviceToHost, ctx->stream), "copy counts");
+    HCHK(hipStreamSynchronize(ctx->stream), "radius");
+    if (stats) return read_stats(ctx, stats, ms);
+    return OCT_OK;
+}
+
+int oct_radius_fetch(oct_ctx *ctx, uint64_t *offsets, uint32_t *entries, size_t entries_capacity) {
+    if (!ctx || !offsets) return fail(ctx, OCT_ERR_ARG, "null argument");
+    if (!ctx->has_lists) return fail(ctx, OCT_ERR_STATE, "no result lists (call oct_radius with want_lists = 1)");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    const size_t nq = ctx->last_nq;
+    HCHK(hipMemcpyAsync(offsets, ctx->qoff.p, (nq + 1) * 8, hipMemcpyDeviceToHost, ctx->stream), "copy offsets");
+    HCHK(hipStreamSynchronize(ctx->stream), "fetch");
+    const size_t words = (size_t)offsets[nq] * (ctx->last_algo == OCT_ALGO_STRUCT ? 2 : 1);
+    if (words && entries) {  // entries == NULL: size query (offsets only)
+        if (entries_capacity < words) return fail(ctx, OCT_ERR_ARG, "entries buffer too small");
+        HCHK(hipMemcpyAsync(entries, ctx->entries.p, words * 4, hipMemcpyDeviceToHost, ctx->stream), "copy lists");
+        HCHK(hipStreamSynchronize(ctx->stream), "fetch");
+    }
+    return OCT_OK;
+}
+
+int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint32_t *center_idx_dev, double radius,
+                   size_t nq, uint32_t *counts_dev, oct_search_stats *stats) {
+    if (!ctx) return OCT_ERR_ARG;
+    int s = check_search_args(ctx, kernel, algo, precision);
+    if (s) return s;
+    if (!center_idx_dev && nq > ctx->n) return fail(ctx, OCT_ERR_ARG, "all-points search with nq > n");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    if (!counts_dev) {
+        HCHK(ctx->counts.reserve(2 * nq + 1), "alloc counts");
+        counts_dev = ctx->counts.p;
+    }
+    RadiusLaunch L;
+    L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0;
+    L.qidx = center_idx_dev;
+    L.radius = radius;
+    L.nq = nq;
+    L.counts = counts_dev;
+    if (stats) {
+        HCHK(ctx->stats.reserve(8), "alloc stats");
+        HCHK(hipMemsetAsync(ctx->stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream), "clear stats");
+        L.stats = ctx->stats.p;
+    }
+    HCHK(launch_radius(ctx, L), "radius");
+    ctx->has_lists = false;
+    if (stats) {
+        HCHK(hipStreamSynchronize(ctx->stream), "radius");
+        return read_stats(ctx, stats, stage_ms(ctx, ST_SEARCH));
+    }
+    return OCT_OK;
+}
+
+int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *center_idx, size_t nq, uint32_t k,
+            int32_t *idx_out, double *d2_out, oct_search_stats *stats) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (precision != OCT_PREC_FP64 && precision != OCT_PREC_FP32_RECHECK) return fail(ctx, OCT_ERR_ARG, "unknown precision");
+    if (!ctx->has_tree) return fail(ctx, OCT_ERR_STATE, "knn before oct_build");
+    if (k == 0) return OCT_OK;
+    if (center_idx)
+        for (size_t i = 0; i < nq; i++)
+            if (center_idx[i] >= ctx->n) return fail(ctx, OCT_ERR_ARG, "center index out of range");
+    if (!centers && !center_idx && nq > ctx->n) return fail(ctx, OCT_ERR_ARG, "all-points kNN with nq > n");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    std::vector<uint32_t> order;
+    query_order(ctx, centers, center_idx, nq, order);
+    HCHK(ctx->qidx.reserve(2 * nq + 1), "alloc queries");
+    uint32_t *d_order = ctx->qidx.p, *d_idx = ctx->qidx.p + nq;
+    if (nq) HCHK(hipMemcpyAsync(d_order, order.data(), nq * 4, hipMemcpyHostToDevice, ctx->stream), "copy order");
+    if (center_idx && nq) HCHK(hipMemcpyAsync(d_idx, center_idx, nq * 4, hipMemcpyHostToDevice, ctx->stream), "copy idx");
+    if (centers) {
+        HCHK(ctx->qxyz.reserve(3 * nq + 1), "alloc centres");
+        if (nq) HCHK(hipMemcpyAsync(ctx->qxyz.p, centers, nq * 24, hipMemcpyHostToDevice, ctx->stream), "copy centres");
+    }
+    HCHK(ctx->kidx.reserve((size_t)nq * k + 1), "alloc knn");
+    HCHK(ctx->kd2.reserve((size_t)nq * k + 1), "alloc knn");
+    HCHK(ctx->stats.reserve(8), "alloc stats");
+    HCHK(hipMemsetAsync(ctx->stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream), "clear stats");
+    std::string err;
+    hipError_t e = launch_knn(ctx, precision, center_idx ? d_idx : nullptr, centers ? ctx->qxyz.p : nullptr, nq, k,
+                              ctx->kidx.p, ctx->kd2.p, ctx->stats.p, err);
+    if (e != hipSuccess) return err.empty() ? hfail(ctx, e, "knn") : fail(ctx, OCT_ERR_STATE, err);
+    (void)d_order;
+    if (idx_out && nq) HCHK(hipMemcpyAsync(idx_out, ctx->kidx.p, nq * k * 4, hipMemcpyDeviceToHost, ctx->stream), "copy idx");
+    if (d2_out && nq) HCHK(hipMemcpyAsync(d2_out, ctx->kd2.p, nq * k * 8, hipMemcpyDeviceToHost, ctx->stream), "copy d2");
+    HCHK(hipStreamSynchronize(ctx->stream), "knn");
+    if (stats) return read_stats(ctx, stats, stage_ms(ctx, ST_KNN));
+    return OCT_OK;
+}
+
+int oct_stage_times(const oct_ctx *ctx, double ms[8]) {
+    if (!ctx || !ms) return OCT_ERR_ARG;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return OCT_ERR_HIP;
+    for (int s = 0; s < 8; s++) ms[s] = s < ST_N ? stage_ms(ctx, (Stage)s) : 0.0;
+    return OCT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+// H1 + H2: bounding box reduction and SFC encoding (gfx950).
+//
+// k_bbox:   FP64 min/max per axis (box.hpp:94-137); 24 B/pt streamed once.
+// k_encode: per point FP64 quantisation (point_encoder.hpp:37-49) + Morton / Hilbert key, and the
+//           digit histograms of all radix passes in the same read (so the sort needs no separate
+//           histogram pass). 24 B/pt read + 8 B/pt written.
+#include <cfloat>
+
+#include "device.hpp"
+
+namespace oct {
+
+constexpr int kBboxBlock = 256;
+constexpr int kBboxBlocks = 1024;
+
+__global__ __launch_bounds__(kBboxBlock) void k_bbox(const double *__restrict__ xyz, size_t n,
+                                                     double *__restrict__ partial) {
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        for (int a = 0; a < 3; a++) {
+            double v = xyz[3 * i + a];
+            mn[a] = (v < mn[a]) ? v : mn[a];  // std::min semantics (NaN never replaces the running value)
+            mx[a] = (mx[a] < v) ? v : mx[a];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int a = 0; a < 3; a++) {
+            double o = __shfl_xor(mn[a], off);
+            mn[a] = (o < mn[a]) ? o : mn[a];
+            o = __shfl_xor(mx[a], off);
+            mx[a] = (mx[a] < o) ? o : mx[a];
+        }
+    }
+    __shared__ double s[kBboxBlock / 64][6];
+    int w = threadIdx.x / 64;
+    if (lane_id() == 0)
+        for (int a = 0; a < 3; a++) s[w][a] = mn[a], s[w][3 + a] = mx[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 1; j < kBboxBlock / 64; j++)
+            for (int a = 0; a < 3; a++) {
+                s[0][a] = (s[j][a] < s[0][a]) ? s[j][a] : s[0][a];
+                s[0][3 + a] = (s[0][3 + a] < s[j][3 + a]) ? s[j][3 + a] : s[0][3 + a];
+            }
+        for (int k = 0; k < 6; k++) partial[(size_t)blockIdx.x * 6 + k] = s[0][k];
+    }
+}
+
+__global__ __launch_bounds__(kBboxBlock) void k_bbox_final(const double *__restrict__ partial, int nb,
+                                                           double *__restrict__ out) {
+    __shared__ double s[kBboxBlock][6];
+    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int a = 0; a < 3; a++) {
+            double lo = partial[(size_t)b * 6 + a], hi = partial[(size_t)b * 6 + 3 + a];
+            v[a] = (lo < v[a]) ? lo : v[a];
+            v[3 + a] = (v[3 + a] < hi) ? hi : v[3 + a];
+        }
+    for (int k = 0; k < 6; k++) s[threadIdx.x][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int t = 1; t < (int)blockDim.x; t++)
+            for (int a = 0; a < 3; a++) {
+                v[a] = (s[t][a] < v[a]) ? s[t][a] : v[a];
+                v[3 + a] = (v[3 + a] < s[t][3 + a]) ? s[t][3 + a] : v[3 + a];
+            }
+        for (int k = 0; k < 6; k++) out[k] = v[k];
+    }
+}
+
+hipError_t launch_bbox(oct_ctx *ctx, double mn[3], double mx[3]) {
+    hipError_t e = ctx->misc.reserve(kBboxBlocks * 12 + 64);
+    if (e != hipSuccess) return e;
+    double *partial = reinterpret_cast<double *>(ctx->misc.p);
+    double *out = partial + (size_t)kBboxBlocks * 6;
+    int nb = (int)((ctx->n + kBboxBlock - 1) / kBboxBlock);
+    nb = nb < 1 ? 1 : (nb > kBboxBlocks ? kBboxBlocks : nb);
+    stage_begin(ctx, ST_BBOX);
+    k_bbox<<<nb, kBboxBlock, 0, ctx->stream>>>(ctx->xyz.p, ctx->n, partial);
+    k_bbox_final<<<1, kBboxBlock, 0, ctx->stream>>>(partial, nb, out);
+    stage_end(ctx, ST_BBOX);
+    double h[6];
+    e = hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    for (int a = 0; a < 3; a++) mn[a] = h[a], mx[a] = h[3 + a];
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ encode + digit histograms
+constexpr int kEncBlock = 256;
+constexpr int kRadixPasses = 8;
+
+struct EncodeBox {
+    double c[3], r[3];
+};
+
+template <int ENC>
+__global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__ xyz, size_t n, EncodeBox box,
+                                                      uint64_t *__restrict__ keys, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t sh[kRadixPasses * 256];
+    __shared__ uint16_t stab[48 * 8];
+    for (int i = threadIdx.x; i < kRadixPasses * 256; i += blockDim.x) sh[i] = 0;
+    if (ENC == OCT_ENC_HILBERT3D)
+        for (int i = threadIdx.x; i < 48 * 8; i += blockDim.x) stab[i] = kHilbert.t[i];
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t x = quantize_axis(xyz[3 * i + 0], box.c[0], box.r[0]);
+        uint32_t y = quantize_axis(xyz[3 * i + 1], box.c[1], box.r[1]);
+        uint32_t z = quantize_axis(xyz[3 * i + 2], box.c[2], box.r[2]);
+        uint64_t k = (ENC == OCT_ENC_HILBERT3D) ? hilbert_encode_tab(stab, x, y, z) : morton_encode(x, y, z);
+        keys[i] = k;
+#pragma unroll
+        for (int p = 0; p < kRadixPasses; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kRadixPasses * 256; i += blockDim.x)
+        if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
+hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], const double radii[3]) {
+    size_t n = ctx->n;
+    hipError_t e;
+    if ((e = ctx->keys[0].reserve(n)) != hipSuccess) return e;
+    if ((e = ctx->hist.reserve(kRadixPasses * 256)) != hipSuccess) return e;
+    EncodeBox box;
+    for (int a = 0; a < 3; a++) box.c[a] = center[a], box.r[a] = radii[a];
+    stage_begin(ctx, ST_ENCODE);
+    e = hipMemsetAsync(ctx->hist.p, 0, kRadixPasses * 256 * sizeof(uint32_t), ctx->stream);
+    if (e != hipSuccess) return e;
+    if (n) {
+        int nb = (int)((n + kEncBlock * 16 - 1) / (kEncBlock * 16));
+        nb = nb < 1 ? 1 : (nb > 2048 ? 2048 : nb);
+        if (encoder == OCT_ENC_HILBERT3D)
+            k_encode<OCT_ENC_HILBERT3D><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+        else
+            k_encode<OCT_ENC_MORTON3D><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+    }
+    stage_end(ctx, ST_ENCODE);
+    return hipGetLastError();
+}
+
+}  // namespace oct

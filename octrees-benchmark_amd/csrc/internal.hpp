@@ -1,0 +1,148 @@
+// Internal declarations shared by the gfx950 kernels and the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/octree_amd.h"
+
+namespace oct {
+
+constexpr uint32_t kMaxDepth = 21;                       // morton_encoder_3d.hpp:23
+constexpr uint64_t kUpperBound = 0x8000000000000000ull;  // morton_encoder_3d.hpp:29
+constexpr uint32_t kMaxCoord = (1u << kMaxDepth) - 1u;
+
+// Node record read by the searches: FP64 centre (computeGeometry, linear_octree.hpp:557-565) and
+// the [first, last) point range (internalRanges). 32 bytes, so one aligned scalar load fetches it.
+struct alignas(32) NodeRec {
+    double cx, cy, cz;
+    uint32_t first, last;
+};
+
+// Device buffer that only grows.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t bytes = (n ? n : 1) * sizeof(T);
+        hipError_t e = hipMalloc((void **)&p, bytes);
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+enum Stage { ST_BBOX = 0, ST_ENCODE, ST_SORT, ST_GATHER, ST_BUILD, ST_SEARCH, ST_KNN, ST_N };
+
+}  // namespace oct
+
+struct oct_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // input cloud
+    size_t n = 0;
+    oct::DevBuf<double> xyz;  // n x 3, input order
+    bool has_points = false;
+
+    // encode + sort
+    int encoder = 0;
+    double center[3] = {0, 0, 0}, radii[3] = {0, 0, 0};
+    oct::DevBuf<uint64_t> keys[2];
+    oct::DevBuf<uint32_t> vals[2];
+    oct::DevBuf<uint32_t> hist;      // 8 x 256 digit histograms (+ scratch)
+    oct::DevBuf<uint32_t> lookback;  // per pass, per tile, per bin
+    oct::DevBuf<uint32_t> misc;      // tile counters, flags, scan partials
+    uint64_t *sorted_keys = nullptr; // points into keys[0|1]
+    uint32_t *perm = nullptr;        // points into vals[0|1]
+    bool has_keys = false, has_sorted = false;
+
+    // sorted cloud
+    oct::DevBuf<double4> p64;  // sorted xyz, padded to 32 B
+    oct::DevBuf<float4> p32;   // leaf-recentred FP32 xyz + |p|^2
+
+    // tree (BFS / level order, as the reference's sorted node arrays)
+    bool has_tree = false;
+    uint32_t max_points_leaf = 0;
+    uint32_t n_leaf = 0, n_internal = 0, n_total = 0, max_depth_seen = 0, n_levels = 0;
+    uint32_t level_start[oct::kMaxDepth + 3] = {0};
+    double boxmin[3] = {0, 0, 0};
+    double half_lengths[3] = {0, 0, 0};
+    double level_radii[22][3] = {{0}};
+    oct::DevBuf<oct::NodeRec> nodes;
+    oct::DevBuf<uint32_t> child;    // first child index, 0 = leaf
+    oct::DevBuf<uint64_t> nkey;     // node start key (SFC format)
+    oct::DevBuf<uint32_t> flags;    // per-node split flags (build scratch)
+    oct::DevBuf<uint32_t> nrank;    // per-node rank among split nodes of its level (build scratch)
+    oct::DevBuf<float> amax;        // per-leaf max squared norm of its recentred FP32 points
+    oct::DevBuf<uint32_t> scan;     // scan scratch
+
+    // searches
+    oct::DevBuf<uint32_t> qidx;     // query order / indices
+    oct::DevBuf<double> qxyz;       // explicit query centres
+    oct::DevBuf<double> qrad;       // per-query radii
+    oct::DevBuf<uint32_t> counts;   // per-query counts
+    oct::DevBuf<uint64_t> qoff;     // per-query list offsets
+    oct::DevBuf<uint32_t> entries;  // result lists
+    oct::DevBuf<unsigned long long> stats;  // search counters
+    size_t last_nq = 0;
+    int last_algo = 0;
+    bool has_lists = false;
+
+    // kNN
+    oct::DevBuf<int32_t> kidx;
+    oct::DevBuf<double> kd2;
+
+    hipEvent_t ev0[oct::ST_N] = {}, ev1[oct::ST_N] = {};
+    bool ev_used[oct::ST_N] = {};
+};
+
+namespace oct {
+
+// ---- launchers implemented in the .hip files (all enqueue on ctx->stream) ----
+hipError_t launch_bbox(oct_ctx *ctx, double mn[3], double mx[3]);
+hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], const double radii[3]);
+hipError_t launch_sort(oct_ctx *ctx, std::string &err);
+hipError_t launch_gather(oct_ctx *ctx);
+hipError_t launch_build(oct_ctx *ctx, uint32_t max_points_leaf, std::string &err);
+// One radius-search launch. mode 0: per-query counts (points -> counts, list entries -> ecounts);
+// mode 1: fill the lists at offsets. Query slot s (0..nq-1) processes query id order[s] (or s); the
+// centre is qxyz[id] if given, else the sorted point qidx[id] (or id); outputs are indexed by id.
+struct RadiusLaunch {
+    int kernel = 1, algo = 3, precision = 0, mode = 0;
+    const uint32_t *order = nullptr;
+    const uint32_t *qidx = nullptr;
+    const double *qxyz = nullptr;
+    const double *qrad = nullptr;
+    double radius = 0.0;
+    size_t nq = 0;
+    uint32_t *counts = nullptr;
+    uint32_t *ecounts = nullptr;
+    const uint64_t *offsets = nullptr;
+    uint32_t *entries = nullptr;
+    unsigned long long *stats = nullptr;
+};
+hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L);
+hipError_t launch_knn(oct_ctx *ctx, int precision, const uint32_t *qidx_dev, const double *qxyz_dev, size_t nq,
+                      uint32_t k, int32_t *idx_dev, double *d2_dev, unsigned long long *stats_dev, std::string &err);
+hipError_t exclusive_scan_u32_to_u64(oct_ctx *ctx, const uint32_t *in, uint64_t *out, size_t n);
+hipError_t exclusive_scan_u32(oct_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev);
+
+void stage_begin(oct_ctx *ctx, Stage s);
+void stage_end(oct_ctx *ctx, Stage s);
+
+}  // namespace oct

@@ -1,0 +1,280 @@
+"""Python binding of the gfx950 octree library (liboctree_amd.so) through its C ABI
+(include/octree_amd.h), plus the deterministic synthetic-cloud generators (liboctree_synth.so).
+
+The library must be built in place first (`make -C octrees-benchmark_amd`, or
+`__graft_entry__.build()`); loading fails loudly when it is missing -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboctree_amd.so")
+SYNTH_PATH = os.path.join(HERE, "liboctree_synth.so")
+
+ENC_MORTON, ENC_HILBERT = 1, 5
+K_CIRCLE, K_SPHERE, K_SQUARE, K_CUBE = 0, 1, 2, 3
+A_NEIGHBORS, A_PRUNE, A_STRUCT = 1, 2, 3
+PREC_FP64, PREC_FP32 = 0, 1
+STAGES = ("bbox", "encode", "sort", "gather", "build", "search", "knn")
+
+# every symbol declared in include/octree_amd.h
+EXPORTS = (
+    "oct_create", "oct_destroy", "oct_last_error", "oct_synchronize", "oct_version", "oct_set_points",
+    "oct_set_points_dev", "oct_bbox", "oct_encode", "oct_sort", "oct_sort_points", "oct_set_sorted", "oct_build",
+    "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_stage_times",
+)
+
+
+class OctError(RuntimeError):
+    pass
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("n_leaf", C.c_uint32), ("n_internal", C.c_uint32), ("n_total", C.c_uint32),
+                ("max_depth_seen", C.c_uint32), ("max_points_leaf", C.c_uint32), ("n_levels", C.c_uint32),
+                ("build_ms", C.c_double)]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [("total_neighbors", C.c_uint64), ("points_tested", C.c_uint64), ("nodes_classified", C.c_uint64),
+                ("rechecked", C.c_uint64), ("kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_LIB = None
+_SYNTH = None
+_vp = C.c_void_p
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise OctError(f"{LIB_PATH} is not built (run `make -C octrees-benchmark_amd`); no CPU fallback exists")
+        L = C.CDLL(LIB_PATH)
+        L.oct_create.argtypes = [C.POINTER(_vp), C.c_int]
+        L.oct_destroy.argtypes = [_vp]
+        L.oct_last_error.argtypes = [_vp]
+        L.oct_last_error.restype = C.c_char_p
+        L.oct_version.restype = C.c_char_p
+        L.oct_synchronize.argtypes = [_vp]
+        L.oct_set_points.argtypes = [_vp, _vp, C.c_size_t, C.c_size_t]
+        L.oct_set_points_dev.argtypes = [_vp, _vp, C.c_size_t, C.c_size_t]
+        L.oct_bbox.argtypes = [_vp, _vp, _vp]
+        L.oct_encode.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
+        L.oct_sort.argtypes = [_vp, _vp, _vp]
+        L.oct_sort_points.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp]
+        L.oct_set_sorted.argtypes = [_vp, _vp, C.c_size_t, C.c_size_t, _vp, C.c_int, _vp, _vp]
+        L.oct_build.argtypes = [_vp, C.c_uint32, C.POINTER(TreeInfo)]
+        L.oct_get_tree.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.oct_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_double, C.c_size_t, C.c_int, _vp,
+                                 C.POINTER(SearchStats)]
+        L.oct_radius_fetch.argtypes = [_vp, _vp, _vp, C.c_size_t]
+        L.oct_radius_dev.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_double, C.c_size_t, _vp,
+                                     C.POINTER(SearchStats)]
+        L.oct_knn.argtypes = [_vp, C.c_int, _vp, _vp, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
+        L.oct_stage_times.argtypes = [_vp, _vp]
+        _LIB = L
+    return _LIB
+
+
+def synth():
+    global _SYNTH
+    if _SYNTH is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise OctError(f"{SYNTH_PATH} is not built (run `make -C octrees-benchmark_amd`)")
+        S = C.CDLL(SYNTH_PATH)
+        S.oct_synth_uniform.argtypes = [C.c_uint64, C.c_size_t] + [C.c_double] * 6 + [_vp]
+        S.oct_synth_lidar_tile.argtypes = [C.c_uint64, C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_int,
+                                           C.c_int, C.c_double, C.c_double, _vp]
+        S.oct_synth_indices.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, _vp]
+        _SYNTH = S
+    return _SYNTH
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ------------------------------------------------------------------ workloads
+def uniform_box(n, seed=1, size=(100.0, 100.0, 20.0), offset=(0.0, 0.0, 0.0)):
+    """Uniform cloud in a box on a LAS millimetre grid (the 1 M-point plain workload uses 100x100x20 m)."""
+    xyz = np.empty((n, 3), np.float64)
+    synth().oct_synth_uniform(seed, n, *size, *offset, _ptr(xyz))
+    return xyz
+
+
+def lidar_tile(n, seed=2, extent=1000.0, offset=(651000.0, 6861000.0), n_buildings=None, n_blobs=None,
+               frac_building=0.15, frac_veg=0.20):
+    """Synthetic airborne-LiDAR tile: sinusoid terrain + box buildings + Gaussian vegetation blobs.
+    Defaults scale building/blob counts with the area (2000 buildings and 4000 blobs per km^2)."""
+    area = extent * extent / 1e6
+    nb = int(round(2000 * area)) if n_buildings is None else n_buildings
+    nv = int(round(4000 * area)) if n_blobs is None else n_blobs
+    xyz = np.empty((n, 3), np.float64)
+    synth().oct_synth_lidar_tile(seed, n, extent, offset[0], offset[1], max(nb, 1), max(nv, 1), frac_building,
+                                 frac_veg, _ptr(xyz))
+    return xyz
+
+
+def random_indices(nq, n, seed=42):
+    out = np.empty(nq, np.uint32)
+    synth().oct_synth_indices(seed, nq, n, _ptr(out))
+    return out
+
+
+# ------------------------------------------------------------------ context
+class Octree:
+    """One device context: points, keys, sorted cloud, tree and search buffers live on the GPU."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = _vp()
+        rc = self.L.oct_create(C.byref(h), device)
+        if rc != 0:
+            raise OctError(f"oct_create(device={device}) failed with status {rc}")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oct_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.oct_last_error(self.h)
+            raise OctError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ---- H1-H3
+    def set_points(self, xyz):
+        """xyz: (n, 3) packed points, or (n, 4) rows laid out like the reference's 32-byte Point."""
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        cols = 4 if (xyz.ndim == 2 and xyz.shape[1] == 4) else 3
+        xyz = xyz.reshape(-1, cols)
+        self.n = xyz.shape[0]
+        self._chk(self.L.oct_set_points(self.h, _ptr(xyz), self.n, 8 * cols), "oct_set_points")
+
+    def bbox(self):
+        c, r = np.zeros(3), np.zeros(3)
+        self._chk(self.L.oct_bbox(self.h, _ptr(c), _ptr(r)), "oct_bbox")
+        return c, r
+
+    def encode(self, encoder, center, radii, fetch=True):
+        keys = np.empty(self.n, np.uint64) if fetch else None
+        c, r = np.ascontiguousarray(center, np.float64), np.ascontiguousarray(radii, np.float64)
+        self._chk(self.L.oct_encode(self.h, encoder, _ptr(c), _ptr(r), _ptr(keys)), "oct_encode")
+        return keys
+
+    def sort(self, fetch=True):
+        perm = np.empty(self.n, np.uint32) if fetch else None
+        keys = np.empty(self.n, np.uint64) if fetch else None
+        self._chk(self.L.oct_sort(self.h, _ptr(perm), _ptr(keys)), "oct_sort")
+        return perm, keys
+
+    def sort_points(self, encoder, fetch=True):
+        c, r = np.zeros(3), np.zeros(3)
+        perm = np.empty(self.n, np.uint32) if fetch else None
+        keys = np.empty(self.n, np.uint64) if fetch else None
+        self._chk(self.L.oct_sort_points(self.h, encoder, _ptr(c), _ptr(r), _ptr(perm), _ptr(keys)), "oct_sort_points")
+        return c, r, perm, keys
+
+    def set_sorted(self, xyz, codes, encoder, center, radii):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        codes = np.ascontiguousarray(codes, np.uint64)
+        self.n = xyz.shape[0]
+        c, r = np.ascontiguousarray(center, np.float64), np.ascontiguousarray(radii, np.float64)
+        self._chk(self.L.oct_set_sorted(self.h, _ptr(xyz), self.n, 24, _ptr(codes), encoder, _ptr(c), _ptr(r)),
+                  "oct_set_sorted")
+
+    # ---- H4
+    def build(self, max_points_leaf=128):
+        info = TreeInfo()
+        self._chk(self.L.oct_build(self.h, max_points_leaf, C.byref(info)), "oct_build")
+        self.info = {f: getattr(info, f) for f, _ in info._fields_}
+        return self.info
+
+    def get_tree(self):
+        nt = self.info["n_total"]
+        pre = np.empty(nt, np.uint64)
+        off = np.empty(nt, np.uint32)
+        rng = np.empty((nt, 2), np.uint64)
+        cen = np.empty((nt, 3), np.float64)
+        rad = np.empty((22, 3), np.float64)
+        self._chk(self.L.oct_get_tree(self.h, _ptr(pre), _ptr(off), _ptr(rng), _ptr(cen), _ptr(rad)), "oct_get_tree")
+        return {"prefixes": pre, "offsets": off, "ranges": rng, "centers": cen, "radii": rad}
+
+    # ---- H5
+    def radius(self, kernel, algo, precision, centers=None, idx=None, radii=None, radius=0.0, nq=None, lists=False):
+        if centers is not None:
+            centers = np.ascontiguousarray(centers, np.float64).reshape(-1, 3)
+            nq = centers.shape[0]
+        elif idx is not None:
+            idx = np.ascontiguousarray(idx, np.uint32)
+            nq = idx.shape[0]
+        elif nq is None:
+            nq = self.n
+        if radii is not None:
+            radii = np.ascontiguousarray(np.broadcast_to(np.asarray(radii, np.float64), (nq,)))
+        counts = np.empty(nq, np.uint32)
+        st = SearchStats()
+        self._chk(self.L.oct_radius(self.h, kernel, algo, precision, _ptr(centers), _ptr(idx), _ptr(radii),
+                                    float(radius), nq, 1 if lists else 0, _ptr(counts), C.byref(st)), "oct_radius")
+        if not lists:
+            return counts, st.as_dict()
+        offs = np.empty(nq + 1, np.uint64)
+        self._chk(self.L.oct_radius_fetch(self.h, _ptr(offs), None, 0), "oct_radius_fetch(offsets)")
+        words = int(offs[-1]) * (2 if algo == A_STRUCT else 1)
+        ent = np.empty(max(words, 1), np.uint32)
+        self._chk(self.L.oct_radius_fetch(self.h, _ptr(offs), _ptr(ent), ent.shape[0]), "oct_radius_fetch")
+        out = []
+        for q in range(nq):
+            a, b = int(offs[q]), int(offs[q + 1])
+            if algo == A_STRUCT:
+                out.append(ent[2 * a:2 * b].reshape(-1, 2).astype(np.int64))
+            else:
+                out.append(ent[a:b].astype(np.int64))
+        return counts, out, st.as_dict()
+
+    def radius_dev(self, kernel, algo, precision, radius, nq=None, stats=False):
+        """All-points search on the device-resident sorted cloud; counts stay on the device."""
+        nq = self.n if nq is None else nq
+        st = SearchStats() if stats else None
+        self._chk(self.L.oct_radius_dev(self.h, kernel, algo, precision, None, float(radius), nq, None,
+                                        C.byref(st) if stats else None), "oct_radius_dev")
+        return st.as_dict() if stats else None
+
+    # ---- H6
+    def knn(self, k, precision=PREC_FP64, centers=None, idx=None, nq=None):
+        if centers is not None:
+            centers = np.ascontiguousarray(centers, np.float64).reshape(-1, 3)
+            nq = centers.shape[0]
+        elif idx is not None:
+            idx = np.ascontiguousarray(idx, np.uint32)
+            nq = idx.shape[0]
+        elif nq is None:
+            nq = self.n
+        oi = np.empty((nq, k), np.int32)
+        od = np.empty((nq, k), np.float64)
+        st = SearchStats()
+        self._chk(self.L.oct_knn(self.h, precision, _ptr(centers), _ptr(idx), nq, k, _ptr(oi), _ptr(od), C.byref(st)),
+                  "oct_knn")
+        return oi, od, st.as_dict()
+
+    def synchronize(self):
+        self._chk(self.L.oct_synchronize(self.h), "oct_synchronize")
+
+    def stage_times(self):
+        ms = np.zeros(8)
+        self._chk(self.L.oct_stage_times(self.h, _ptr(ms)), "oct_stage_times")
+        return dict(zip(STAGES, ms[:len(STAGES)].tolist()))

@@ -1,0 +1,124 @@
+/*
+ * oracle.h -- CPU restatement of the reference's SFC encode -> reorder -> linear-octree build ->
+ * radius / kNN search path (derivada/octrees-benchmark).
+ *
+ * THIS IS TEST INFRASTRUCTURE, NOT PRODUCT CODE. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker (or as the timed CPU
+ * baseline). The GPU library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the reference
+ * repository root). Arithmetic follows the reference's operation order with no FP contraction
+ * (build flag -ffp-contract=off), because the reference's x86 Release build (-O2 -mavx2 -mbmi2,
+ * cmake/CMakeConfig.cmake:41,44) has no FMA.
+ *
+ * Parity pinning (what checks each function against something other than itself):
+ *   orc_mbb                : reference KATs (test_geometry.cpp:38-51, test_encoders.cpp:46-50) + closed form
+ *   orc_quantize/encode    : closed form incl. the x86 double->uint64 edge cases (measured with g++)
+ *   orc_morton_*           : vendored libmorton compiled from the reference sources (oracle/_ref),
+ *                            reference KAT test_complex.cpp:96-114
+ *   orc_hilbert_*          : invariants only (bijection, unit-step adjacency, start cell, level-1 table
+ *                            hilbert_encoder_3d.hpp:22); the exact orientation convention is
+ *                            "parity unpinned" beyond these invariants (the reference needs Eigen and
+ *                            PAPI headers that this image lacks, so it cannot be compiled here)
+ *   orc_sort               : closed form (= stable sort by key), reference KAT test_encoders.cpp:36-44
+ *   orc_build              : closed form (canonical "split iff count > maxLeaf" tree, BFS order) and
+ *                            brute-force recursion on small inputs
+ *   orc_radius             : brute force over all points with the same predicate (equal except in
+ *                            rounding-pathological node classifications), reference KATs
+ *                            test_octree.cpp:147-164, test_octree_advanced.cpp:213-295,
+ *                            test_complex.cpp:37-94
+ *   orc_knn / orc_knn_brute: brute-force (d2, index) order; reference KATs test_octree.cpp:126-145,
+ *                            test_octree_advanced.cpp:213-231, test_complex.cpp:144-172
+ */
+#ifndef OCTREE_ORACLE_H
+#define OCTREE_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Encoder ids: values of the reference's EncoderType enum (inc/main_options.hpp:73-83). */
+#define ORC_ENC_MORTON3D 1
+#define ORC_ENC_HILBERT3D 5
+/* Kernel ids: values of the reference's Kernel_t enum (inc/main_options.hpp:97-103). */
+#define ORC_K_CIRCLE 0
+#define ORC_K_SPHERE 1
+#define ORC_K_SQUARE 2
+#define ORC_K_CUBE 3
+/* Search algorithms: values of the reference's SearchAlgo enum (inc/main_options.hpp:29-35). */
+#define ORC_A_NEIGHBORS 1
+#define ORC_A_PRUNE 2
+#define ORC_A_STRUCT 3
+
+/* geometry (inc/geometry/box.hpp:94-137, util.hpp:139-143) */
+void orc_mbb(const double *xyz, size_t n, double center[3], double radii[3]);
+/* quantisation (point_encoder.hpp:37-49) with the x86 conversion semantics */
+uint64_t orc_quantize_axis(double p, double c, double r);
+uint64_t orc_morton_encode(uint64_t x, uint64_t y, uint64_t z);
+void orc_morton_decode(uint64_t key, uint64_t out[3]);
+uint64_t orc_hilbert_encode(uint64_t x, uint64_t y, uint64_t z);
+void orc_hilbert_decode(uint64_t key, uint64_t out[3]);
+/* encodePoints (point_encoder.hpp:54-63) */
+void orc_encode(int enc, const double *xyz, size_t n, const double center[3], const double radii[3],
+                uint64_t *keys);
+/* sortPoints radix passes (point_encoder.hpp:285-383): keys sorted in place, perm[i] = original
+ * index of the point now at position i. nthreads emulates the per-thread histogram layout. */
+void orc_sort(uint64_t *keys, uint32_t *perm, size_t n, int nthreads);
+
+/* ---------------- linear octree (inc/structures/linear_octree.hpp) ---------------- */
+typedef struct orc_tree orc_tree;
+/* Builds from sorted codes and the sortPoints Box(center, radii). Returns NULL on failure. */
+orc_tree *orc_build(const uint64_t *codes, size_t n, uint32_t max_points_leaf, int enc,
+                    const double center[3], const double radii[3]);
+void orc_tree_free(orc_tree *t);
+/* counts: nLeaf, nInternal, nTotal, maxDepthSeen, rounds */
+void orc_tree_info(const orc_tree *t, uint32_t info[5]);
+const uint64_t *orc_tree_leaves(const orc_tree *t);   /* nLeaf+1 */
+const uint64_t *orc_tree_counts(const orc_tree *t);   /* nLeaf */
+const uint64_t *orc_tree_layout(const orc_tree *t);   /* nLeaf+1 */
+const uint64_t *orc_tree_prefixes(const orc_tree *t); /* nTotal */
+const uint32_t *orc_tree_offsets(const orc_tree *t);  /* nTotal+1 */
+const uint64_t *orc_tree_ranges(const orc_tree *t);   /* nTotal x 2 */
+const double *orc_tree_centers(const orc_tree *t);    /* nTotal x 3 */
+const double *orc_tree_radii(const orc_tree *t);      /* 22 x 3 */
+
+/* ---------------- searches ---------------- */
+typedef struct orc_result orc_result;
+/* One search per query. centers: nq x 3 doubles; radii: nq doubles. Output per query in the
+ * reference's own emission order: neighbors/Prune -> point indices; Struct -> [first,last) ranges
+ * (including empty INSIDE ranges, neighbor_set.hpp:84-89). stats (nullable, nq x 2): per query
+ * (points tested with isInside, nodes classified). */
+orc_result *orc_radius(const orc_tree *t, const double *sorted_xyz, int kernel, int algo,
+                       const double *centers, const double *radii, size_t nq, uint64_t *stats);
+size_t orc_result_total(const orc_result *r);        /* total entries (indices, or ranges) */
+const uint64_t *orc_result_offsets(const orc_result *r); /* nq+1 offsets into entries */
+const uint64_t *orc_result_entries(const orc_result *r); /* indices, or 2*ranges interleaved */
+const uint64_t *orc_result_sizes(const orc_result *r);   /* nq: number of neighbour points */
+void orc_result_free(orc_result *r);
+/* Count-only variant (what the reference harness keeps: result.size()); returns sum of sizes. */
+uint64_t orc_radius_count(const orc_tree *t, const double *sorted_xyz, int kernel, int algo,
+                          const double *centers, const double *radii, size_t nq, uint64_t *sizes);
+/* The reference harness' per-query work: build each result, keep its size (timed CPU baseline). */
+uint64_t orc_radius_harness(const orc_tree *t, const double *sorted_xyz, int kernel, int algo,
+                            const double *centers, const double *radii, size_t nq);
+/* Brute force with the same per-point predicate (test aid). */
+orc_result *orc_radius_brute(const double *sorted_xyz, size_t n, int kernel, const double *centers,
+                             const double *radii, size_t nq);
+
+/* kNN best-first (linear_octree.hpp:819-942). idx/d2: nq x k, unused slots = -1 / NaN.
+ * Returns total found. */
+uint64_t orc_knn(const orc_tree *t, const double *sorted_xyz, size_t n, const double *centers,
+                 size_t nq, uint32_t k, int64_t *idx, double *d2);
+/* Brute force k smallest under the (d2, index) order. */
+uint64_t orc_knn_brute(const double *sorted_xyz, size_t n, const double *centers, size_t nq,
+                       uint32_t k, int64_t *idx, double *d2);
+
+int orc_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on identical inputs.
+
+Bar: bit-exact for keys, permutation, tree arrays (including FP64 centres), neighbour lists (same
+entries in the same order as the reference's emission order), kNN indices and FP64 distances.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import octree_amd as A
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_pipeline(xyz, enc, max_leaf):
+    sx, keys, perm, c, r = O.sort_points(enc, xyz)
+    tree = O.Tree(keys, max_leaf, enc, c, r)
+    return sx, keys, perm, c, r, tree
+
+
+def gpu_pipeline(ctx, xyz, enc, max_leaf, tag=None):
+    ctx.set_points(xyz)
+    c, r, perm, keys = ctx.sort_points(enc)
+    info = ctx.build(max_leaf)
+    ctx.current = tag
+    return c, r, perm, keys, info
+
+
+def ensure(ctx, case):
+    """Rebuild the case's GPU tree only if another test replaced it."""
+    name, xyz, enc, ml = case[:4]
+    if getattr(ctx, "current", None) != name:
+        say(f"[gpu] rebuild {name}")
+        gpu_pipeline(ctx, xyz, enc, ml, tag=name)
+
+
+def check_tree(ctx, tree):
+    g = ctx.get_tree()
+    assert ctx.info["n_leaf"] == tree.nLeaf
+    assert ctx.info["n_internal"] == tree.nInternal
+    assert ctx.info["n_total"] == tree.nTotal
+    assert ctx.info["max_depth_seen"] == tree.maxDepthSeen
+    np.testing.assert_array_equal(g["prefixes"], tree.prefixes)
+    np.testing.assert_array_equal(g["offsets"], tree.offsets[:tree.nTotal])
+    np.testing.assert_array_equal(g["ranges"], tree.ranges)
+    assert np.array_equal(g["centers"].view(np.uint64), tree.centers.view(np.uint64)), "centres not bit-exact"
+    assert np.array_equal(g["radii"].view(np.uint64), tree.radii.view(np.uint64))
+
+
+CASES = {
+    "uniform_1M_morton_32": lambda: (A.uniform_box(1_000_000, seed=1), A.ENC_MORTON, 32),
+    "tile_200m_hilbert_32": lambda: (A.lidar_tile(2_000_000, seed=2, extent=200.0), A.ENC_HILBERT, 32),
+    "tile_200m_hilbert_128": lambda: (A.lidar_tile(2_000_000, seed=2, extent=200.0), A.ENC_HILBERT, 128),
+    "tile_100m_morton_128": lambda: (A.lidar_tile(500_000, seed=3, extent=100.0), A.ENC_MORTON, 128),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request, ctx):
+    name = request.param
+    say(f"[case] {name}")
+    xyz, enc, ml = CASES[name]()
+    ref = oracle_pipeline(xyz, enc, ml)
+    say(f"[gpu] pipeline {name} n={len(xyz)}")
+    got = gpu_pipeline(ctx, xyz, enc, ml, tag=name)
+    return name, xyz, enc, ml, ref, got
+
+
+def test_bbox_keys_perm(case, ctx):
+    name, xyz, enc, ml, (sx, keys, perm, c, r, tree), (gc, gr, gperm, gkeys, info) = case
+    assert np.array_equal(gc.view(np.uint64), c.view(np.uint64))
+    assert np.array_equal(gr.view(np.uint64), r.view(np.uint64))
+    np.testing.assert_array_equal(gkeys, keys)
+    np.testing.assert_array_equal(gperm, perm)
+
+
+def test_unsorted_keys(case, ctx):
+    name, xyz, enc, ml, (sx, keys, perm, c, r, tree), _ = case
+    say(f"[gpu] encode {name}")
+    ctx.set_points(xyz)
+    ctx.current = None
+    k = ctx.encode(enc, c, r)
+    np.testing.assert_array_equal(k, O.encode(enc, xyz, c, r))
+
+
+def test_tree(case, ctx):
+    name, xyz, enc, ml, (sx, keys, perm, c, r, tree), _ = case
+    say(f"[gpu] tree {name}")
+    ensure(ctx, case)
+    check_tree(ctx, tree)
+
+
+RADII = {"uniform_1M_morton_32": [2.5], "tile_200m_hilbert_32": [1.0, 2.5, 5.0],
+         "tile_200m_hilbert_128": [2.5], "tile_100m_morton_128": [0.75, 2.5]}
+
+
+@pytest.mark.parametrize("kernel", [A.K_SPHERE, A.K_CIRCLE, A.K_CUBE, A.K_SQUARE])
+@pytest.mark.parametrize("algo", [A.A_NEIGHBORS, A.A_PRUNE, A.A_STRUCT])
+def test_radius_lists(case, ctx, kernel, algo):
+    name, xyz, enc, ml, (sx, keys, perm, c, r, tree), _ = case
+    ensure(ctx, case)
+    n = len(xyz)
+    idx = A.random_indices(4000, n, seed=11)
+    for rad in RADII[name]:
+        ref, ref_sizes = O.radius(tree, sx, kernel, algo, sx[idx.astype(np.int64)], rad)
+        for prec in (A.PREC_FP64, A.PREC_FP32):
+            say(f"[gpu] radius {name} kernel={kernel} algo={algo} r={rad} prec={prec}")
+            counts, lists, st = ctx.radius(kernel, algo, prec, idx=idx, radius=rad, lists=True)
+            say(f"      kernel {st['kernel_ms']:.2f} ms, rechecked {st['rechecked']}")
+            np.testing.assert_array_equal(counts, ref_sizes)
+            for q in range(len(idx)):
+                if not np.array_equal(lists[q], ref[q]):
+                    raise AssertionError(f"query {q} (point {idx[q]}): lists differ\n gpu={lists[q][:20]}\n ref={ref[q][:20]}")
+
+
+@pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32])
+def test_radius_all_points_counts(case, ctx, prec):
+    """All-points counts (the harness' -s all mode) against the oracle on a strided sample."""
+    name, xyz, enc, ml, (sx, keys, perm, c, r, tree), _ = case
+    ensure(ctx, case)
+    n = len(xyz)
+    say(f"[gpu] all-points {name} prec={prec}")
+    counts, st = ctx.radius(A.K_SPHERE, A.A_STRUCT, prec, radius=2.5, nq=n)
+    sample = np.arange(0, n, 97)
+    ref = O.radius_count(tree, sx, O.K_SPHERE, O.A_STRUCT, sx[sample], 2.5)
+    np.testing.assert_array_equal(counts[sample], ref)
+    assert st["total_neighbors"] == int(counts.astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("k", [1, 10, 50, 250, 1000])
+def test_knn_exact(ctx, k):
+    say(f"[gpu] knn k={k}")
+    xyz = A.lidar_tile(60_000, seed=5, extent=40.0)
+    sx, keys, perm, c, r, tree = oracle_pipeline(xyz, A.ENC_HILBERT, 32)
+    gpu_pipeline(ctx, xyz, A.ENC_HILBERT, 32, tag="knn")
+    idx = A.random_indices(500, len(xyz), seed=13)
+    gi, gd, st = ctx.knn(k, idx=idx)
+    bi, bd = O.knn_brute(sx, sx[idx.astype(np.int64)], k)
+    np.testing.assert_array_equal(gi.astype(np.int64), bi)
+    assert np.array_equal(gd.view(np.uint64), bd.view(np.uint64))
+    # the reference's best-first order returns the same squared distances
+    hi, hd = O.knn(tree, sx, sx[idx.astype(np.int64)], k)
+    assert np.array_equal(hd.view(np.uint64), bd.view(np.uint64))
