@@ -67,6 +67,8 @@ typedef struct oct_search_stats {
     uint64_t nodes_classified; /* node predicate evaluations */
     uint64_t rechecked;        /* FP32 decisions re-evaluated in FP64 (points + nodes) */
     double kernel_ms;          /* device time of the search kernel(s) */
+    uint64_t class_slots;      /* wave lane-slots spent on node tests (64 per wave-level test) */
+    uint64_t test_slots;       /* wave lane-slots spent on point tests */
 } oct_search_stats;
 
 /* ---- context -------------------------------------------------------------------------------- */

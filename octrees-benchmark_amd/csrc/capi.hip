@@ -348,7 +348,7 @@ static int check_search_args(oct_ctx *ctx, int kernel, int algo, int precision) 
 }
 
 static int read_stats(oct_ctx *ctx, oct_search_stats *stats, double ms) {
-    unsigned long long h[4];
+    unsigned long long h[6];
     HCHK(hipMemcpyAsync(h, ctx->stats.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "copy stats");
     HCHK(hipStreamSynchronize(ctx->stream), "stats");
     stats->nodes_classified = h[0];
@@ -356,6 +356,8 @@ static int read_stats(oct_ctx *ctx, oct_search_stats *stats, double ms) {
     stats->rechecked = h[2];
     stats->total_neighbors = h[3];
     stats->kernel_ms = ms;
+    stats->class_slots = h[4];
+    stats->test_slots = h[5];
     return OCT_OK;
 }
 
@@ -389,7 +391,7 @@ int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *
     HCHK(ctx->stats.reserve(8), "alloc stats");
     HCHK(hipMemsetAsync(ctx->stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream), "clear stats");
     RadiusLaunch L;
-    L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0;
+    L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = want_lists ? 1 : 0;
     L.order = d_order;
     L.qidx = center_idx ? d_idx : nullptr;
     L.qxyz = centers ? ctx->qxyz.p : nullptr;
@@ -412,7 +414,7 @@ int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *
         HCHK(hipStreamSynchronize(ctx->stream), "scan");
         const size_t words = (size_t)total * (algo == OCT_ALGO_STRUCT ? 2 : 1);
         HCHK(ctx->entries.reserve(words + 1), "alloc lists");
-        L.mode = 1;
+        L.mode = 2;
         L.offsets = ctx->qoff.p;
         L.entries = ctx->entries.p;
         L.stats = nullptr;

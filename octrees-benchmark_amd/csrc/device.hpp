@@ -175,12 +175,14 @@ __device__ inline uint64_t uniform_u64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Device-scope acquire/release accesses for cross-workgroup flags (visible across XCDs).
-__device__ inline uint32_t load_acquire(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Device-scope relaxed atomic accesses for self-contained flag words (flag and value in one 32-bit
+// word, nothing else published through them). They are coherent across the XCDs' L2s without the
+// whole-L2 writeback / invalidate that release / acquire ordering adds to every access.
+__device__ inline uint32_t load_flag(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void store_release(uint32_t *p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline void store_flag(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace oct

@@ -119,8 +119,8 @@ hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], cons
 hipError_t launch_sort(oct_ctx *ctx, std::string &err);
 hipError_t launch_gather(oct_ctx *ctx);
 hipError_t launch_build(oct_ctx *ctx, uint32_t max_points_leaf, std::string &err);
-// One radius-search launch. mode 0: per-query counts (points -> counts, list entries -> ecounts);
-// mode 1: fill the lists at offsets. Query slot s (0..nq-1) processes query id order[s] (or s); the
+// One radius-search launch. mode 0: per-query neighbour counts; mode 1: counts + list-entry counts
+// (ecounts); mode 2: fill the lists at offsets. Query slot s (0..nq-1) processes query id order[s] (or s); the
 // centre is qxyz[id] if given, else the sorted point qidx[id] (or id); outputs are indexed by id.
 struct RadiusLaunch {
     int kernel = 1, algo = 3, precision = 0, mode = 0;

@@ -13,11 +13,14 @@
 // Per-point tests:
 //  - FP64: the reference's isInside expression (kernel_3d.hpp:127-138, 232-242; kernel_2d.hpp:53-63,
 //    153-162), evaluated as written (no contraction).
-//  - FP32_RECHECK (sphere, circle): leaf-recentred FP32 coordinates a (|a|^2 = w precomputed) and
-//    b = fl32(q - c_leaf); t = w - 2 a.b by three FMAs (circle: a.(a - 2b) in 2-D); decide
-//    inside if t < L, outside if t > H, where [L, H] = thr +- delta, thr = R - |b|^2, and
-//    delta = 2^-20 (amax + |b|^2 + R)(1 + 2^-10) bounds every rounding of both this evaluation and
-//    the reference's FP64 one. Points inside the band are re-evaluated with the FP64 expression.
+//  - FP32_RECHECK (sphere, circle): leaf-recentred FP32 coordinates a (w = |a|^2 precomputed) and
+//    b = fl32(q - c_leaf); M = fl32(R - |b|^2); u = fma(ax, -2bx, fma(ay, -2by, fma(az, -2bz, w - M)))
+//    (circle: fma(ax, ax - 2bx, fma(ay, ay - 2by, -M))) approximates |p - q|^2 - R. Every rounding of
+//    this evaluation and of the reference's FP64 one is bounded by
+//    hw = 2^-20 (amax + |b|^2 + R)(1 + 2^-10) + |M - (R - |b|^2)|, amax = max |a|^2 over the leaf
+//    (u32 = 2^-24: |error| <= u32 (13.01 amax + 12.01 |b|^2 + 4 R) + |M - thr| + O(u64)), so
+//    u < -hw is inside, u > hw outside, and points with |u| <= hw are re-evaluated with the FP64
+//    expression. In count mode a lane re-counts a whole leaf in FP64 if any point fell in the band.
 //    Cube/square tests are comparisons only and run on the FP64 coordinates in both modes.
 #include "device.hpp"
 
@@ -120,7 +123,7 @@ __device__ inline bool inside64(const Query &q, double px, double py, double pz)
     }
 }
 
-// Per-lane output sink.
+// Per-lane output sink. MODE 0: neighbour counts only; 1: counts + list-entry counts; 2: fill lists.
 template <int ALGO, int MODE>
 struct Sink {
     uint32_t cnt = 0;   // neighbour points
@@ -130,26 +133,26 @@ struct Sink {
 
     __device__ inline void range(uint32_t first, uint32_t last) {  // INSIDE node
         if (ALGO == OCT_ALGO_STRUCT) {                             // NeighborSet::addRange keeps empty ranges
-            if (MODE == 1) {
+            if (MODE == 2) {
                 entries[2 * (base + ecnt)] = first;
                 entries[2 * (base + ecnt) + 1] = last;
             }
             ecnt++;
             cnt += last - first;
         } else {
-            if (MODE == 1)
+            if (MODE == 2)
                 for (uint32_t i = first; i < last; i++) entries[base + ecnt + (i - first)] = i;
             ecnt += last - first;
             cnt += last - first;
         }
     }
     __device__ inline void point(uint32_t i) {  // Prune / neighbors leaf hit
-        if (MODE == 1) entries[base + ecnt] = i;
+        if (MODE == 2) entries[base + ecnt] = i;
         ecnt++;
         cnt++;
     }
     __device__ inline void run(uint32_t first, uint32_t last) {  // Struct leaf run (non-empty)
-        if (MODE == 1) {
+        if (MODE == 2) {
             entries[2 * (base + ecnt)] = first;
             entries[2 * (base + ecnt) + 1] = last;
         }
@@ -194,11 +197,11 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
         q.hix = q.x + q.r, q.hiy = q.y + q.r, q.hiz = q.z + q.r;
     }
     Sink<ALGO, MODE> sink;
-    if (MODE == 1 && valid) {
+    if (MODE == 2 && valid) {
         sink.base = a.offsets[qid];
         sink.entries = a.entries;
     }
-    uint64_t n_class = 0, n_test = 0, n_recheck = 0;
+    uint64_t n_class = 0, n_test = 0, n_recheck = 0, n_cslots = 0, n_tslots = 0;
 
     // ---- classification helper: returns the lane's judgement for node c at depth d
     auto judge = [&](uint64_t m, const NodeRec &nr, uint32_t d) -> int {
@@ -211,54 +214,105 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
     // ---- leaf: lanes in m test the leaf's points (endpointAction, linear_octree.hpp:716-732, 771-780, 804-813)
     auto leaf = [&](uint64_t m, uint32_t c, const NodeRec &nr) {
         const bool act = (m & lanebit) != 0;
-        const uint32_t first = nr.first, last = nr.last;
+        const uint32_t first = uniform_u32(nr.first), last = uniform_u32(nr.last);
         n_test += (uint64_t)__popcll(m) * (last - first);
-        uint32_t rs = first;  // Struct run start
-        if (PREC == OCT_PREC_FP32_RECHECK && (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE)) {
+        n_tslots += 64ull * (last - first);
+        constexpr bool kFp32 = PREC == OCT_PREC_FP32_RECHECK && (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE);
+        // FP32 band setup (per lane, per leaf). u = w - 2 a.b - M approximates |p - q|^2 - R with
+        // |error| <= hw (see the file header), so u < -hw is inside, u > hw outside, else recheck.
+        float M = 0.f, hw = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+        if (kFp32) {
             const float bx = (float)(q.x - nr.cx), by = (float)(q.y - nr.cy), bz = (float)(q.z - nr.cz);
             double b2 = (double)bx * (double)bx + (double)by * (double)by;
             if (KERNEL == OCT_KERNEL_SPHERE) b2 += (double)bz * (double)bz;
             const double am = (double)a.amax[c];
             const double thr = q.R - b2;
+            M = (float)thr;
             const double delta = 9.5367431640625e-07 * (am + b2 + q.R) * 1.0009765625 + 1e-30;  // 2^-20 (..)(1 + 2^-10)
-            const float L = __double2float_rd(thr - delta), H = __double2float_ru(thr + delta);
-            const float ux = -2.0f * bx, uy = -2.0f * by, uz = -2.0f * bz;
-            for (uint32_t i = first; i < last; i++) {
-                const float4 p = a.p32[i];
-                float t;
-                if (KERNEL == OCT_KERNEL_SPHERE)
-                    t = __builtin_fmaf(p.x, ux, __builtin_fmaf(p.y, uy, __builtin_fmaf(p.z, uz, p.w)));
-                else
-                    t = __builtin_fmaf(p.x, p.x + ux, p.y * (p.y + uy));
-                bool in = t < L;
-                const bool maybe = !in && !(t > H);
-                if (__builtin_expect(act && maybe, 0)) {
+            hw = __double2float_ru(delta + fabs((double)M - thr));
+            ux = -2.0f * bx, uy = -2.0f * by, uz = -2.0f * bz;
+        }
+        auto u32f = [&](const float4 &p) -> float {
+            if (KERNEL == OCT_KERNEL_SPHERE) return __builtin_fmaf(p.x, ux, __builtin_fmaf(p.y, uy, __builtin_fmaf(p.z, uz, p.w - M)));
+            return __builtin_fmaf(p.x, p.x + ux, __builtin_fmaf(p.y, p.y + uy, -M));
+        };
+        if (MODE == 0) {
+            // count only: no per-point control flow. cin counts t < L; mn tracks min |t - mid| over
+            // the leaf, so a lane has a point inside the band [L, H] iff mn <= hw.
+            uint32_t cin = 0;
+            if (kFp32) {
+                const float nhw = -hw;
+                float mn = __builtin_huge_valf();
+                uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // one counter per unroll slot: compare + carry-add
+                uint32_t i = first;
+                for (; i + 4 <= last; i += 4) {
+                    const float4 p0 = a.p32[i], p1 = a.p32[i + 1], p2 = a.p32[i + 2], p3 = a.p32[i + 3];
+                    const float u0 = u32f(p0), u1 = u32f(p1), u2 = u32f(p2), u3 = u32f(p3);
+                    c0 += (u0 < nhw) ? 1u : 0u;
+                    c1 += (u1 < nhw) ? 1u : 0u;
+                    c2 += (u2 < nhw) ? 1u : 0u;
+                    c3 += (u3 < nhw) ? 1u : 0u;
+                    mn = fminf(mn, fminf(fabsf(u0), fabsf(u1)));
+                    mn = fminf(mn, fminf(fabsf(u2), fabsf(u3)));
+                }
+                for (; i < last; i++) {
+                    const float u = u32f(a.p32[i]);
+                    c0 += (u < nhw) ? 1u : 0u;
+                    mn = fminf(mn, fabsf(u));
+                }
+                cin = (c0 + c1) + (c2 + c3);
+                const bool unc = act && !(mn > hw);
+                if (ballot64(unc)) {  // rare: re-count this leaf exactly for the lanes with a band hit
+                    if (unc) {
+                        n_recheck += last - first;
+                        cin = 0;
+                        for (uint32_t j = first; j < last; j++) {
+                            const double4 pd = a.p64[j];
+                            cin += inside64<KERNEL>(q, pd.x, pd.y, pd.z);
+                        }
+                    }
+                }
+            } else {
+                uint32_t i = first;
+                for (; i + 4 <= last; i += 4) {
+                    const double4 p0 = a.p64[i], p1 = a.p64[i + 1], p2 = a.p64[i + 2], p3 = a.p64[i + 3];
+                    cin += inside64<KERNEL>(q, p0.x, p0.y, p0.z);
+                    cin += inside64<KERNEL>(q, p1.x, p1.y, p1.z);
+                    cin += inside64<KERNEL>(q, p2.x, p2.y, p2.z);
+                    cin += inside64<KERNEL>(q, p3.x, p3.y, p3.z);
+                }
+                for (; i < last; i++) {
+                    const double4 p = a.p64[i];
+                    cin += inside64<KERNEL>(q, p.x, p.y, p.z);
+                }
+            }
+            if (act) sink.cnt += cin;
+            return;
+        }
+        // list modes: per-point decisions in the reference's order
+        uint32_t rs = first;  // Struct run start
+        for (uint32_t i = first; i < last; i++) {
+            bool in;
+            if (kFp32) {
+                const float u = u32f(a.p32[i]);
+                in = u < -hw;
+                if (__builtin_expect(act && !in && u <= hw, 0)) {
                     const double4 pd = a.p64[i];
                     in = inside64<KERNEL>(q, pd.x, pd.y, pd.z);
                     n_recheck++;
                 }
-                in = in && act;
-                if (ALGO == OCT_ALGO_STRUCT) {
-                    if (act && !in) {
-                        if (rs < i) sink.run(rs, i);
-                        rs = i + 1;
-                    }
-                } else if (in) {
-                    sink.point(i);
-                }
-            }
-        } else {
-            for (uint32_t i = first; i < last; i++) {
+            } else {
                 const double4 p = a.p64[i];
-                const bool in = act && inside64<KERNEL>(q, p.x, p.y, p.z);
-                if (ALGO == OCT_ALGO_STRUCT) {
-                    if (act && !in) {
-                        if (rs < i) sink.run(rs, i);
-                        rs = i + 1;
-                    }
-                } else if (in) {
-                    sink.point(i);
+                in = inside64<KERNEL>(q, p.x, p.y, p.z);
+            }
+            in = in && act;
+            if (ALGO == OCT_ALGO_STRUCT) {
+                if (act && !in) {
+                    if (rs < i) sink.run(rs, i);
+                    rs = i + 1;
                 }
+            } else if (in) {
+                sink.point(i);
             }
         }
         if (ALGO == OCT_ALGO_STRUCT && act && rs < last) sink.run(rs, last);
@@ -270,6 +324,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
         const NodeRec nr = a.nodes[0];
         const uint64_t m0 = ballot64(valid);
         n_class += __popcll(m0);
+        n_cslots += 64;
         const int j = judge(m0, nr, 0);
         if (j == J_INSIDE) sink.range(nr.first, nr.last);
         const uint64_t ov = ballot64(j == J_OVERLAP);
@@ -290,10 +345,14 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
         const uint64_t m = uniform_u64(s_mask[w][sp]);
         const uint32_t d = uniform_u32(s_depth[w][sp]) + 1;
         const uint32_t first_child = uniform_u32(a.child[node]);
-        n_class += 8ull * __popcll(m);
         for (uint32_t oct = 0; oct < 8; oct++) {
             const uint32_t c = first_child + oct;
             const NodeRec nr = a.nodes[c];
+            // An empty child cannot change counts or index lists whatever its class; only Struct
+            // range lists keep empty INSIDE ranges (neighbor_set.hpp:84-89).
+            if (!(ALGO == OCT_ALGO_STRUCT && MODE != 0) && uniform_u32(nr.first) == uniform_u32(nr.last)) continue;
+            n_class += __popcll(m);
+            n_cslots += 64;
             const int j = judge(m, nr, d);
             if (j == J_INSIDE) sink.range(nr.first, nr.last);
             const uint64_t ov = ballot64(j == J_OVERLAP);
@@ -310,14 +369,16 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
     }
 
     if (valid) {
-        if (MODE == 0) {
+        if (MODE != 2) {
             if (a.counts) a.counts[qid] = sink.cnt;
-            if (a.ecounts) a.ecounts[qid] = sink.ecnt;
+            if (MODE == 1 && a.ecounts) a.ecounts[qid] = sink.ecnt;
         }
     }
     if (a.stats && lane == 0) {
         atomicAdd(&a.stats[0], (unsigned long long)n_class);
         atomicAdd(&a.stats[1], (unsigned long long)n_test);
+        atomicAdd(&a.stats[4], (unsigned long long)n_cslots);
+        atomicAdd(&a.stats[5], (unsigned long long)n_tslots);
     }
     if (a.stats && PREC == OCT_PREC_FP32_RECHECK) {
         // n_recheck is per lane
@@ -336,8 +397,10 @@ template <int KERNEL, int ALGO, int PREC>
 static void launch_k(const SearchArgs &a, int mode, unsigned nb, hipStream_t s) {
     if (mode == 0)
         k_radius<KERNEL, ALGO, PREC, 0><<<nb, kSearchBlock, 0, s>>>(a);
-    else
+    else if (mode == 1)
         k_radius<KERNEL, ALGO, PREC, 1><<<nb, kSearchBlock, 0, s>>>(a);
+    else
+        k_radius<KERNEL, ALGO, PREC, 2><<<nb, kSearchBlock, 0, s>>>(a);
 }
 
 template <int KERNEL, int ALGO>

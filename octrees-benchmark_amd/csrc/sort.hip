@@ -95,9 +95,9 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
     }
     uint32_t *lb = lookback + (size_t)tile * 256;
     if (tile == 0)
-        store_release(&lb[b], kFlagInc | tile_count);
+        store_flag(&lb[b], kFlagInc | tile_count);
     else
-        store_release(&lb[b], kFlagAgg | tile_count);
+        store_flag(&lb[b], kFlagAgg | tile_count);
 
     // local exclusive scan of the tile counts over bins
     uint32_t incl = wave_incl_scan(tile_count);
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
         int64_t t = (int64_t)tile - 1;
         uint32_t spins = 0;
         while (t >= 0) {
-            uint32_t x = load_acquire(&lookback[(size_t)t * 256 + b]);
+            uint32_t x = load_flag(&lookback[(size_t)t * 256 + b]);
             if ((x >> 30) == 0) {
                 if (++spins > kSpinLimit) {
                     atomicOr(err, 1u);
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
             if ((x >> 30) == 2) break;
             t--;
         }
-        store_release(&lb[b], kFlagInc | ((excl + tile_count) & kValMask));
+        store_flag(&lb[b], kFlagInc | ((excl + tile_count) & kValMask));
     }
     s_global[b] = bin_base[b] + excl;
     __syncthreads();

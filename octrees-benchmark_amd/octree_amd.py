@@ -39,7 +39,8 @@ class TreeInfo(C.Structure):
 
 class SearchStats(C.Structure):
     _fields_ = [("total_neighbors", C.c_uint64), ("points_tested", C.c_uint64), ("nodes_classified", C.c_uint64),
-                ("rechecked", C.c_uint64), ("kernel_ms", C.c_double)]
+                ("rechecked", C.c_uint64), ("kernel_ms", C.c_double), ("class_slots", C.c_uint64),
+                ("test_slots", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
