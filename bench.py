@@ -85,7 +85,15 @@ def main():
 
     K, W = args.steps, args.warmup
     results = {}
-    for prec in (A.PREC_FP32, A.PREC_FP64):
+    tuned = {}
+    # FP32 point tests on the vector ALUs (for comparison), then on the matrix cores (the measured
+    # configuration), then plain FP64. Switching the point-test unit clears the tuned tile shapes.
+    variants = [("fp32_valu", A.PREC_FP32, False), ("fp32", A.PREC_FP32, True), ("fp64", A.PREC_FP64, True)]
+    for name, prec, mx in variants:
+        ctx.set_mfma(mx)
+        best, cand = ctx.autotune_radius(A.K_SPHERE, A.A_STRUCT, prec, args.radius, sample_nq=min(n, 1 << 21))
+        tuned[name] = {"queries_per_wave": best, "candidate_ms": cand}
+        log(f"[bench] autotune {name}: best G={best} candidates(ms)={cand}")
         for _ in range(W):
             ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, prec, args.radius)
         sync_all(ctx)
@@ -97,16 +105,21 @@ def main():
         sync_all(ctx)
         wall = (time.perf_counter() - t) / K
         stats = ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, prec, args.radius, stats=True)
-        results[prec] = {"wall_s": wall, "kernel_ms": float(np.mean(kms)), "stats": stats}
-        log(f"[bench] prec={prec} step {wall * 1e3:.2f} ms kernel {np.mean(kms):.2f} ms stats {stats}")
+        results[name] = {"wall_s": wall, "kernel_ms": float(np.mean(kms)), "stats": stats}
+        log(f"[bench] {name} step {wall * 1e3:.2f} ms kernel {np.mean(kms):.2f} ms stats {stats}")
 
-    fp32, fp64 = results[A.PREC_FP32], results[A.PREC_FP64]
+    fp32, fp64, fp32v = results["fp32"], results["fp64"], results["fp32_valu"]
     qps = n / fp32["wall_s"]
     verify = {}
     if not args.no_verify:
         c32, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, radius=args.radius, nq=n)
         c64, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP64, radius=args.radius, nq=n)
         verify["fp32_eq_fp64_all_queries"] = bool(np.array_equal(c32, c64))
+        ctx.set_mfma(False)
+        c32v, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, radius=args.radius, nq=n)
+        ctx.set_mfma(True)
+        verify["fp32_valu_eq_fp64_all_queries"] = bool(np.array_equal(c32v, c64))
+        del c32v
         verify["avg_result_size"] = int(c64.astype(np.int64).sum() // max(n, 1))
 
     cpu = None
@@ -177,6 +190,7 @@ def main():
                    "encoder": "hilbert"},
         "fp64_queries_per_s": n / fp64["wall_s"],
         "fp32_vs_fp64_speedup": fp64["wall_s"] / fp32["wall_s"],
+        "fp32_valu_queries_per_s": n / fp32v["wall_s"],
         "recheck_rate": fp32["stats"]["rechecked"] / max(1, fp32["stats"]["points_tested"]),
         "neighbors_per_s": fp32["stats"]["total_neighbors"] / fp32["wall_s"],
         "build_ms": med(build_ms),
@@ -185,8 +199,12 @@ def main():
         "points_encoded_per_s": n / (med(enc_ms) * 1e-3),
         "tree": {k: info[k] for k in ("n_leaf", "n_internal", "n_total", "max_depth_seen")},
         "stages_ms": {"bbox": med(bbox_ms), "encode": med(enc_ms), "sort": med(sort_ms), "gather": med(gather_ms),
-                      "build": med(build_ms), "search_fp32": fp32["kernel_ms"], "search_fp64": fp64["kernel_ms"]},
+                      "build": med(build_ms), "search_fp32": fp32["kernel_ms"], "search_fp32_valu": fp32v["kernel_ms"],
+                      "search_fp64": fp64["kernel_ms"]},
         "verify": verify,
+        "autotune": tuned,
+        "union_efficiency": {"fp32_test_slots_used": fp32["stats"]["points_tested"] / max(1, fp32["stats"]["test_slots"]),
+                             "fp32_class_slots_used": fp32["stats"]["nodes_classified"] / max(1, fp32["stats"]["class_slots"])},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

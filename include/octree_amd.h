@@ -93,6 +93,12 @@ int oct_bbox(oct_ctx *ctx, double center[3], double radii[3]);
  * (hilbert_encoder_3d.hpp:29-66). keys_out (nullable): n keys in input order. */
 int oct_encode(oct_ctx *ctx, int encoder, const double center[3], const double radii[3], uint64_t *keys_out);
 
+/* Key algebra of single coordinates, batched on the device: PointEncoder::encode / decode
+ * (morton_encoder_3d.hpp:35-46; hilbert_encoder_3d.hpp:29-66, 256-289). xyz: n x 3 u32 cell
+ * coordinates (bits above 21 are ignored, as in the reference); keys: n u64. */
+int oct_encode_coords(oct_ctx *ctx, int encoder, const uint32_t *xyz, size_t n, uint64_t *keys);
+int oct_decode_keys(oct_ctx *ctx, int encoder, const uint64_t *keys, size_t n, uint32_t *xyz);
+
 /* ---- H3 reorder: the radix passes of sortPoints (point_encoder.hpp:285-383) -- a stable sort by
  * key. perm_out[i] (nullable) = input index of the point now at sorted position i; keys_out
  * (nullable) = sorted keys. The device keeps the sorted cloud for the build and searches. */
@@ -131,6 +137,14 @@ int oct_get_tree(oct_ctx *ctx, uint64_t *prefixes, uint32_t *offsets, uint64_t *
 int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *centers, const uint32_t *center_idx,
                const double *radii, double radius, size_t nq, int want_lists, uint32_t *counts_out,
                oct_search_stats *stats);
+/* Vector-radii overloads (linear_octree.hpp:1044-1060, 1069-1075, 1082-1087; kernel_factory.hpp:33-41):
+ * cube or square kernels with per-axis half extents, box = centre -/+ radii (kernel_abstract.hpp:39-45).
+ * radii3: nq x 3 per-query half extents, or NULL to use radius3 for every query. Queries and lists
+ * as for oct_radius (the lists are fetched with oct_radius_fetch). Comparisons only: no precision
+ * argument. Other kernels return OCT_ERR_UNSUPPORTED, as the reference's static_assert rejects them. */
+int oct_radius_box(oct_ctx *ctx, int kernel, int algo, const double *centers, const uint32_t *center_idx,
+                   const double *radii3, const double radius3[3], size_t nq, int want_lists, uint32_t *counts_out,
+                   oct_search_stats *stats);
 /* After oct_radius(want_lists=1): offsets (nq+1) and entries (offsets[nq] u32 indices, or 2 u32 per range).
  * entries == NULL fetches the offsets only (size query). */
 int oct_radius_fetch(oct_ctx *ctx, uint64_t *offsets, uint32_t *entries, size_t entries_capacity);
@@ -143,6 +157,22 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
  * ascending. idx_out: nq x k int32 (-1 pads), d2_out: nq x k FP64 squared distances (NaN pads). */
 int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *center_idx, size_t nq, uint32_t k,
             int32_t *idx_out, double *d2_out, oct_search_stats *stats);
+
+/* ---- tuning. OCT_TUNE_RADIUS_GROUP: queries per wave of the count-mode radius kernel (16, 32 or
+ * 64 queries share one wave's traversal, 64 / G lanes serve each query), or 0 for the
+ * one-lane-per-query kernel; value -1 restores "autotuned / default". Applies to every
+ * (kernel, algorithm, precision) unless the autotuner stored a choice for that combination.
+ * oct_autotune_radius: times each candidate on sample_nq queries taken from the sorted cloud (16
+ * equal strided chunks of consecutive SFC points) with the given radius, stores the fastest for
+ * (kernel, algo, precision), returns it in *best and the per-candidate milliseconds in
+ * ms_out[0..3] for G = 0, 16, 32, 64. No counterpart in the reference (its per-query loop has no
+ * tile shape to choose).
+ * OCT_TUNE_RADIUS_MFMA: 1 (default) runs the FP32 sphere count-mode point tests on the matrix
+ * cores (16 points x 16 queries per v_mfma_f32_16x16x4_f32), 0 on the vector ALUs; same results. */
+enum oct_tune_knob { OCT_TUNE_RADIUS_GROUP = 1, OCT_TUNE_RADIUS_MFMA = 2 };
+int oct_set_tuning(oct_ctx *ctx, int knob, int value);
+int oct_autotune_radius(oct_ctx *ctx, int kernel, int algo, int precision, double radius, size_t sample_nq, int *best,
+                        double ms_out[4]);
 
 /* ---- timing: device milliseconds of the stages of the last calls (HIP events on the context's
  * stream): [0] bbox, [1] encode, [2] sort, [3] gather, [4] build, [5] search, [6] knn. */

@@ -93,18 +93,35 @@ __global__ __launch_bounds__(256) void k_recentre(const double4 *__restrict__ p6
     if (child[wave] != 0) return;
     const NodeRec r = nodes[wave];
     double m = 0.0;
+    bool bad = false;  // a non-finite FP32 point (NaN / overflow): the leaf's FP32 tests are never trusted
     for (uint32_t i = r.first + lane; i < r.last; i += 64) {
         double4 p = p64[i];
         float ax = (float)(p.x - r.cx), ay = (float)(p.y - r.cy), az = (float)(p.z - r.cz);
         double w = (double)ax * (double)ax + (double)ay * (double)ay + (double)az * (double)az;
-        p32[i] = make_float4(ax, ay, az, (float)w);
+        const float fw = (float)w;
+        p32[i] = make_float4(ax, ay, az, fw);
+        bad |= !(fw < __builtin_huge_valf());
         m = w > m ? w : m;
     }
     for (int off = 32; off > 0; off >>= 1) {
         double o = __shfl_xor(m, off);
         m = o > m ? o : m;
     }
+    if (__ballot(bad)) m = __builtin_huge_val();
     if (lane == 0) amax[wave] = __double2float_ru(m);
+}
+
+// FP32 node records: centre - box centre, computed in FP64 and rounded once.
+__global__ __launch_bounds__(256) void k_nodes32(const NodeRec *__restrict__ nodes, uint32_t n_total, double ox, double oy,
+                                                double oz, NodeRec32 *__restrict__ out) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_total; u += gridDim.x * blockDim.x) {
+        const NodeRec r = nodes[u];
+        NodeRec32 o;
+        o.cx = (float)(r.cx - ox), o.cy = (float)(r.cy - oy), o.cz = (float)(r.cz - oz);
+        o.first = r.first, o.last = r.last;
+        o.pad[0] = o.pad[1] = o.pad[2] = 0;
+        out[u] = o;
+    }
 }
 
 template <class T>
@@ -194,6 +211,13 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t M, std::string &err) {
     // leaf-recentred FP32 cloud
     if ((e = ctx->p32.reserve(ctx->n)) != hipSuccess) return e;
     if ((e = ctx->amax.reserve(n_total)) != hipSuccess) return e;
+    if ((e = ctx->nodes32.reserve(n_total)) != hipSuccess) return e;
+    {
+        unsigned nb = (unsigned)((n_total + 255) / 256);
+        nb = nb > 4096 ? 4096 : (nb ? nb : 1);
+        k_nodes32<<<nb, 256, 0, ctx->stream>>>(ctx->nodes.p, n_total, ctx->center[0], ctx->center[1], ctx->center[2],
+                                               ctx->nodes32.p);
+    }
     if (n_total) {
         size_t threads = (size_t)n_total * 64;
         unsigned nb = (unsigned)((threads + 255) / 256);

@@ -132,6 +132,9 @@ int oct_create(oct_ctx **out, int device) {
     if (hipSetDevice(device) != hipSuccess) return OCT_ERR_HIP;
     oct_ctx *ctx = new oct_ctx;
     ctx->device = device;
+    for (auto &a : ctx->group_tuned)
+        for (auto &b : a)
+            for (auto &c : b) c = -1;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return OCT_ERR_HIP;
@@ -159,6 +162,7 @@ int oct_destroy(oct_ctx *ctx) {
     ctx->amax.release(), ctx->scan.release();
     ctx->qidx.release(), ctx->qxyz.release(), ctx->qrad.release(), ctx->counts.release(), ctx->qoff.release();
     ctx->entries.release(), ctx->stats.release(), ctx->kidx.release(), ctx->kd2.release();
+    ctx->nodes32.release(), ctx->qrad3.release(), ctx->kc_xyz.release(), ctx->kc_keys.release();
     for (int s = 0; s < ST_N; s++) {
         if (ctx->ev0[s]) (void)hipEventDestroy(ctx->ev0[s]);
         if (ctx->ev1[s]) (void)hipEventDestroy(ctx->ev1[s]);
@@ -347,6 +351,13 @@ static int check_search_args(oct_ctx *ctx, int kernel, int algo, int precision) 
     return OCT_OK;
 }
 
+// Queries per wave for the count-mode kernel: explicit override, else autotuned, else 16.
+static int radius_group(const oct_ctx *ctx, int kernel, int algo, int precision) {
+    if (ctx->group_override >= 0) return ctx->group_override;
+    int t = ctx->group_tuned[kernel & 3][algo & 3][precision & 1];
+    return t >= 0 ? t : 16;
+}
+
 static int read_stats(oct_ctx *ctx, oct_search_stats *stats, double ms) {
     unsigned long long h[6];
     HCHK(hipMemcpyAsync(h, ctx->stats.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "copy stats");
@@ -361,10 +372,9 @@ static int read_stats(oct_ctx *ctx, oct_search_stats *stats, double ms) {
     return OCT_OK;
 }
 
-int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *centers, const uint32_t *center_idx,
-               const double *radii, double radius, size_t nq, int want_lists, uint32_t *counts_out,
-               oct_search_stats *stats) {
-    if (!ctx) return OCT_ERR_ARG;
+static int radius_impl(oct_ctx *ctx, int kernel, int algo, int precision, const double *centers,
+                       const uint32_t *center_idx, const double *radii, double radius, const double *radii3,
+                       const double *radius3, size_t nq, int want_lists, uint32_t *counts_out, oct_search_stats *stats) {
     int s = check_search_args(ctx, kernel, algo, precision);
     if (s) return s;
     if (nq >= (1ull << 31)) return fail(ctx, OCT_ERR_ARG, "too many queries");
@@ -387,16 +397,28 @@ int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *
         HCHK(ctx->qrad.reserve(nq + 1), "alloc radii");
         if (nq) HCHK(hipMemcpyAsync(ctx->qrad.p, radii, nq * 8, hipMemcpyHostToDevice, ctx->stream), "copy radii");
     }
+    if (radii3) {
+        HCHK(ctx->qrad3.reserve(3 * nq + 1), "alloc radii");
+        if (nq) HCHK(hipMemcpyAsync(ctx->qrad3.p, radii3, nq * 24, hipMemcpyHostToDevice, ctx->stream), "copy radii");
+    }
     HCHK(ctx->counts.reserve(2 * nq + 1), "alloc counts");
     HCHK(ctx->stats.reserve(8), "alloc stats");
     HCHK(hipMemsetAsync(ctx->stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream), "clear stats");
     RadiusLaunch L;
     L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = want_lists ? 1 : 0;
+    L.group = radius_group(ctx, kernel, algo, precision);
+    L.mx = ctx->use_mx != 0;
     L.order = d_order;
     L.qidx = center_idx ? d_idx : nullptr;
     L.qxyz = centers ? ctx->qxyz.p : nullptr;
     L.qrad = radii ? ctx->qrad.p : nullptr;
     L.radius = radius;
+    if (radii3 || radius3) {
+        L.use_r3 = true;
+        L.qrad3 = radii3 ? ctx->qrad3.p : nullptr;
+        if (radius3)
+            for (int k = 0; k < 3; k++) L.radius3[k] = radius3[k];
+    }
     L.nq = nq;
     L.counts = ctx->counts.p;
     L.ecounts = ctx->counts.p + nq;
@@ -431,6 +453,54 @@ int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *
     return OCT_OK;
 }
 
+int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *centers, const uint32_t *center_idx,
+               const double *radii, double radius, size_t nq, int want_lists, uint32_t *counts_out,
+               oct_search_stats *stats) {
+    if (!ctx) return OCT_ERR_ARG;
+    return radius_impl(ctx, kernel, algo, precision, centers, center_idx, radii, radius, nullptr, nullptr, nq, want_lists,
+                       counts_out, stats);
+}
+
+int oct_radius_box(oct_ctx *ctx, int kernel, int algo, const double *centers, const uint32_t *center_idx,
+                   const double *radii3, const double radius3[3], size_t nq, int want_lists, uint32_t *counts_out,
+                   oct_search_stats *stats) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (kernel != OCT_KERNEL_CUBE && kernel != OCT_KERNEL_SQUARE)
+        return fail(ctx, OCT_ERR_UNSUPPORTED, "per-axis radii need the cube or square kernel (kernel_factory.hpp:33-41)");
+    if (!radii3 && !radius3) return fail(ctx, OCT_ERR_ARG, "no radii");
+    return radius_impl(ctx, kernel, algo, OCT_PREC_FP64, centers, center_idx, nullptr, 0.0, radii3, radius3, nq,
+                       want_lists, counts_out, stats);
+}
+
+static int key_codec(oct_ctx *ctx, int encoder, bool decode, uint32_t *xyz, uint64_t *keys, size_t n) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (encoder != OCT_ENC_MORTON3D && encoder != OCT_ENC_HILBERT3D) return fail(ctx, OCT_ERR_UNSUPPORTED, "encoder");
+    if (n && (!xyz || !keys)) return fail(ctx, OCT_ERR_ARG, "null argument");
+    if (!n) return OCT_OK;
+    HCHK(hipSetDevice(ctx->device), "set device");
+    HCHK(ctx->kc_xyz.reserve(3 * n), "alloc coordinates");
+    HCHK(ctx->kc_keys.reserve(n), "alloc keys");
+    if (decode)
+        HCHK(hipMemcpyAsync(ctx->kc_keys.p, keys, n * 8, hipMemcpyHostToDevice, ctx->stream), "copy keys");
+    else
+        HCHK(hipMemcpyAsync(ctx->kc_xyz.p, xyz, n * 12, hipMemcpyHostToDevice, ctx->stream), "copy coordinates");
+    HCHK(launch_key_codec(ctx, encoder, decode, ctx->kc_xyz.p, ctx->kc_keys.p, n), "key codec");
+    if (decode)
+        HCHK(hipMemcpyAsync(xyz, ctx->kc_xyz.p, n * 12, hipMemcpyDeviceToHost, ctx->stream), "copy coordinates");
+    else
+        HCHK(hipMemcpyAsync(keys, ctx->kc_keys.p, n * 8, hipMemcpyDeviceToHost, ctx->stream), "copy keys");
+    HCHK(hipStreamSynchronize(ctx->stream), "key codec");
+    return OCT_OK;
+}
+
+int oct_encode_coords(oct_ctx *ctx, int encoder, const uint32_t *xyz, size_t n, uint64_t *keys) {
+    return key_codec(ctx, encoder, false, const_cast<uint32_t *>(xyz), keys, n);
+}
+
+int oct_decode_keys(oct_ctx *ctx, int encoder, const uint64_t *keys, size_t n, uint32_t *xyz) {
+    return key_codec(ctx, encoder, true, xyz, const_cast<uint64_t *>(keys), n);
+}
+
 int oct_radius_fetch(oct_ctx *ctx, uint64_t *offsets, uint32_t *entries, size_t entries_capacity) {
     if (!ctx || !offsets) return fail(ctx, OCT_ERR_ARG, "null argument");
     if (!ctx->has_lists) return fail(ctx, OCT_ERR_STATE, "no result lists (call oct_radius with want_lists = 1)");
@@ -460,6 +530,8 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
     }
     RadiusLaunch L;
     L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0;
+    L.group = radius_group(ctx, kernel, algo, precision);
+    L.mx = ctx->use_mx != 0;
     L.qidx = center_idx_dev;
     L.radius = radius;
     L.nq = nq;
@@ -475,6 +547,68 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
         HCHK(hipStreamSynchronize(ctx->stream), "radius");
         return read_stats(ctx, stats, stage_ms(ctx, ST_SEARCH));
     }
+    return OCT_OK;
+}
+
+int oct_set_tuning(oct_ctx *ctx, int knob, int value) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (knob == OCT_TUNE_RADIUS_MFMA) {
+        if (value != 0 && value != 1) return fail(ctx, OCT_ERR_ARG, "radius mfma switch must be 0 or 1");
+        if (ctx->use_mx != value)  // tuned tile shapes were measured for the other point-test unit
+            for (auto &t : ctx->group_tuned)
+                for (auto &u : t) u[0] = u[1] = -1;
+        ctx->use_mx = value;
+        return OCT_OK;
+    }
+    if (knob != OCT_TUNE_RADIUS_GROUP) return fail(ctx, OCT_ERR_ARG, "unknown tuning knob");
+    if (value != -1 && value != 0 && value != 16 && value != 32 && value != 64)
+        return fail(ctx, OCT_ERR_ARG, "radius group must be -1, 0, 16, 32 or 64");
+    ctx->group_override = value;
+    return OCT_OK;
+}
+
+int oct_autotune_radius(oct_ctx *ctx, int kernel, int algo, int precision, double radius, size_t sample_nq, int *best,
+                        double ms_out[4]) {
+    if (!ctx) return OCT_ERR_ARG;
+    int s = check_search_args(ctx, kernel, algo, precision);
+    if (s) return s;
+    if (ctx->n == 0) return fail(ctx, OCT_ERR_STATE, "autotune on an empty cloud");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    size_t nq = std::min(sample_nq ? sample_nq : (size_t)1 << 20, ctx->n);
+    // 16 chunks of consecutive SFC positions spread over the cloud
+    std::vector<uint32_t> idx(nq);
+    const size_t chunks = 16, per = (nq + chunks - 1) / chunks;
+    size_t k = 0;
+    for (size_t c = 0; c < chunks && k < nq; c++) {
+        size_t start = (ctx->n / chunks) * c;
+        for (size_t i = 0; i < per && k < nq; i++) idx[k++] = (uint32_t)std::min(start + i, ctx->n - 1);
+    }
+    HCHK(ctx->qidx.reserve(nq + 1), "alloc queries");
+    HCHK(ctx->counts.reserve(nq + 1), "alloc counts");
+    HCHK(hipMemcpyAsync(ctx->qidx.p, idx.data(), nq * 4, hipMemcpyHostToDevice, ctx->stream), "copy sample");
+    const int cand[4] = {0, 16, 32, 64};
+    double bestms = 1e300;
+    int bestg = 16;
+    for (int ci = 0; ci < 4; ci++) {
+        RadiusLaunch L;
+        L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = cand[ci];
+        L.mx = ctx->use_mx != 0;
+        L.qidx = ctx->qidx.p;
+        L.radius = radius;
+        L.nq = nq;
+        L.counts = ctx->counts.p;
+        double t = 1e300;
+        for (int rep = 0; rep < 3; rep++) {  // first repetition warms caches and code objects
+            HCHK(launch_radius(ctx, L), "autotune");
+            HCHK(hipStreamSynchronize(ctx->stream), "autotune");
+            if (rep) t = std::min(t, stage_ms(ctx, ST_SEARCH));
+        }
+        if (ms_out) ms_out[ci] = t;
+        if (t < bestms) bestms = t, bestg = cand[ci];
+    }
+    ctx->group_tuned[kernel & 3][algo & 3][precision & 1] = bestg;
+    ctx->has_lists = false;
+    if (best) *best = bestg;
     return OCT_OK;
 }
 

@@ -169,6 +169,13 @@ __device__ inline uint64_t lanemask_lt() {
     return l ? (~0ull >> (64 - l)) : 0ull;
 }
 __device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint32_t readlane_u32(uint32_t v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ inline double readlane_f64(double v, int lane) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 __device__ inline uint64_t uniform_u64(uint64_t v) {
     uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
     uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));

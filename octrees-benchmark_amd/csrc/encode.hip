@@ -119,6 +119,41 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__
         if (sh[i]) atomicAdd(&hist[i], sh[i]);
 }
 
+// Batched key algebra for single coordinates (PointEncoder::encode / decode,
+// morton_encoder_3d.hpp:35-46, hilbert_encoder_3d.hpp:29-66, 256-289).
+template <int ENC>
+__global__ __launch_bounds__(256) void k_key_codec(uint32_t *__restrict__ xyz, uint64_t *__restrict__ keys, size_t n,
+                                                  bool decode) {
+    __shared__ uint16_t s_tab[48 * 8];
+    if (ENC == OCT_ENC_HILBERT3D && !decode)
+        for (int t = threadIdx.x; t < 48 * 8; t += blockDim.x) s_tab[t] = kHilbert.t[t];
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (decode) {
+            uint32_t x, y, z;
+            if (ENC == OCT_ENC_HILBERT3D)
+                hilbert_decode(keys[i], x, y, z);
+            else
+                morton_decode(keys[i], x, y, z);
+            xyz[3 * i] = x, xyz[3 * i + 1] = y, xyz[3 * i + 2] = z;
+        } else {
+            const uint32_t x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+            keys[i] = ENC == OCT_ENC_HILBERT3D ? hilbert_encode_tab(s_tab, x, y, z) : morton_encode(x, y, z);
+        }
+    }
+}
+
+hipError_t launch_key_codec(oct_ctx *ctx, int encoder, bool decode, uint32_t *xyz_dev, uint64_t *keys_dev, size_t n) {
+    if (!n) return hipSuccess;
+    unsigned nb = (unsigned)((n + 255) / 256);
+    nb = nb > 2048 ? 2048 : nb;
+    if (encoder == OCT_ENC_HILBERT3D)
+        k_key_codec<OCT_ENC_HILBERT3D><<<nb, 256, 0, ctx->stream>>>(xyz_dev, keys_dev, n, decode);
+    else
+        k_key_codec<OCT_ENC_MORTON3D><<<nb, 256, 0, ctx->stream>>>(xyz_dev, keys_dev, n, decode);
+    return hipGetLastError();
+}
+
 hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], const double radii[3]) {
     size_t n = ctx->n;
     hipError_t e;

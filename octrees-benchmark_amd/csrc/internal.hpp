@@ -23,6 +23,14 @@ struct alignas(32) NodeRec {
     uint32_t first, last;
 };
 
+// FP32 node record for the FP32 searches: centre relative to the box centre (rounded once from the
+// FP64 difference) and the point range. 32 bytes.
+struct alignas(32) NodeRec32 {
+    float cx, cy, cz;
+    uint32_t first, last;
+    uint32_t pad[3];
+};
+
 // Device buffer that only grows.
 template <class T>
 struct DevBuf {
@@ -89,18 +97,25 @@ struct oct_ctx {
     oct::DevBuf<uint32_t> flags;    // per-node split flags (build scratch)
     oct::DevBuf<uint32_t> nrank;    // per-node rank among split nodes of its level (build scratch)
     oct::DevBuf<float> amax;        // per-leaf max squared norm of its recentred FP32 points
+    oct::DevBuf<oct::NodeRec32> nodes32;  // FP32 node centres relative to the box centre
     oct::DevBuf<uint32_t> scan;     // scan scratch
 
     // searches
     oct::DevBuf<uint32_t> qidx;     // query order / indices
     oct::DevBuf<double> qxyz;       // explicit query centres
     oct::DevBuf<double> qrad;       // per-query radii
+    oct::DevBuf<double> qrad3;      // per-query per-axis half extents (box kernels)
+    oct::DevBuf<uint32_t> kc_xyz;   // key codec staging
+    oct::DevBuf<uint64_t> kc_keys;
     oct::DevBuf<uint32_t> counts;   // per-query counts
     oct::DevBuf<uint64_t> qoff;     // per-query list offsets
     oct::DevBuf<uint32_t> entries;  // result lists
     oct::DevBuf<unsigned long long> stats;  // search counters
     size_t last_nq = 0;
     int last_algo = 0;
+    int group_override = -1;        // OCT_TUNE_RADIUS_GROUP (-1: use tuned / default)
+    int use_mx = 1;                 // OCT_TUNE_RADIUS_MFMA
+    int group_tuned[4][4][2];       // autotuned group per (kernel, algo, precision), -1 = not tuned
     bool has_lists = false;
 
     // kNN
@@ -124,11 +139,16 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t max_points_leaf, std::string &err
 // centre is qxyz[id] if given, else the sorted point qidx[id] (or id); outputs are indexed by id.
 struct RadiusLaunch {
     int kernel = 1, algo = 3, precision = 0, mode = 0;
+    int group = 16;  // count mode: queries per wave (16, 32, 64), or 0 for the one-lane-per-query kernel
+    bool mx = true;  // count mode, FP32 sphere: point tests on the matrix cores
     const uint32_t *order = nullptr;
     const uint32_t *qidx = nullptr;
     const double *qxyz = nullptr;
     const double *qrad = nullptr;
     double radius = 0.0;
+    const double *qrad3 = nullptr;  // box kernels with per-axis half extents: nq x 3, or
+    double radius3[3] = {0, 0, 0};  // one triple for every query when use_r3 && !qrad3
+    bool use_r3 = false;
     size_t nq = 0;
     uint32_t *counts = nullptr;
     uint32_t *ecounts = nullptr;
@@ -137,6 +157,7 @@ struct RadiusLaunch {
     unsigned long long *stats = nullptr;
 };
 hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L);
+hipError_t launch_key_codec(oct_ctx *ctx, int encoder, bool decode, uint32_t *xyz_dev, uint64_t *keys_dev, size_t n);
 hipError_t launch_knn(oct_ctx *ctx, int precision, const uint32_t *qidx_dev, const double *qxyz_dev, size_t nq,
                       uint32_t k, int32_t *idx_dev, double *d2_dev, unsigned long long *stats_dev, std::string &err);
 hipError_t exclusive_scan_u32_to_u64(oct_ctx *ctx, const uint32_t *in, uint64_t *out, size_t n);

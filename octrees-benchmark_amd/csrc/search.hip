@@ -21,6 +21,10 @@
 //    (u32 = 2^-24: |error| <= u32 (13.01 amax + 12.01 |b|^2 + 4 R) + |M - thr| + O(u64)), so
 //    u < -hw is inside, u > hw outside, and points with |u| <= hw are re-evaluated with the FP64
 //    expression. In count mode a lane re-counts a whole leaf in FP64 if any point fell in the band.
+//    A leaf/query pair whose FP32 terms could overflow (2 amax + 3 |b|^2 + 2 R >= 1e38, or a
+//    non-finite FP32 point, which sets amax = inf) is decided in FP64 throughout.
+//  - FP32_RECHECK sphere, count mode: the same u evaluated on the matrix cores, 16 points x 16
+//    queries per v_mfma_f32_16x16x4_f32 (see leaf_mx in k_count for its band).
 //    Cube/square tests are comparisons only and run on the FP64 coordinates in both modes.
 #include "device.hpp"
 
@@ -43,6 +47,9 @@ struct SearchArgs {
     const double *__restrict__ qxyz;
     const double *__restrict__ qrad;
     double radius;
+    const double *__restrict__ qrad3;
+    double r3[3];
+    int use_r3;
     uint32_t nq;
     uint32_t *__restrict__ counts;
     uint32_t *__restrict__ ecounts;
@@ -58,6 +65,16 @@ struct Query {
 };
 
 __device__ inline double dmax0(double a) { return (a < 0.0) ? 0.0 : a; }  // std::max(a, 0.0)
+
+// Cube / square kernels built from a Vector of half extents (KernelAbstract(center, radii) and
+// makeBox, kernel_abstract.hpp:39-45): box = centre -/+ radii per axis.
+__device__ inline void box_radii(const SearchArgs &a, bool valid, uint32_t qid, Query &q) {
+    if (!a.use_r3) return;
+    double rx = a.r3[0], ry = a.r3[1], rz = a.r3[2];
+    if (a.qrad3 && valid) rx = a.qrad3[3 * (size_t)qid], ry = a.qrad3[3 * (size_t)qid + 1], rz = a.qrad3[3 * (size_t)qid + 2];
+    q.lox = q.x - rx, q.loy = q.y - ry, q.loz = q.z - rz;
+    q.hix = q.x + rx, q.hiy = q.y + ry, q.hiz = q.z + rz;
+}
 
 // boxIntersect(center, Vector radii) of the four kernels (kernel_3d.hpp:181-223, 277-302;
 // kernel_2d.hpp:107-144, 195-218)
@@ -195,6 +212,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
         q.R = q.r * q.r;
         q.lox = q.x - q.r, q.loy = q.y - q.r, q.loz = q.z - q.r;
         q.hix = q.x + q.r, q.hiy = q.y + q.r, q.hiz = q.z + q.r;
+        if (KERNEL == OCT_KERNEL_CUBE || KERNEL == OCT_KERNEL_SQUARE) box_radii(a, valid, qid, q);
     }
     Sink<ALGO, MODE> sink;
     if (MODE == 2 && valid) {
@@ -221,6 +239,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
         // FP32 band setup (per lane, per leaf). u = w - 2 a.b - M approximates |p - q|^2 - R with
         // |error| <= hw (see the file header), so u < -hw is inside, u > hw outside, else recheck.
         float M = 0.f, hw = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+        bool force = false;  // FP32 terms could overflow: every point of the leaf is decided in FP64
         if (kFp32) {
             const float bx = (float)(q.x - nr.cx), by = (float)(q.y - nr.cy), bz = (float)(q.z - nr.cz);
             double b2 = (double)bx * (double)bx + (double)by * (double)by;
@@ -230,6 +249,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
             M = (float)thr;
             const double delta = 9.5367431640625e-07 * (am + b2 + q.R) * 1.0009765625 + 1e-30;  // 2^-20 (..)(1 + 2^-10)
             hw = __double2float_ru(delta + fabs((double)M - thr));
+            force = !(2.0 * am + 3.0 * b2 + 2.0 * q.R < 1e38);
             ux = -2.0f * bx, uy = -2.0f * by, uz = -2.0f * bz;
         }
         auto u32f = [&](const float4 &p) -> float {
@@ -261,7 +281,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
                     mn = fminf(mn, fabsf(u));
                 }
                 cin = (c0 + c1) + (c2 + c3);
-                const bool unc = act && !(mn > hw);
+                const bool unc = act && (!(mn > hw) || force);
                 if (ballot64(unc)) {  // rare: re-count this leaf exactly for the lanes with a band hit
                     if (unc) {
                         n_recheck += last - first;
@@ -295,8 +315,8 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
             bool in;
             if (kFp32) {
                 const float u = u32f(a.p32[i]);
-                in = u < -hw;
-                if (__builtin_expect(act && !in && u <= hw, 0)) {
+                in = !force && u < -hw;
+                if (__builtin_expect(act && !in && (u <= hw || force), 0)) {
                     const double4 pd = a.p64[i];
                     in = inside64<KERNEL>(q, pd.x, pd.y, pd.z);
                     n_recheck++;
@@ -393,9 +413,424 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ count-mode kernel
+// Per-query neighbour counts only (the harness keeps result.size(), neighbor_benchmarks.hpp:509).
+// G queries per wave, S = 64 / G lanes per query: lane = g * G + j serves query j with "group" g.
+//  - the 8 children of a popped node are classified S at a time (group g takes octant k*S + g), so
+//    no classification is duplicated; results are consumed in octant order as in singleTraversal;
+//  - a leaf's points are staged once per wave in LDS and split across the S lanes of each query
+//    (lane group g takes points g, g + S, ...); each staged chunk is padded with sentinel points
+//    that evaluate to +inf (never inside, never in the FP32 band), so every lane runs the same
+//    4x-unrolled trip count with immediate LDS offsets; partial counts are summed at the end;
+//  - a smaller G shrinks the union of the wave's traversals (fewer masked-off lane slots).
+// Empty children are skipped: they add nothing to a count whatever their class.
+constexpr int kStageBytes = 2048;  // per-wave LDS staging (128 FP32 or 64 FP64 points) + padding
+constexpr int kUnroll = 4;
+
+template <int KERNEL, int ALGO, int PREC, int G, bool STATS, bool MX>
+__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_count(const SearchArgs a) {
+    constexpr int S = 64 / G;
+    constexpr bool kFp32 = PREC == OCT_PREC_FP32_RECHECK && (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE);
+    static_assert(!MX || (kFp32 && KERNEL == OCT_KERNEL_SPHERE), "matrix-core leaf tests are FP32 sphere only");
+    constexpr int NH = MX ? G / 16 : 1;  // 16-query column blocks per wave
+    constexpr int CAP = kFp32 ? kStageBytes / 16 : kStageBytes / 32;  // real points per chunk
+    constexpr int PAD = kUnroll * S;                                   // sentinel slots after a chunk
+    __shared__ uint32_t s_node[kWaves][kStack];
+    __shared__ uint64_t s_mask[kWaves][kStack];
+    __shared__ uint8_t s_depth[kWaves][kStack];
+    __shared__ __attribute__((aligned(16))) unsigned char s_stage[kWaves][kStageBytes + PAD * (kFp32 ? 16 : 32)];
+
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t g = lane / G, j = lane % G;
+    const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    const uint32_t tile = blockIdx.x * kWaves + w;
+    if (uniform_u32(tile * G) >= a.nq) return;
+    const uint32_t slot = tile * G + j;
+    const bool valid = slot < a.nq;
+    const uint64_t qbit = 1ull << j;
+    float4 *sp32 = reinterpret_cast<float4 *>(&s_stage[w][0]);
+    double4 *sp64 = reinterpret_cast<double4 *>(&s_stage[w][0]);
+
+    Query q;
+    uint32_t qid = 0;
+    q.x = q.y = q.z = 0.0;
+    q.r = 0.0;
+    if (valid) {
+        qid = a.order ? a.order[slot] : slot;
+        if (a.qxyz) {
+            q.x = a.qxyz[3 * (size_t)qid], q.y = a.qxyz[3 * (size_t)qid + 1], q.z = a.qxyz[3 * (size_t)qid + 2];
+        } else {
+            const double4 p = a.p64[a.qidx ? a.qidx[qid] : qid];
+            q.x = p.x, q.y = p.y, q.z = p.z;
+        }
+        q.r = a.qrad ? a.qrad[qid] : a.radius;
+    }
+    q.R = q.r * q.r;
+    q.lox = q.x - q.r, q.loy = q.y - q.r, q.loz = q.z - q.r;
+    q.hix = q.x + q.r, q.hiy = q.y + q.r, q.hiz = q.z + q.r;
+    if (KERNEL == OCT_KERNEL_CUBE || KERNEL == OCT_KERNEL_SQUARE) box_radii(a, valid, qid, q);
+
+    uint32_t cnt = 0;
+    uint64_t n_class = 0, n_test = 0, n_recheck = 0, n_cslots = 0, n_tslots = 0;
+
+    // Matrix-core layout (MX): lane l owns column c16 = l % 16 and row group kk = l / 16 of each
+    // 16 x 16 tile D = A B + C of v_mfma_f32_16x16x4_f32; column block h holds queries 16h .. 16h+15
+    // of this wave. mq[h] is the centre and squared radius of query 16h + c16.
+    const uint32_t c16 = lane & 15, kk = lane >> 4;
+    double mqx[NH], mqy[NH], mqz[NH], mqR[NH];
+    uint32_t mcnt[NH];
+#pragma unroll
+    for (int h = 0; h < NH; h++) {
+        mqx[h] = mqy[h] = mqz[h] = mqR[h] = 0.0;
+        mcnt[h] = 0;
+        if (MX) {
+            const uint32_t sl = tile * G + 16 * h + c16;
+            if (sl < a.nq) {
+                const uint32_t id = a.order ? a.order[sl] : sl;
+                if (a.qxyz) {
+                    mqx[h] = a.qxyz[3 * (size_t)id], mqy[h] = a.qxyz[3 * (size_t)id + 1], mqz[h] = a.qxyz[3 * (size_t)id + 2];
+                } else {
+                    const double4 p = a.p64[a.qidx ? a.qidx[id] : id];
+                    mqx[h] = p.x, mqy[h] = p.y, mqz[h] = p.z;
+                }
+                const double r = a.qrad ? a.qrad[id] : a.radius;
+                mqR[h] = r * r;
+            }
+        }
+    }
+
+    auto judge = [&](bool act, const NodeRec &nr, uint32_t d) -> int {
+        if (!act) return -1;
+        if (ALGO == OCT_ALGO_NEIGHBORS)
+            return overlap64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]) ? J_OVERLAP : J_OUTSIDE;
+        return classify64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]);
+    };
+
+    // stage points [base, base + P) of the sorted cloud, then sentinels up to a multiple of kUnroll*S
+    auto stage = [&](uint32_t base, uint32_t P) -> uint32_t {
+        const uint32_t iters = (P + S * kUnroll - 1) / (S * kUnroll) * kUnroll;  // per-lane iterations
+        const uint32_t slots = iters * S;
+        for (uint32_t t = lane; t < slots; t += 64) {
+            if (kFp32) {
+                sp32[t] = t < P ? a.p32[base + t] : make_float4(1e30f, 1e30f, 1e30f, __builtin_huge_valf());
+            } else {
+                sp64[t] = t < P ? a.p64[base + t] : make_double4(1e300, 1e300, 1e300, 0.0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        return iters;
+    };
+
+    // Matrix-core leaf (FP32 sphere): 16 points x 16 queries per v_mfma_f32_16x16x4_f32,
+    // D = A B + C with A = the points' [ax, ay, az, w] rows, B = the queries' [-2b; 1] columns and
+    // C = fl32(hw - thr) per column, so D approximates u + hw with u = |p - q|^2 - R.
+    // The tile's summation order is not specified, so the error is bounded for any association with
+    // every product and sum rounded (to nearest or toward zero), plus the rounding of a, b, w and C:
+    // |D - (u + hw)| <= 2^-24 (25 amax + 25 |b|^2 + 11 R + hw) < hw = 2^-19 (amax + |b|^2 + R)(1 + 2^-10).
+    // Hence D <= 0 (sign bit set, -0 included) is inside, D > 2 hw is outside, and 0 <= D <= 2 hw -
+    // an unsigned compare of the bit patterns, which also sends +0 there - is re-tested in FP64.
+    // Rows past the leaf are sentinels (0, 0, 0, 1e30): D ~ 1e30, positive. A leaf/query pair whose
+    // terms could overflow FP32 (2 amax + 3 |b|^2 + 2 R >= 1e38) is re-tested in FP64 outright.
+    auto leaf_mx = [&](uint64_t m, uint32_t c, const NodeRec &nr, uint32_t first, uint32_t last) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const float *pf = reinterpret_cast<const float *>(a.p32);
+        const double am = (double)a.amax[c];
+        const uint32_t nblk = (last - first + 15) >> 4;
+        float bop[NH], ci[NH];
+        uint32_t lim[NH], mnb[NH], c8[NH];
+        bool on[NH], force[NH];
+#pragma unroll
+        for (int h = 0; h < NH; h++) {
+            on[h] = ((m >> (16 * h)) & 0xffffull) != 0;  // wave-uniform
+            c8[h] = 0;
+            mnb[h] = 0xffffffffu;
+            bop[h] = ci[h] = 0.f;
+            lim[h] = 0;
+            force[h] = false;
+            if (on[h]) {
+                const float bx = (float)(mqx[h] - nr.cx), by = (float)(mqy[h] - nr.cy), bz = (float)(mqz[h] - nr.cz);
+                const double b2 = (double)bx * (double)bx + (double)by * (double)by + (double)bz * (double)bz;
+                const double thr = mqR[h] - b2;
+                const float hw = __double2float_ru(1.9073486328125e-06 * (am + b2 + mqR[h]) * 1.0009765625 + 1e-30);
+                ci[h] = (float)((double)hw - thr);
+                lim[h] = __float_as_uint(2.0f * hw);
+                force[h] = !(2.0 * am + 3.0 * b2 + 2.0 * mqR[h] < 1e38);
+                bop[h] = kk == 0 ? -2.0f * bx : kk == 1 ? -2.0f * by : kk == 2 ? -2.0f * bz : 1.0f;
+            }
+        }
+        for (uint32_t b0 = 0; b0 < nblk; b0 += 4) {
+            float av[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t row = first + 16 * (b0 + t) + c16;
+                av[t] = row < last ? pf[4 * (size_t)row + kk] : (kk == 3 ? 1e30f : 0.0f);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                if (b0 + t < nblk) {
+#pragma unroll
+                    for (int h = 0; h < NH; h++) {
+                        if (on[h]) {
+                            const f4 cv = {ci[h], ci[h], ci[h], ci[h]};
+                            const f4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bop[h], cv, 0, 0, 0);
+                            const uint32_t d0 = __float_as_uint(d[0]), d1 = __float_as_uint(d[1]);
+                            const uint32_t d2 = __float_as_uint(d[2]), d3 = __float_as_uint(d[3]);
+                            // v_perm selector 9 / 11: byte = 8 copies of bit 31 of src1 / src0
+                            c8[h] += __builtin_popcount(__builtin_amdgcn_perm(d0, d1, 0x0c0c0b09u));
+                            c8[h] += __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u));
+                            mnb[h] = min(mnb[h], min(d0, d1));
+                            mnb[h] = min(mnb[h], min(d2, d3));
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < NH; h++) {
+            if (!on[h]) continue;
+            const bool act = ((m >> (16 * h + c16)) & 1ull) != 0;
+            uint32_t lc = c8[h] >> 3;
+            const uint64_t bm = ballot64(act && (mnb[h] <= lim[h] || force[h]));
+            const uint32_t cols = (uint32_t)((bm | (bm >> 16) | (bm >> 32) | (bm >> 48)) & 0xffffull);
+            if (cols) {  // rare: exact FP64 re-count of this column's rows for queries with a band hit
+                if ((cols >> c16) & 1u) {
+                    Query qh;
+                    qh.x = mqx[h], qh.y = mqy[h], qh.z = mqz[h], qh.R = mqR[h];
+                    lc = 0;
+                    for (uint32_t i = first + 4 * kk; i < last; i += 16)
+                        for (uint32_t r = 0; r < 4 && i + r < last; r++) {
+                            const double4 pd = a.p64[i + r];
+                            lc += inside64<KERNEL>(qh, pd.x, pd.y, pd.z);
+                            if (STATS) n_recheck++;
+                        }
+                }
+            }
+            if (act) mcnt[h] += lc;
+            if (STATS) n_tslots += 256ull * nblk;  // (point, query) pairs evaluated
+        }
+    };
+
+    // leaf: queries in m count the leaf's points; lane group g takes points g, g + S, ...
+    auto leaf = [&](uint64_t m, uint32_t c, const NodeRec &nr) {
+        const uint32_t first = nr.first, last = nr.last;
+        const bool act = (m & qbit) != 0 && valid;
+        if (STATS) {
+            n_test += (uint64_t)__popcll(m) * (last - first);
+            if (!MX) n_tslots += 64ull * ((last - first + S - 1) / S);
+        }
+        if constexpr (MX) {
+            leaf_mx(m, c, nr, first, last);
+            return;
+        }
+        uint32_t lc = 0;
+        if (kFp32) {
+            const float bx = (float)(q.x - nr.cx), by = (float)(q.y - nr.cy), bz = (float)(q.z - nr.cz);
+            double b2 = (double)bx * (double)bx + (double)by * (double)by;
+            if (KERNEL == OCT_KERNEL_SPHERE) b2 += (double)bz * (double)bz;
+            const double thr = q.R - b2;
+            const float M = (float)thr;
+            const double delta = 9.5367431640625e-07 * ((double)a.amax[c] + b2 + q.R) * 1.0009765625 + 1e-30;
+            const float hw = __double2float_ru(delta + fabs((double)M - thr)), nhw = -hw;
+            const bool force = !(2.0 * (double)a.amax[c] + 3.0 * b2 + 2.0 * q.R < 1e38);  // FP32 terms could overflow
+            const float ux = -2.0f * bx, uy = -2.0f * by, uz = -2.0f * bz;
+            float mn = __builtin_huge_valf();
+            for (uint32_t base = first; base < last; base += CAP) {
+                const uint32_t P = (last - base) < (uint32_t)CAP ? (last - base) : (uint32_t)CAP;
+                const uint32_t iters = stage(base, P);
+                const float4 *pl = sp32 + g;
+                uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * S) {
+                    const float4 p0 = pl[0], p1 = pl[S], p2 = pl[2 * S], p3 = pl[3 * S];
+                    float u0, u1, u2, u3;
+                    if (KERNEL == OCT_KERNEL_SPHERE) {
+                        u0 = __builtin_fmaf(p0.x, ux, __builtin_fmaf(p0.y, uy, __builtin_fmaf(p0.z, uz, p0.w - M)));
+                        u1 = __builtin_fmaf(p1.x, ux, __builtin_fmaf(p1.y, uy, __builtin_fmaf(p1.z, uz, p1.w - M)));
+                        u2 = __builtin_fmaf(p2.x, ux, __builtin_fmaf(p2.y, uy, __builtin_fmaf(p2.z, uz, p2.w - M)));
+                        u3 = __builtin_fmaf(p3.x, ux, __builtin_fmaf(p3.y, uy, __builtin_fmaf(p3.z, uz, p3.w - M)));
+                    } else {
+                        u0 = __builtin_fmaf(p0.x, p0.x + ux, __builtin_fmaf(p0.y, p0.y + uy, -M));
+                        u1 = __builtin_fmaf(p1.x, p1.x + ux, __builtin_fmaf(p1.y, p1.y + uy, -M));
+                        u2 = __builtin_fmaf(p2.x, p2.x + ux, __builtin_fmaf(p2.y, p2.y + uy, -M));
+                        u3 = __builtin_fmaf(p3.x, p3.x + ux, __builtin_fmaf(p3.y, p3.y + uy, -M));
+                    }
+                    c0 += (u0 < nhw) ? 1u : 0u;
+                    c1 += (u1 < nhw) ? 1u : 0u;
+                    c2 += (u2 < nhw) ? 1u : 0u;
+                    c3 += (u3 < nhw) ? 1u : 0u;
+                    mn = fminf(mn, fminf(fabsf(u0), fabsf(u1)));
+                    mn = fminf(mn, fminf(fabsf(u2), fabsf(u3)));
+                }
+                lc += (c0 + c1) + (c2 + c3);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+            const bool band = act && (!(mn > hw) || force);
+            if (ballot64(band)) {  // rare: exact FP64 re-count of this lane's share of the leaf
+                if (band) {
+                    lc = 0;
+                    for (uint32_t i = first + g; i < last; i += S) {
+                        const double4 pd = a.p64[i];
+                        lc += inside64<KERNEL>(q, pd.x, pd.y, pd.z);
+                        if (STATS) n_recheck++;
+                    }
+                }
+            }
+        } else {
+            for (uint32_t base = first; base < last; base += CAP) {
+                const uint32_t P = (last - base) < (uint32_t)CAP ? (last - base) : (uint32_t)CAP;
+                const uint32_t iters = stage(base, P);
+                const double4 *pl = sp64 + g;
+                uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * S) {
+                    const double4 p0 = pl[0], p1 = pl[S], p2 = pl[2 * S], p3 = pl[3 * S];
+                    c0 += inside64<KERNEL>(q, p0.x, p0.y, p0.z) ? 1u : 0u;
+                    c1 += inside64<KERNEL>(q, p1.x, p1.y, p1.z) ? 1u : 0u;
+                    c2 += inside64<KERNEL>(q, p2.x, p2.y, p2.z) ? 1u : 0u;
+                    c3 += inside64<KERNEL>(q, p3.x, p3.y, p3.z) ? 1u : 0u;
+                }
+                lc += (c0 + c1) + (c2 + c3);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (act) cnt += lc;
+    };
+
+    // root (singleTraversal, linear_octree.hpp:655-662); group 0 owns the root's INSIDE count.
+    // Stack entries hold the FIRST CHILD of an OVERLAP internal node (known when it is pushed), so a
+    // pop needs one round of memory traffic: the 8 children's records and child indices together.
+    uint32_t sp = 0;
+    {
+        const NodeRec nr = a.nodes[0];
+        const int jr = judge(valid, nr, 0);
+        if (STATS) n_class += __popcll(ballot64(valid) & gmask), n_cslots += 64;
+        if (g == 0 && jr == J_INSIDE) cnt += nr.last - nr.first;
+        const uint64_t ov = ballot64(jr == J_OVERLAP) & gmask;
+        if (ov) {
+            const uint32_t c0 = uniform_u32(a.child[0]);
+            if (c0 == 0) {
+                NodeRec ur;
+                ur.cx = nr.cx, ur.cy = nr.cy, ur.cz = nr.cz;
+                ur.first = uniform_u32(nr.first), ur.last = uniform_u32(nr.last);
+                leaf(ov, 0, ur);
+            } else {
+                s_node[w][0] = c0, s_mask[w][0] = ov, s_depth[w][0] = 0;
+                sp = 1;
+            }
+        }
+    }
+    while (sp > 0) {
+        sp--;
+        const uint32_t fc = uniform_u32(s_node[w][sp]);
+        const uint64_t m = uniform_u64(s_mask[w][sp]);
+        const uint32_t d = uniform_u32(s_depth[w][sp]) + 1;
+        const bool inm = (m & qbit) != 0;
+        int jv[8 / S];
+        NodeRec nv[8 / S];
+        uint32_t cv[8 / S];
+#pragma unroll
+        for (int k = 0; k < 8 / S; k++) {
+            nv[k] = a.nodes[fc + k * S + g];
+            cv[k] = a.child[fc + k * S + g];
+        }
+#pragma unroll
+        for (int k = 0; k < 8 / S; k++) {
+            const bool act = inm && nv[k].first != nv[k].last;
+            jv[k] = judge(act, nv[k], d);
+            if (jv[k] == J_INSIDE) cnt += nv[k].last - nv[k].first;
+            if (STATS) n_class += __popcll(ballot64(act));
+        }
+        if (STATS) n_cslots += 64ull * (8 / S);
+#pragma unroll
+        for (int oct = 0; oct < 8; oct++) {
+            const uint64_t ov = (ballot64(jv[oct / S] == J_OVERLAP) >> ((oct % S) * G)) & gmask;
+            if (ov) {
+                const int src = (oct % S) * G;  // a lane of the group that loaded child oct
+                const uint32_t cc = readlane_u32(cv[oct / S], src);
+                if (cc == 0) {
+                    NodeRec ur;
+                    ur.cx = readlane_f64(nv[oct / S].cx, src);
+                    ur.cy = readlane_f64(nv[oct / S].cy, src);
+                    ur.cz = readlane_f64(nv[oct / S].cz, src);
+                    ur.first = readlane_u32(nv[oct / S].first, src);
+                    ur.last = readlane_u32(nv[oct / S].last, src);
+                    leaf(ov, fc + oct, ur);
+                } else if (sp < kStack) {
+                    s_node[w][sp] = cc, s_mask[w][sp] = ov, s_depth[w][sp] = (uint8_t)d;
+                    sp++;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G; off < 64; off <<= 1) cnt += __shfl_xor(cnt, off);
+    if (MX) {
+        // column totals: lanes c16, c16 + 16, c16 + 32, c16 + 48 hold query 16h + c16's row groups;
+        // lane j (< G) has c16 = j % 16 and takes block h = j / 16
+        uint32_t tot = 0;
+#pragma unroll
+        for (int h = 0; h < NH; h++) {
+            uint32_t t = mcnt[h];
+            t += __shfl_xor(t, 16);
+            t += __shfl_xor(t, 32);
+            if ((int)(j >> 4) == h) tot = t;
+        }
+        cnt += tot;
+    }
+    if (valid && g == 0 && a.counts) a.counts[qid] = cnt;
+    if (STATS) {
+        if (lane == 0) {
+            atomicAdd(&a.stats[0], (unsigned long long)n_class);
+            atomicAdd(&a.stats[1], (unsigned long long)n_test);
+            atomicAdd(&a.stats[4], (unsigned long long)n_cslots);
+            atomicAdd(&a.stats[5], (unsigned long long)n_tslots);
+        }
+        uint64_t r = n_recheck, nb = (valid && g == 0) ? cnt : 0;
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off), nb += __shfl_xor(nb, off);
+        if (lane == 0) {
+            if (r) atomicAdd(&a.stats[2], (unsigned long long)r);
+            atomicAdd(&a.stats[3], (unsigned long long)nb);
+        }
+    }
+}
+
+template <int KERNEL, int ALGO, int PREC, bool STATS, bool MX>
+static void launch_count_s(const SearchArgs &a, int G, hipStream_t s) {
+    const size_t tiles = ((size_t)a.nq + G - 1) / G;
+    const unsigned nb = (unsigned)((tiles + kWaves - 1) / kWaves);
+    if (G == 16)
+        k_count<KERNEL, ALGO, PREC, 16, STATS, MX><<<nb, kSearchBlock, 0, s>>>(a);
+    else if (G == 32)
+        k_count<KERNEL, ALGO, PREC, 32, STATS, MX><<<nb, kSearchBlock, 0, s>>>(a);
+    else
+        k_count<KERNEL, ALGO, PREC, 64, STATS, MX><<<nb, kSearchBlock, 0, s>>>(a);
+}
+
 template <int KERNEL, int ALGO, int PREC>
-static void launch_k(const SearchArgs &a, int mode, unsigned nb, hipStream_t s) {
-    if (mode == 0)
+static void launch_count(const SearchArgs &a, int G, bool mx, hipStream_t s) {
+    if constexpr (KERNEL == OCT_KERNEL_SPHERE && PREC == OCT_PREC_FP32_RECHECK) {
+        if (mx) {
+            if (a.stats)
+                launch_count_s<KERNEL, ALGO, PREC, true, true>(a, G, s);
+            else
+                launch_count_s<KERNEL, ALGO, PREC, false, true>(a, G, s);
+            return;
+        }
+    }
+    if (a.stats)
+        launch_count_s<KERNEL, ALGO, PREC, true, false>(a, G, s);
+    else
+        launch_count_s<KERNEL, ALGO, PREC, false, false>(a, G, s);
+}
+
+template <int KERNEL, int ALGO, int PREC>
+static void launch_k(const SearchArgs &a, int mode, int G, bool mx, unsigned nb, hipStream_t s) {
+    if (mode == 0 && G > 0)
+        launch_count<KERNEL, ALGO, PREC>(a, G, mx, s);
+    else if (mode == 0)
         k_radius<KERNEL, ALGO, PREC, 0><<<nb, kSearchBlock, 0, s>>>(a);
     else if (mode == 1)
         k_radius<KERNEL, ALGO, PREC, 1><<<nb, kSearchBlock, 0, s>>>(a);
@@ -404,21 +839,21 @@ static void launch_k(const SearchArgs &a, int mode, unsigned nb, hipStream_t s) 
 }
 
 template <int KERNEL, int ALGO>
-static void launch_p(const SearchArgs &a, int prec, int mode, unsigned nb, hipStream_t s) {
+static void launch_p(const SearchArgs &a, int prec, int mode, int G, bool mx, unsigned nb, hipStream_t s) {
     if (prec == OCT_PREC_FP32_RECHECK)
-        launch_k<KERNEL, ALGO, OCT_PREC_FP32_RECHECK>(a, mode, nb, s);
+        launch_k<KERNEL, ALGO, OCT_PREC_FP32_RECHECK>(a, mode, G, mx, nb, s);
     else
-        launch_k<KERNEL, ALGO, OCT_PREC_FP64>(a, mode, nb, s);
+        launch_k<KERNEL, ALGO, OCT_PREC_FP64>(a, mode, G, mx, nb, s);
 }
 
 template <int KERNEL>
-static void launch_a(const SearchArgs &a, int algo, int prec, int mode, unsigned nb, hipStream_t s) {
+static void launch_a(const SearchArgs &a, int algo, int prec, int mode, int G, bool mx, unsigned nb, hipStream_t s) {
     if (algo == OCT_ALGO_NEIGHBORS)
-        launch_p<KERNEL, OCT_ALGO_NEIGHBORS>(a, prec, mode, nb, s);
+        launch_p<KERNEL, OCT_ALGO_NEIGHBORS>(a, prec, mode, G, mx, nb, s);
     else if (algo == OCT_ALGO_PRUNE)
-        launch_p<KERNEL, OCT_ALGO_PRUNE>(a, prec, mode, nb, s);
+        launch_p<KERNEL, OCT_ALGO_PRUNE>(a, prec, mode, G, mx, nb, s);
     else
-        launch_p<KERNEL, OCT_ALGO_STRUCT>(a, prec, mode, nb, s);
+        launch_p<KERNEL, OCT_ALGO_STRUCT>(a, prec, mode, G, mx, nb, s);
 }
 
 hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
@@ -434,6 +869,9 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     a.qxyz = L.qxyz;
     a.qrad = L.qrad;
     a.radius = L.radius;
+    a.qrad3 = L.qrad3;
+    for (int k = 0; k < 3; k++) a.r3[k] = L.radius3[k];
+    a.use_r3 = L.use_r3 ? 1 : 0;
     a.nq = (uint32_t)L.nq;
     a.counts = L.counts;
     a.ecounts = L.ecounts;
@@ -445,11 +883,12 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     const size_t tiles = (L.nq + 63) / 64;
     const unsigned nb = (unsigned)((tiles + kWaves - 1) / kWaves);
     stage_begin(ctx, ST_SEARCH);
+    const int G = L.group;
     switch (L.kernel) {
-    case OCT_KERNEL_SPHERE: launch_a<OCT_KERNEL_SPHERE>(a, L.algo, L.precision, L.mode, nb, ctx->stream); break;
-    case OCT_KERNEL_CIRCLE: launch_a<OCT_KERNEL_CIRCLE>(a, L.algo, L.precision, L.mode, nb, ctx->stream); break;
-    case OCT_KERNEL_CUBE: launch_a<OCT_KERNEL_CUBE>(a, L.algo, L.precision, L.mode, nb, ctx->stream); break;
-    default: launch_a<OCT_KERNEL_SQUARE>(a, L.algo, L.precision, L.mode, nb, ctx->stream); break;
+    case OCT_KERNEL_SPHERE: launch_a<OCT_KERNEL_SPHERE>(a, L.algo, L.precision, L.mode, G, L.mx, nb, ctx->stream); break;
+    case OCT_KERNEL_CIRCLE: launch_a<OCT_KERNEL_CIRCLE>(a, L.algo, L.precision, L.mode, G, L.mx, nb, ctx->stream); break;
+    case OCT_KERNEL_CUBE: launch_a<OCT_KERNEL_CUBE>(a, L.algo, L.precision, L.mode, G, L.mx, nb, ctx->stream); break;
+    default: launch_a<OCT_KERNEL_SQUARE>(a, L.algo, L.precision, L.mode, G, L.mx, nb, ctx->stream); break;
     }
     stage_end(ctx, ST_SEARCH);
     return hipGetLastError();

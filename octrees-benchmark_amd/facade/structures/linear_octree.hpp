@@ -1,0 +1,241 @@
+// Drop-in LinearOctree (reference inc/structures/linear_octree.hpp:614-1111): the same constructor
+// and search signatures, backed by the gfx950 build and searches through the C ABI. Each octree
+// owns one device context (its cloud, keys and tree stay resident on the GPU).
+//
+// Per-query calls run as batches of one (exact, as the reference); the batch calls at the end
+// (neighborsCount, neighborsCountAll, knnBatch) are the throughput API. Results are identical to
+// the reference's: indices into the sorted cloud, lists in the reference's emission order.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <utility>
+#include <vector>
+
+#include "../encoding/point_encoder.hpp"
+#include "../geometry/box.hpp"
+#include "../main_options.hpp"
+#include "../octree_amd_runtime.hpp"
+#include "neighbor_set.hpp"
+
+struct BuildLog {
+    size_t cloudSize = 0;
+    EncoderType encoding = EncoderType::NO_ENCODING;
+    size_t maxLeafPoints = 0;
+    double linearOctreeLeafTime = 0.0, linearOctreeInternalTime = 0.0, totalTime = 0.0;
+    size_t octreeLeaves = 0, octreeTotalNodes = 0;
+};
+
+template <class Container>
+class LinearOctree {
+    using key_t = PointEncoding::key_t;
+    static constexpr int kPrecision = OCT_PREC_FP32_RECHECK;  // identical sets to FP64, faster
+
+public:
+    explicit LinearOctree(Container &points, std::vector<key_t> &codes, Box box, PointEncoding::PointEncoder &enc,
+                          std::shared_ptr<BuildLog> log = nullptr)
+        : points_(points), enc_(enc), codes_(codes), box_(box), ctx_(new octamd::Context()) {
+        if (codes.size() != points.size()) throw std::invalid_argument("octree_amd: codes and points differ in size");
+        const double ctr[3] = {box.center().getX(), box.center().getY(), box.center().getZ()};
+        const double rad[3] = {box.radii().getX(), box.radii().getY(), box.radii().getZ()};
+        octamd::check(ctx(), oct_set_sorted(ctx(), xyz_ptr(points), points.size(), sizeof(Point),
+                                            reinterpret_cast<const uint64_t *>(codes.data()), (int)enc.getEncoder(), ctr, rad),
+                      "oct_set_sorted");
+        octamd::check(ctx(), oct_build(ctx(), (uint32_t)mainOptions.maxPointsLeaf, &info_), "oct_build");
+        if (log) {
+            log->cloudSize = points.size();
+            log->encoding = enc.getEncoder();
+            log->maxLeafPoints = mainOptions.maxPointsLeaf;
+            log->linearOctreeLeafTime = info_.build_ms * 1e-3;
+            log->totalTime = info_.build_ms * 1e-3;
+            log->octreeLeaves = info_.n_leaf;
+            log->octreeTotalNodes = info_.n_total;
+        }
+    }
+
+    // ---- radius searches (linear_octree.hpp:1038-1087)
+    template <Kernel_t K = Kernel_t::square>
+    NeighborSet<Container> neighborsStruct(const Point &p, double radius) const {
+        return as_set(list((int)K, OCT_ALGO_STRUCT, p, radius, nullptr));
+    }
+    template <Kernel_t K = Kernel_t::cube>
+    NeighborSet<Container> neighborsStruct(const Point &p, const Vector &radii) const {
+        return as_set(list((int)K, OCT_ALGO_STRUCT, p, 0.0, &radii));
+    }
+    template <Kernel_t K = Kernel_t::square>
+    std::vector<size_t> neighborsPrune(const Point &p, double radius) const {
+        return as_indices(list((int)K, OCT_ALGO_PRUNE, p, radius, nullptr));
+    }
+    template <Kernel_t K = Kernel_t::cube>
+    std::vector<size_t> neighborsPrune(const Point &p, const Vector &radii) const {
+        return as_indices(list((int)K, OCT_ALGO_PRUNE, p, 0.0, &radii));
+    }
+    template <Kernel_t K = Kernel_t::square>
+    std::vector<size_t> neighbors(const Point &p, double radius) const {
+        return as_indices(list((int)K, OCT_ALGO_NEIGHBORS, p, radius, nullptr));
+    }
+    template <Kernel_t K = Kernel_t::cube>
+    std::vector<size_t> neighbors(const Point &p, const Vector &radii) const {
+        return as_indices(list((int)K, OCT_ALGO_NEIGHBORS, p, 0.0, &radii));
+    }
+    // isInside && condition(point) (linear_octree.hpp:1076-1087): the condition is the caller's
+    // host function, applied to the device's kernel hits in their order
+    template <Kernel_t K = Kernel_t::square, class Function>
+    std::vector<size_t> neighbors(const Point &p, double radius, Function &&condition) const {
+        return filter(neighbors<K>(p, radius), condition);
+    }
+    template <Kernel_t K = Kernel_t::cube, class Function>
+    std::vector<size_t> neighbors(const Point &p, const Vector &radii, Function &&condition) const {
+        return filter(neighbors<K>(p, radii), condition);
+    }
+
+    // ---- kNN (linear_octree.hpp:899-942): min(n, k) nearest, ascending squared distance
+    size_t knn(const Point &p, const size_t k, std::vector<size_t> &indexes, std::vector<double> &dists) const {
+        const size_t m = std::min(points_.size(), k);
+        if (m == 0) return 0;
+        std::vector<int32_t> idx(k);
+        std::vector<double> d2(k);
+        const double c[3] = {p.getX(), p.getY(), p.getZ()};
+        {
+            std::lock_guard<std::mutex> g(ctx_->mutex());
+            octamd::check(ctx(), oct_knn(ctx(), OCT_PREC_FP64, c, nullptr, 1, (uint32_t)k, idx.data(), d2.data(), nullptr),
+                          "oct_knn");
+        }
+        if (indexes.size() < m) indexes.resize(m);
+        if (dists.size() < m) dists.resize(m);
+        for (size_t i = 0; i < m; i++) indexes[i] = (size_t)idx[i], dists[i] = d2[i];
+        return m;
+    }
+
+    // ---- batch API: the throughput path
+    // per-centre neighbour counts for a batch of centres
+    template <Kernel_t K = Kernel_t::sphere>
+    std::vector<uint32_t> neighborsCount(const std::vector<Point> &centers, double radius) const {
+        std::vector<double> c(3 * centers.size());
+        for (size_t i = 0; i < centers.size(); i++)
+            c[3 * i] = centers[i].getX(), c[3 * i + 1] = centers[i].getY(), c[3 * i + 2] = centers[i].getZ();
+        std::vector<uint32_t> counts(centers.size());
+        std::lock_guard<std::mutex> g(ctx_->mutex());
+        octamd::check(ctx(), oct_radius(ctx(), (int)K, OCT_ALGO_STRUCT, kPrecision, c.data(), nullptr, nullptr, radius,
+                                        centers.size(), 0, counts.data(), nullptr),
+                      "oct_radius");
+        return counts;
+    }
+    // every point of the (sorted) cloud as a centre, in SFC order (the harness' -s all mode)
+    template <Kernel_t K = Kernel_t::sphere>
+    std::vector<uint32_t> neighborsCountAll(double radius) const {
+        std::vector<uint32_t> counts(points_.size());
+        std::lock_guard<std::mutex> g(ctx_->mutex());
+        octamd::check(ctx(), oct_radius(ctx(), (int)K, OCT_ALGO_STRUCT, kPrecision, nullptr, nullptr, nullptr, radius,
+                                        points_.size(), 0, counts.data(), nullptr),
+                      "oct_radius");
+        return counts;
+    }
+
+    // ---- structure queries (linear_octree.hpp:626-651)
+    size_t computeMemoryFootprint() const {
+        return sizeof(LinearOctree) + (size_t)info_.n_total * (2 * sizeof(size_t) + sizeof(uint32_t) + sizeof(Point)) +
+               sizeof(uint32_t) + 22 * sizeof(Vector);
+    }
+    double getDensity() const {
+        return (double)points_.size() / (box_.radii().getX() * box_.radii().getY() * box_.radii().getZ() * 8.0f);
+    }
+    const oct_tree_info &treeInfo() const { return info_; }
+
+    // singleTraversal (linear_octree.hpp:653-686) with the caller's host callbacks, over a copy of the
+    // device tree: continuationCriterion(node, depth) -> bool, endpointAction(leaf)
+    template <class C, class A>
+    void singleTraversal(C &&continuationCriterion, A &&endpointAction) const {
+        fetch_tree();
+        if (!continuationCriterion(0u, 0u)) return;
+        if (offsets_[0] == 0) {
+            endpointAction(0u);
+            return;
+        }
+        std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, 0u}};
+        uint32_t node = 0, depth = 0;
+        do {
+            for (uint32_t o = 0; o < 8; o++) {
+                const uint32_t child = offsets_[node] + o;
+                if (continuationCriterion(child, depth + 1)) {
+                    if (offsets_[child] == 0)
+                        endpointAction(child);
+                    else
+                        stack.emplace_back(child, depth + 1);
+                }
+            }
+            std::tie(node, depth) = stack.back();
+            stack.pop_back();
+        } while (node != 0);
+    }
+    // node arrays of the reference (centers, internalRanges, offsets), copied from the device on first use
+    const std::vector<Point> &centers() const { return fetch_tree(), centers_; }
+    const std::vector<std::pair<size_t, size_t>> &internalRanges() const { return fetch_tree(), ranges_; }
+    const std::vector<uint32_t> &offsets() const { return fetch_tree(), offsets_; }
+
+private:
+    Container &points_;
+    PointEncoding::PointEncoder &enc_;
+    std::vector<key_t> &codes_;
+    Box box_;
+    std::unique_ptr<octamd::Context> ctx_;
+    oct_tree_info info_{};
+    mutable std::vector<Point> centers_;
+    mutable std::vector<std::pair<size_t, size_t>> ranges_;
+    mutable std::vector<uint32_t> offsets_;
+
+    oct_ctx *ctx() const { return ctx_->get(); }
+
+    struct Lists {
+        int algo;
+        std::vector<uint32_t> entries;
+    };
+    Lists list(int kernel, int algo, const Point &p, double radius, const Vector *radii) const {
+        const double c[3] = {p.getX(), p.getY(), p.getZ()};
+        std::lock_guard<std::mutex> g(ctx_->mutex());
+        if (radii) {
+            const double r3[3] = {radii->getX(), radii->getY(), radii->getZ()};
+            octamd::check(ctx(), oct_radius_box(ctx(), kernel, algo, c, nullptr, nullptr, r3, 1, 1, nullptr, nullptr),
+                          "oct_radius_box");
+        } else {
+            octamd::check(ctx(), oct_radius(ctx(), kernel, algo, kPrecision, c, nullptr, nullptr, radius, 1, 1, nullptr, nullptr),
+                          "oct_radius");
+        }
+        uint64_t offs[2];
+        octamd::check(ctx(), oct_radius_fetch(ctx(), offs, nullptr, 0), "oct_radius_fetch");
+        Lists l{algo, std::vector<uint32_t>((size_t)offs[1] * (algo == OCT_ALGO_STRUCT ? 2 : 1))};
+        if (!l.entries.empty())
+            octamd::check(ctx(), oct_radius_fetch(ctx(), offs, l.entries.data(), l.entries.size()), "oct_radius_fetch");
+        return l;
+    }
+    NeighborSet<Container> as_set(const Lists &l) const {
+        NeighborSet<Container> s(&points_);
+        for (size_t i = 0; i + 1 < l.entries.size(); i += 2) s.addRange(l.entries[i], l.entries[i + 1]);
+        return s;
+    }
+    static std::vector<size_t> as_indices(const Lists &l) { return std::vector<size_t>(l.entries.begin(), l.entries.end()); }
+    template <class Function>
+    std::vector<size_t> filter(std::vector<size_t> v, Function &&cond) const {
+        std::vector<size_t> out;
+        out.reserve(v.size());
+        for (size_t i : v)
+            if (cond(points_[i])) out.push_back(i);
+        return out;
+    }
+    void fetch_tree() const {
+        if (!offsets_.empty()) return;
+        const size_t nt = info_.n_total;
+        std::vector<uint64_t> rg(2 * nt);
+        std::vector<double> ce(3 * nt);
+        offsets_.resize(nt);
+        std::lock_guard<std::mutex> g(ctx_->mutex());
+        octamd::check(ctx(), oct_get_tree(ctx(), nullptr, offsets_.data(), rg.data(), ce.data(), nullptr), "oct_get_tree");
+        centers_.resize(nt);
+        ranges_.resize(nt);
+        for (size_t i = 0; i < nt; i++) {
+            centers_[i] = Point(ce[3 * i], ce[3 * i + 1], ce[3 * i + 2]);
+            ranges_[i] = {(size_t)rg[2 * i], (size_t)rg[2 * i + 1]};
+        }
+    }
+};

@@ -24,6 +24,7 @@ EXPORTS = (
     "oct_create", "oct_destroy", "oct_last_error", "oct_synchronize", "oct_version", "oct_set_points",
     "oct_set_points_dev", "oct_bbox", "oct_encode", "oct_sort", "oct_sort_points", "oct_set_sorted", "oct_build",
     "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_stage_times",
+    "oct_set_tuning", "oct_autotune_radius", "oct_radius_box", "oct_encode_coords", "oct_decode_keys",
 )
 
 
@@ -79,6 +80,13 @@ def lib():
                                      C.POINTER(SearchStats)]
         L.oct_knn.argtypes = [_vp, C.c_int, _vp, _vp, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_stage_times.argtypes = [_vp, _vp]
+        L.oct_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
+        L.oct_autotune_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_size_t,
+                                          C.POINTER(C.c_int), _vp]
+        L.oct_radius_box.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, C.c_size_t, C.c_int, _vp,
+                                     C.POINTER(SearchStats)]
+        L.oct_encode_coords.argtypes = [_vp, C.c_int, _vp, C.c_size_t, _vp]
+        L.oct_decode_keys.argtypes = [_vp, C.c_int, _vp, C.c_size_t, _vp]
         _LIB = L
     return _LIB
 
@@ -233,6 +241,42 @@ class Octree:
                                     float(radius), nq, 1 if lists else 0, _ptr(counts), C.byref(st)), "oct_radius")
         if not lists:
             return counts, st.as_dict()
+        return self._fetch_lists(algo, nq, counts, st)
+
+    def radius_box(self, kernel, algo, radii3, centers=None, idx=None, nq=None, lists=False):
+        """Cube / square searches with per-axis half extents: radii3 is one (3,) triple or (nq, 3)."""
+        if centers is not None:
+            centers = np.ascontiguousarray(centers, np.float64).reshape(-1, 3)
+            nq = centers.shape[0]
+        elif idx is not None:
+            idx = np.ascontiguousarray(idx, np.uint32)
+            nq = idx.shape[0]
+        elif nq is None:
+            nq = self.n
+        r3 = np.ascontiguousarray(radii3, np.float64)
+        per_query = r3.ndim == 2
+        counts = np.empty(nq, np.uint32)
+        st = SearchStats()
+        self._chk(self.L.oct_radius_box(self.h, kernel, algo, _ptr(centers), _ptr(idx), _ptr(r3) if per_query else None,
+                                        None if per_query else _ptr(r3), nq, 1 if lists else 0, _ptr(counts),
+                                        C.byref(st)), "oct_radius_box")
+        if not lists:
+            return counts, st.as_dict()
+        return self._fetch_lists(algo, nq, counts, st)
+
+    def encode_coords(self, encoder, xyz):
+        xyz = np.ascontiguousarray(xyz, np.uint32).reshape(-1, 3)
+        keys = np.empty(xyz.shape[0], np.uint64)
+        self._chk(self.L.oct_encode_coords(self.h, encoder, _ptr(xyz), xyz.shape[0], _ptr(keys)), "oct_encode_coords")
+        return keys
+
+    def decode_keys(self, encoder, keys):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        xyz = np.empty((keys.shape[0], 3), np.uint32)
+        self._chk(self.L.oct_decode_keys(self.h, encoder, _ptr(keys), keys.shape[0], _ptr(xyz)), "oct_decode_keys")
+        return xyz
+
+    def _fetch_lists(self, algo, nq, counts, st):
         offs = np.empty(nq + 1, np.uint64)
         self._chk(self.L.oct_radius_fetch(self.h, _ptr(offs), None, 0), "oct_radius_fetch(offsets)")
         words = int(offs[-1]) * (2 if algo == A_STRUCT else 1)
@@ -271,6 +315,21 @@ class Octree:
         self._chk(self.L.oct_knn(self.h, precision, _ptr(centers), _ptr(idx), nq, k, _ptr(oi), _ptr(od), C.byref(st)),
                   "oct_knn")
         return oi, od, st.as_dict()
+
+    def set_group(self, g):
+        """Queries per wave of the count-mode radius kernel (0, 16, 32, 64; -1 = tuned/default)."""
+        self._chk(self.L.oct_set_tuning(self.h, 1, g), "oct_set_tuning")
+
+    def set_mfma(self, on):
+        """FP32 sphere count-mode point tests on the matrix cores (True, default) or vector ALUs."""
+        self._chk(self.L.oct_set_tuning(self.h, 2, 1 if on else 0), "oct_set_tuning")
+
+    def autotune_radius(self, kernel, algo, precision, radius, sample_nq=1 << 20):
+        best = C.c_int(0)
+        ms = np.zeros(4)
+        self._chk(self.L.oct_autotune_radius(self.h, kernel, algo, precision, float(radius), sample_nq, C.byref(best),
+                                             _ptr(ms)), "oct_autotune_radius")
+        return best.value, dict(zip((0, 16, 32, 64), ms.tolist()))
 
     def synchronize(self):
         self._chk(self.L.oct_synchronize(self.h), "oct_synchronize")

@@ -766,8 +766,31 @@ struct orc_result {
     uint64_t *offsets, *entries, *sizes;
 };
 
+/* Cube / square kernel from a Vector of half extents: KernelAbstract(center, radii) + makeBox
+ * (kernel_abstract.hpp:39-45), box = centre -/+ radii per axis. */
+static void kern_init3(kern_t *k, int type, const double c[3], const double r3[3]) {
+    kern_init(k, type, c, r3[0]);
+    for (int a = 0; a < 3; a++) {
+        k->bmin[a] = c[a] - r3[a];
+        k->bmax[a] = c[a] + r3[a];
+    }
+}
+
+static orc_result *radius_any(const orc_tree *t, const double *xyz, int kernel, int algo, const double *centers,
+                              const double *radii, const double *radii3, size_t nq, uint64_t *stats);
+
 orc_result *orc_radius(const orc_tree *t, const double *xyz, int kernel, int algo, const double *centers,
                        const double *radii, size_t nq, uint64_t *stats) {
+    return radius_any(t, xyz, kernel, algo, centers, radii, NULL, nq, stats);
+}
+
+orc_result *orc_radius_box(const orc_tree *t, const double *xyz, int kernel, int algo, const double *centers,
+                           const double *radii3, size_t nq) {
+    return radius_any(t, xyz, kernel, algo, centers, NULL, radii3, nq, NULL);
+}
+
+static orc_result *radius_any(const orc_tree *t, const double *xyz, int kernel, int algo, const double *centers,
+                              const double *radii, const double *radii3, size_t nq, uint64_t *stats) {
     vec_t *outs = (vec_t *)calloc(nq ? nq : 1, sizeof(vec_t));
     orc_result *r = (orc_result *)calloc(1, sizeof(orc_result));
     r->nq = nq;
@@ -777,7 +800,10 @@ orc_result *orc_radius(const orc_tree *t, const double *xyz, int kernel, int alg
         search_t s;
         memset(&s, 0, sizeof(s));
         s.t = t, s.xyz = xyz, s.algo = algo;
-        kern_init(&s.k, kernel, &centers[3 * q], radii[q]);
+        if (radii3)
+            kern_init3(&s.k, kernel, &centers[3 * q], &radii3[3 * q]);
+        else
+            kern_init(&s.k, kernel, &centers[3 * q], radii[q]);
         single_traversal(&s);
         outs[q] = s.out;
         r->sizes[q] = s.size;

@@ -93,6 +93,10 @@ typedef struct orc_result orc_result;
  * (points tested with isInside, nodes classified). */
 orc_result *orc_radius(const orc_tree *t, const double *sorted_xyz, int kernel, int algo,
                        const double *centers, const double *radii, size_t nq, uint64_t *stats);
+/* Cube / square searches with per-axis half extents (nq x 3), Vector overloads of
+ * linear_octree.hpp:1044-1060, 1069-1075. */
+orc_result *orc_radius_box(const orc_tree *t, const double *sorted_xyz, int kernel, int algo, const double *centers,
+                           const double *radii3, size_t nq);
 size_t orc_result_total(const orc_result *r);        /* total entries (indices, or ranges) */
 const uint64_t *orc_result_offsets(const orc_result *r); /* nq+1 offsets into entries */
 const uint64_t *orc_result_entries(const orc_result *r); /* indices, or 2*ranges interleaved */

@@ -59,6 +59,8 @@ def lib():
         L.orc_radius_count.restype = C.c_uint64
         L.orc_radius_harness.argtypes = [_P, _F64P, C.c_int, C.c_int, _F64P, _F64P, C.c_size_t]
         L.orc_radius_harness.restype = C.c_uint64
+        L.orc_radius_box.argtypes = [_P, _F64P, C.c_int, C.c_int, _F64P, _F64P, C.c_size_t]
+        L.orc_radius_box.restype = _P
         L.orc_result_total.argtypes = [_P]
         L.orc_result_total.restype = C.c_size_t
         for f in ("orc_result_offsets", "orc_result_entries", "orc_result_sizes"):
@@ -207,6 +209,15 @@ def radius(tree, sorted_xyz, kernel, algo, centers, radii, stats=False):
     if stats:
         return out, sizes, st.reshape(-1, 2)[:nq]
     return out, sizes
+
+
+def radius_box(tree, sorted_xyz, kernel, algo, centers, radii3):
+    """Cube / square with per-axis half extents; radii3 is (3,) or (nq, 3)."""
+    centers = _f64(centers).reshape(-1, 3)
+    nq = centers.shape[0]
+    r3 = _f64(np.broadcast_to(np.asarray(radii3, np.float64), (nq, 3)))
+    res = lib().orc_radius_box(tree.h, _f64(sorted_xyz), kernel, algo, centers, r3, nq)
+    return _unpack(res, nq, algo == A_STRUCT)
 
 
 def radius_count(tree, sorted_xyz, kernel, algo, centers, radii):

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import octree_amd as A
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -41,3 +43,17 @@ def test_synth_is_deterministic():
 
 
 import numpy as np  # noqa: E402
+
+
+@pytest.mark.parametrize("std", ["c++17", "c++20"])
+def test_facade_headers_compile(std):
+    """The drop-in C++ facade (reference-named headers) compiles against the C ABI header."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ absent")
+    pkg = os.path.join(ROOT, "octrees-benchmark_amd")
+    src = os.path.join(pkg, "facade", "examples", "facade_check.cpp")
+    r = subprocess.run(["g++", f"-std={std}", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(pkg, "facade"),
+                        "-I", os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

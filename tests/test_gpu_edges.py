@@ -28,7 +28,23 @@ def run_both(ctx, pts, enc, ml, kernel, algo, centers, radius):
         for q in range(len(lists)):
             np.testing.assert_array_equal(lists[q], ref[q])
         out.append(lists)
+    check_counts(ctx, kernel, algo, centers, radius, ref_sizes)
     return sx, out[0]
+
+
+def check_counts(ctx, kernel, algo, centers, radius, ref_sizes):
+    """Every count-mode variant (tile shapes, matrix-core / vector FP32 point tests) on the same case."""
+    try:
+        for mx in (True, False):
+            ctx.set_mfma(mx)
+            for g in (0, 16, 32, 64):
+                ctx.set_group(g)
+                for prec in (A.PREC_FP64, A.PREC_FP32):
+                    counts, _ = ctx.radius(kernel, algo, prec, centers=np.asarray(centers, np.float64), radius=radius)
+                    np.testing.assert_array_equal(counts, ref_sizes, err_msg=f"mx={mx} G={g} prec={prec}")
+    finally:
+        ctx.set_group(-1)
+        ctx.set_mfma(True)
 
 
 OCT8 = [[0, 0, 0], [0.1, 0, 0], [0, 0.1, 0], [10, 10, 10], [10.1, 10, 10], [10, 10.1, 10], [5, 5, 5], [5, 5, 5.1]]
@@ -71,6 +87,16 @@ def test_large_coordinates(ctx):  # test_complex.cpp:77-94
     _, l = run_both(ctx, [[o, o, o], [o + 0.1, o, o], [o, o + 0.1, o]], A.ENC_MORTON, 128, A.K_SPHERE, A.A_PRUNE,
                     [[o + 0.05, o + 0.05, o]], 0.1)
     assert len(l[0]) == 3
+
+
+def test_huge_extent_and_radius(ctx):
+    """Coordinates and radii near 1e20: FP32 terms of the leaf tests would overflow, so those
+    leaf/query pairs must be decided in FP64."""
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(-1e20, 1e20, size=(3000, 3))
+    centers = np.vstack([pts[:40], rng.uniform(-1e20, 1e20, size=(40, 3))])
+    for kernel in (A.K_SPHERE, A.K_CIRCLE):
+        run_both(ctx, pts, A.ENC_HILBERT, 16, kernel, A.A_STRUCT, centers, 4e19)
 
 
 def test_duplicates_force_max_depth(ctx):
@@ -130,3 +156,39 @@ def test_point_struct_layout(ctx):
     _, _, p2, k2 = ctx.sort_points(A.ENC_HILBERT)
     np.testing.assert_array_equal(k1, k2)
     np.testing.assert_array_equal(p1, p2)
+
+
+def test_key_codec_matches_oracle(ctx):
+    """PointEncoder::encode / decode of single coordinates, batched on the device."""
+    rng = np.random.default_rng(21)
+    m = (1 << 21) - 1
+    xyz = np.vstack([rng.integers(0, 1 << 21, size=(5000, 3)), [[0, 0, 0], [m, m, m], [1000, 2000, 3000]]]).astype(np.uint32)
+    for enc, fe, fd in ((A.ENC_MORTON, O.morton_encode, O.morton_decode), (A.ENC_HILBERT, O.hilbert_encode, O.hilbert_decode)):
+        keys = ctx.encode_coords(enc, xyz)
+        want = np.array([fe(*t) for t in xyz], np.uint64)
+        np.testing.assert_array_equal(keys, want)
+        np.testing.assert_array_equal(ctx.decode_keys(enc, keys), xyz)
+
+
+def test_radius_box_vector_radii(ctx):
+    """Cube / square searches with per-axis half extents (Vector overloads) against the oracle."""
+    rng = np.random.default_rng(8)
+    pts = np.vstack([rng.uniform(0, 30, size=(20000, 3)) * [1, 1, 0.2], rng.normal(size=(3000, 3)) + [10, 10, 3]])
+    sx, keys, perm, c, r = O.sort_points(A.ENC_HILBERT, pts)
+    tree = O.Tree(keys, 32, A.ENC_HILBERT, c, r)
+    ctx.set_points(pts)
+    ctx.sort_points(A.ENC_HILBERT)
+    ctx.build(32)
+    centers = np.vstack([sx[rng.integers(0, len(sx), 300)], rng.uniform(-1, 31, size=(50, 3))])
+    r3q = rng.uniform(0, 2.5, size=(len(centers), 3))
+    for kernel in (A.K_CUBE, A.K_SQUARE):
+        for algo in (A.A_NEIGHBORS, A.A_PRUNE, A.A_STRUCT):
+            for r3 in (r3q, np.array([0.7, 1.9, 0.4])):
+                ref, ref_sizes = O.radius_box(tree, sx, kernel, algo, centers, r3)
+                cnt, _ = ctx.radius_box(kernel, algo, r3, centers=centers)
+                np.testing.assert_array_equal(cnt, ref_sizes)
+                cnt, lists, _ = ctx.radius_box(kernel, algo, r3, centers=centers, lists=True)
+                for q in range(len(centers)):
+                    np.testing.assert_array_equal(lists[q], ref[q])
+    with pytest.raises(A.OctError):
+        ctx.radius_box(A.K_SPHERE, A.A_PRUNE, [1.0, 1.0, 1.0], centers=centers)

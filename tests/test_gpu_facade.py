@@ -1,0 +1,89 @@
+"""The drop-in C++ facade (octrees-benchmark_amd/facade/, reference-named headers) driven the way the
+reference's example and tests drive LinearOctree (examples/example.cpp; tests/structures/*.cpp):
+sortPoints, the LinearOctree constructor, the three searches over the four kernels, knn, the batch
+count, singleTraversal and the key codec. Every printed result is compared with the oracle."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import octree_amd as A
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "octrees-benchmark_amd", "facade_check")
+
+
+def parse(text):
+    out = {"pt": [], "struct": {}, "prune": {}, "neighbors": {}, "knn": {}}
+    for line in text.splitlines():
+        f = line.split()
+        if not f:
+            continue
+        if f[0] == "box":
+            out["box"] = np.array([float(v) for v in f[1:]])
+        elif f[0] == "pt":
+            out["pt"].append((int(f[2]), int(f[3])))
+        elif f[0] == "tree":
+            out["tree"] = tuple(int(v) for v in f[1:])
+        elif f[0] in ("struct", "prune", "neighbors"):
+            key = (f[1], int(f[2]))
+            vals = f[4:]
+            if f[0] == "struct":
+                out["struct"][key] = np.array([[int(a) for a in v.split(":")] for v in vals], np.int64).reshape(-1, 2)
+            else:
+                out[f[0]][key] = np.array([int(v) for v in vals], np.int64)
+        elif f[0] == "knn":
+            out["knn"][int(f[1])] = [(int(v.split(":")[0]), float(v.split(":")[1])) for v in f[3:]]
+        elif f[0] in ("countall", "traversal_leaves"):
+            out[f[0]] = int(f[1])
+        elif f[0] == "codec":
+            out["codec"] = tuple(int(v) for v in f[1:])
+        elif f[0] == "iterator":
+            raise AssertionError(line)
+    return out
+
+
+@pytest.mark.parametrize("enc", [A.ENC_MORTON, A.ENC_HILBERT])
+def test_facade_matches_oracle(enc):
+    if not os.path.exists(EXE):
+        pytest.fail("facade_check is not built (make -C octrees-benchmark_amd)")
+    rng = np.random.default_rng(enc)
+    pts = np.vstack([rng.uniform(0, 20, size=(6000, 3)) * [1, 1, 0.3], rng.normal(size=(1000, 3)) * 0.4 + [5, 5, 2],
+                     np.repeat([[12.0, 3.0, 1.0]], 20, axis=0)])
+    centers = np.vstack([pts[rng.integers(0, len(pts), 25)], rng.uniform(-1, 21, size=(5, 3)) * [1, 1, 0.3]])
+    radius, ml = 1.25, 16
+    with tempfile.TemporaryDirectory() as d:
+        pf, cf = os.path.join(d, "p.bin"), os.path.join(d, "c.bin")
+        pts.astype("<f8").tofile(pf)
+        centers.astype("<f8").tofile(cf)
+        print(f"[gpu] facade_check enc={enc}", flush=True)
+        r = subprocess.run([EXE, str(enc), str(ml), pf, cf, repr(radius)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = parse(r.stdout)
+
+    sx, keys, perm, c, rr = O.sort_points(enc, pts)
+    np.testing.assert_array_equal(got["box"], np.concatenate([c - rr, c + rr]))
+    np.testing.assert_array_equal([i for i, _ in got["pt"]], perm)
+    np.testing.assert_array_equal(np.array([k for _, k in got["pt"]], np.uint64), keys)
+    tree = O.Tree(keys, ml, enc, c, rr)
+    assert got["tree"] == (tree.nLeaf, tree.nInternal, tree.nTotal)
+    names = {"sphere": O.K_SPHERE, "circle": O.K_CIRCLE, "cube": O.K_CUBE, "square": O.K_SQUARE}
+    for name, kernel in names.items():
+        for algo, tag in ((O.A_STRUCT, "struct"), (O.A_PRUNE, "prune"), (O.A_NEIGHBORS, "neighbors")):
+            ref, sizes = O.radius(tree, sx, kernel, algo, centers, radius)
+            for q in range(len(centers)):
+                np.testing.assert_array_equal(got[tag][(name, q)], ref[q], err_msg=f"{tag} {name} q={q}")
+    bi, bd = O.knn_brute(sx, centers, 7)
+    for q in range(len(centers)):
+        np.testing.assert_array_equal([i for i, _ in got["knn"][q]], bi[q])
+        np.testing.assert_array_equal([v for _, v in got["knn"][q]], bd[q])
+    allc = O.radius_count(tree, sx, O.K_SPHERE, O.A_STRUCT, sx, radius)
+    assert got["countall"] == int(allc.sum())
+    assert got["traversal_leaves"] == tree.nLeaf
+    k = (O.hilbert_encode if enc == A.ENC_HILBERT else O.morton_encode)(1000, 2000, 3000)
+    assert got["codec"] == (k, 1000, 2000, 3000)

@@ -426,3 +426,31 @@ def test_knn_kats():
     t = O.Tree(keys, 128, O.ENC_MORTON, c, r)
     idx, d2 = O.knn(t, sx, [[1, 1, 1]], 3)
     assert d2[0, 0] == 0 and d2[0, 1] == 0 and d2[0, 2] == 3.0
+
+
+@pytest.mark.parametrize("kernel", [O.K_CUBE, O.K_SQUARE])
+@pytest.mark.parametrize("algo", [O.A_NEIGHBORS, O.A_PRUNE, O.A_STRUCT])
+def test_radius_box_vector_radii_equal_brute_force(radius_case, kernel, algo):
+    """Vector-radii overloads (kernel_factory.hpp:33-41): box = centre -/+ radii per axis."""
+    sx, t, qs = radius_case
+    rng = np.random.default_rng(4)
+    r3 = rng.uniform(0.0, 3.0, size=(len(qs), 3))
+    out, sizes = O.radius_box(t, sx, kernel, algo, qs, r3)
+    for qi, q in enumerate(qs):
+        lo, hi = q - r3[qi], q + r3[qi]
+        m = (sx[:, 0] >= lo[0]) & (sx[:, 0] <= hi[0]) & (sx[:, 1] >= lo[1]) & (sx[:, 1] <= hi[1])
+        if kernel == O.K_CUBE:
+            m &= (sx[:, 2] >= lo[2]) & (sx[:, 2] <= hi[2])
+        want = np.flatnonzero(m)
+        rr = out[qi]
+        if algo == O.A_STRUCT:
+            got = np.concatenate([np.arange(a, b) for a, b in rr]) if len(rr) else np.zeros(0, np.int64)
+        else:
+            got = rr
+        np.testing.assert_array_equal(np.sort(got), want)
+    # equal radii reduce to the scalar search
+    o1, s1 = O.radius_box(t, sx, kernel, algo, qs, [1.3, 1.3, 1.3])
+    o2, s2 = O.radius(t, sx, kernel, algo, qs, 1.3)
+    np.testing.assert_array_equal(s1, s2)
+    for a, b in zip(o1, o2):
+        np.testing.assert_array_equal(a, b)
