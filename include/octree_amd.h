@@ -158,21 +158,22 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
 int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *center_idx, size_t nq, uint32_t k,
             int32_t *idx_out, double *d2_out, oct_search_stats *stats);
 
-/* ---- tuning. OCT_TUNE_RADIUS_GROUP: queries per wave of the count-mode radius kernel (16, 32 or
- * 64 queries share one wave's traversal, 64 / G lanes serve each query), or 0 for the
- * one-lane-per-query kernel; value -1 restores "autotuned / default". Applies to every
- * (kernel, algorithm, precision) unless the autotuner stored a choice for that combination.
- * oct_autotune_radius: times each candidate on sample_nq queries taken from the sorted cloud (16
- * equal strided chunks of consecutive SFC points) with the given radius, stores the fastest for
- * (kernel, algo, precision), returns it in *best and the per-candidate milliseconds in
- * ms_out[0..3] for G = 0, 16, 32, 64. No counterpart in the reference (its per-query loop has no
- * tile shape to choose).
- * OCT_TUNE_RADIUS_MFMA: 1 (default) runs the FP32 sphere count-mode point tests on the matrix
- * cores (16 points x 16 queries per v_mfma_f32_16x16x4_f32), 0 on the vector ALUs; same results. */
+/* ---- tuning (no counterpart in the reference: its per-query loop has no tile shape to choose).
+ * OCT_TUNE_RADIUS_GROUP: queries per wave of the count-mode radius kernel (16, 32 or 64 queries
+ * share one wave's traversal, 64 / G lanes serve each query), or 0 for the one-lane-per-query
+ * kernel; -1 restores "autotuned, else 32". OCT_TUNE_RADIUS_MFMA: 1 runs the FP32 sphere count-mode
+ * point tests on the matrix cores (16 points x 16 queries per v_mfma_f32_16x16x4_f32), 0 on the
+ * vector ALUs, -1 restores "autotuned, else vector ALUs"; every setting gives the same results.
+ * oct_autotune_radius: times each candidate (G = 0, 16, 32, 64, and for FP32 sphere both point-test
+ * units unless fixed by oct_set_tuning) on sample_nq queries taken from the sorted cloud (16 equal
+ * strided chunks of consecutive SFC points; sample_nq >= n uses every point), stores the fastest for
+ * (kernel, algo, precision), returns its G in *best and the best milliseconds per G in ms_out[0..3].
+ * oct_tuned_radius: the G and point-test unit a count-mode search would use now. */
 enum oct_tune_knob { OCT_TUNE_RADIUS_GROUP = 1, OCT_TUNE_RADIUS_MFMA = 2 };
 int oct_set_tuning(oct_ctx *ctx, int knob, int value);
 int oct_autotune_radius(oct_ctx *ctx, int kernel, int algo, int precision, double radius, size_t sample_nq, int *best,
                         double ms_out[4]);
+int oct_tuned_radius(oct_ctx *ctx, int kernel, int algo, int precision, int *group, int *mfma);
 
 /* ---- timing: device milliseconds of the stages of the last calls (HIP events on the context's
  * stream): [0] bbox, [1] encode, [2] sort, [3] gather, [4] build, [5] search, [6] knn. */

@@ -111,17 +111,40 @@ __global__ __launch_bounds__(256) void k_recentre(const double4 *__restrict__ p6
     if (lane == 0) amax[wave] = __double2float_ru(m);
 }
 
-// FP32 node records: centre - box centre, computed in FP64 and rounded once.
-__global__ __launch_bounds__(256) void k_nodes32(const NodeRec *__restrict__ nodes, uint32_t n_total, double ox, double oy,
-                                                double oz, NodeRec32 *__restrict__ out) {
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_total; u += gridDim.x * blockDim.x) {
-        const NodeRec r = nodes[u];
-        NodeRec32 o;
-        o.cx = (float)(r.cx - ox), o.cy = (float)(r.cy - oy), o.cz = (float)(r.cz - oz);
-        o.first = r.first, o.last = r.last;
-        o.pad[0] = o.pad[1] = o.pad[2] = 0;
-        out[u] = o;
+// Per internal node u (first child fc, level L): its children regrouped by corner bits
+// (NodeExt). A child's bits come from its decoded key at level L + 1, so they are exact also on
+// degenerate axes where two children share a centre value.
+struct LevelTable {
+    uint32_t start[kMaxDepth + 3];
+};
+__global__ __launch_bounds__(256) void k_node_ext(const NodeRec *__restrict__ nodes, const uint64_t *__restrict__ nkey,
+                                                 const uint32_t *__restrict__ child, const float *__restrict__ amax,
+                                                 uint32_t n_total, int encoder, LevelTable lt, NodeExt *__restrict__ ext) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n_total) return;
+    const uint32_t fc = child[u];
+    if (fc == 0) return;
+    uint32_t level = 0;
+    while (level + 1 < kMaxDepth + 2 && lt.start[level + 1] <= u) level++;
+    const uint32_t sh = kMaxDepth - (level + 1);  // corner bit of the children's level
+    NodeExt e;
+    for (int k = 0; k < 4; k++) e.pad[k] = 0;
+    for (uint32_t k = 0; k < 8; k++) {
+        const uint32_t v = fc + k;
+        uint32_t x, y, z;
+        if (encoder == OCT_ENC_HILBERT3D)
+            hilbert_decode(nkey[v], x, y, z);
+        else
+            morton_decode(nkey[v], x, y, z);
+        const uint32_t bx = (x >> sh) & 1u, by = (y >> sh) & 1u, bz = (z >> sh) & 1u;
+        const uint32_t c = bx | (by << 1) | (bz << 2);
+        const NodeRec r = nodes[v];
+        e.ax[bx] = r.cx, e.ax[2 + by] = r.cy, e.ax[4 + bz] = r.cz;
+        e.first[c] = r.first, e.last[c] = r.last;
+        e.gc[c] = child[v];
+        e.amax[c] = child[v] == 0 ? amax[v] : 0.0f;
     }
+    ext[(fc - 1) >> 3] = e;
 }
 
 template <class T>
@@ -211,17 +234,20 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t M, std::string &err) {
     // leaf-recentred FP32 cloud
     if ((e = ctx->p32.reserve(ctx->n)) != hipSuccess) return e;
     if ((e = ctx->amax.reserve(n_total)) != hipSuccess) return e;
-    if ((e = ctx->nodes32.reserve(n_total)) != hipSuccess) return e;
-    {
-        unsigned nb = (unsigned)((n_total + 255) / 256);
-        nb = nb > 4096 ? 4096 : (nb ? nb : 1);
-        k_nodes32<<<nb, 256, 0, ctx->stream>>>(ctx->nodes.p, n_total, ctx->center[0], ctx->center[1], ctx->center[2],
-                                               ctx->nodes32.p);
-    }
+
     if (n_total) {
         size_t threads = (size_t)n_total * 64;
         unsigned nb = (unsigned)((threads + 255) / 256);
         k_recentre<<<nb, 256, 0, ctx->stream>>>(ctx->p64.p, ctx->nodes.p, ctx->child.p, n_total, ctx->p32.p, ctx->amax.p);
+    }
+    // children regrouped by corner bits, for the count-mode searches
+    if ((e = ctx->next.reserve(n_internal ? n_internal : 1)) != hipSuccess) return e;
+    if (n_internal) {
+        LevelTable lt;
+        for (uint32_t l = 0; l < kMaxDepth + 3; l++) lt.start[l] = ctx->level_start[l];
+        unsigned nb = (unsigned)((n_total + 255) / 256);
+        k_node_ext<<<nb, 256, 0, ctx->stream>>>(ctx->nodes.p, ctx->nkey.p, ctx->child.p, ctx->amax.p, n_total, ctx->encoder, lt,
+                                                ctx->next.p);
     }
     stage_end(ctx, ST_BUILD);
     return hipGetLastError();

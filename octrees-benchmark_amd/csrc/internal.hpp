@@ -23,13 +23,19 @@ struct alignas(32) NodeRec {
     uint32_t first, last;
 };
 
-// FP32 node record for the FP32 searches: centre relative to the box centre (rounded once from the
-// FP64 difference) and the point range. 32 bytes.
-struct alignas(32) NodeRec32 {
-    float cx, cy, cz;
-    uint32_t first, last;
-    uint32_t pad[3];
+// Children of one internal node, in "combo" order c = bx | by << 1 | bz << 2, where (bx, by, bz) are
+// the child's corner bits at its level. The 8 children take only two centre values per axis, so
+// ax holds them: child c's centre is (ax[bx], ax[2 + by], ax[4 + bz]), bit-identical to its own
+// FP64 centre (same corner integers, same formula). Indexed by (first child - 1) / 8 of the node.
+struct alignas(64) NodeExt {
+    double ax[6];       // x lo / hi, y lo / hi, z lo / hi centre coordinates of the children
+    float amax[8];      // combo c: max |a|^2 of the leaf's recentred FP32 points (leaves only)
+    uint32_t first[8];  // combo c: point range [first, last)
+    uint32_t last[8];
+    uint32_t gc[8];     // combo c: its first child, 0 = leaf
+    uint32_t pad[4];
 };
+static_assert(sizeof(NodeExt) == 192, "NodeExt is three 64-byte lines");
 
 // Device buffer that only grows.
 template <class T>
@@ -97,7 +103,7 @@ struct oct_ctx {
     oct::DevBuf<uint32_t> flags;    // per-node split flags (build scratch)
     oct::DevBuf<uint32_t> nrank;    // per-node rank among split nodes of its level (build scratch)
     oct::DevBuf<float> amax;        // per-leaf max squared norm of its recentred FP32 points
-    oct::DevBuf<oct::NodeRec32> nodes32;  // FP32 node centres relative to the box centre
+    oct::DevBuf<oct::NodeExt> next;  // per internal node: its children in combo order
     oct::DevBuf<uint32_t> scan;     // scan scratch
 
     // searches
@@ -114,8 +120,9 @@ struct oct_ctx {
     size_t last_nq = 0;
     int last_algo = 0;
     int group_override = -1;        // OCT_TUNE_RADIUS_GROUP (-1: use tuned / default)
-    int use_mx = 1;                 // OCT_TUNE_RADIUS_MFMA
+    int mx_override = -1;           // OCT_TUNE_RADIUS_MFMA (-1: use tuned / default)
     int group_tuned[4][4][2];       // autotuned group per (kernel, algo, precision), -1 = not tuned
+    int mx_tuned[4][4][2];          // autotuned point-test unit (1 = matrix cores), -1 = not tuned
     bool has_lists = false;
 
     // kNN
@@ -140,7 +147,7 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t max_points_leaf, std::string &err
 struct RadiusLaunch {
     int kernel = 1, algo = 3, precision = 0, mode = 0;
     int group = 16;  // count mode: queries per wave (16, 32, 64), or 0 for the one-lane-per-query kernel
-    bool mx = true;  // count mode, FP32 sphere: point tests on the matrix cores
+    bool mx = false;  // count mode, FP32 sphere: point tests on the matrix cores
     const uint32_t *order = nullptr;
     const uint32_t *qidx = nullptr;
     const double *qxyz = nullptr;

@@ -13,132 +13,25 @@
 // Per-point tests:
 //  - FP64: the reference's isInside expression (kernel_3d.hpp:127-138, 232-242; kernel_2d.hpp:53-63,
 //    153-162), evaluated as written (no contraction).
-//  - FP32_RECHECK (sphere, circle): leaf-recentred FP32 coordinates a (w = |a|^2 precomputed) and
-//    b = fl32(q - c_leaf); M = fl32(R - |b|^2); u = fma(ax, -2bx, fma(ay, -2by, fma(az, -2bz, w - M)))
-//    (circle: fma(ax, ax - 2bx, fma(ay, ay - 2by, -M))) approximates |p - q|^2 - R. Every rounding of
-//    this evaluation and of the reference's FP64 one is bounded by
-//    hw = 2^-20 (amax + |b|^2 + R)(1 + 2^-10) + |M - (R - |b|^2)|, amax = max |a|^2 over the leaf
-//    (u32 = 2^-24: |error| <= u32 (13.01 amax + 12.01 |b|^2 + 4 R) + |M - thr| + O(u64)), so
-//    u < -hw is inside, u > hw outside, and points with |u| <= hw are re-evaluated with the FP64
-//    expression. In count mode a lane re-counts a whole leaf in FP64 if any point fell in the band.
-//    A leaf/query pair whose FP32 terms could overflow (2 amax + 3 |b|^2 + 2 R >= 1e38, or a
-//    non-finite FP32 point, which sets amax = inf) is decided in FP64 throughout.
-//  - FP32_RECHECK sphere, count mode: the same u evaluated on the matrix cores, 16 points x 16
-//    queries per v_mfma_f32_16x16x4_f32 (see leaf_mx in k_count for its band).
-//    Cube/square tests are comparisons only and run on the FP64 coordinates in both modes.
-#include "device.hpp"
+//  - FP32_RECHECK (sphere, circle), list kernel k_radius: leaf-recentred FP32 coordinates a
+//    (w = |a|^2 precomputed) and b = fl32(q - c_leaf); M = fl32(R - |b|^2);
+//    u = fma(ax, -2bx, fma(ay, -2by, fma(az, -2bz, w - M))) (circle: fma(ax, ax - 2bx,
+//    fma(ay, ay - 2by, -M))) approximates |p - q|^2 - R. Every rounding of this evaluation and of
+//    the reference's FP64 one is bounded by hw = 2^-20 (amax + |b|^2 + R)(1 + 2^-10) + |M - thr|,
+//    amax = max |a|^2 over the leaf (u32 = 2^-24: |error| <= u32 (13.01 amax + 12.01 |b|^2 + 4 R)
+//    + |M - thr| + O(u64)), so u < -hw is inside, u > hw outside, and points with |u| <= hw are
+//    re-evaluated with the FP64 expression. In count mode a lane re-counts a whole leaf in FP64 if
+//    any point fell in the band.
+//  - FP32_RECHECK, count kernel k_count: the same u with the band folded in, t = u + hw, from an
+//    FP32 set-up and a wider band hw = 2^-18 (amax + |b|^2 + R) that also covers any summation
+//    order, so the matrix-core tiles (16 points x 16 queries per v_mfma_f32_16x16x4_f32, sphere)
+//    and the vector path share one bound (see leaf_mx in k_count); counts come from sign bits.
+//  - A leaf/query pair whose FP32 terms could overflow, or a leaf holding a non-finite FP32 point
+//    (then amax = inf), is decided in FP64 throughout.
+//  - Cube/square tests are comparisons only and run on the FP64 coordinates in both modes.
+#include "search_common.hpp"
 
 namespace oct {
-
-constexpr int kSearchBlock = 256;
-constexpr int kWaves = kSearchBlock / 64;
-constexpr int kStack = 8 * 22 + 8;
-
-enum { J_OUTSIDE = 0, J_OVERLAP = 1, J_INSIDE = 2 };
-
-struct SearchArgs {
-    const double4 *__restrict__ p64;
-    const float4 *__restrict__ p32;
-    const float *__restrict__ amax;
-    const NodeRec *__restrict__ nodes;
-    const uint32_t *__restrict__ child;
-    const uint32_t *__restrict__ order;
-    const uint32_t *__restrict__ qidx;
-    const double *__restrict__ qxyz;
-    const double *__restrict__ qrad;
-    double radius;
-    const double *__restrict__ qrad3;
-    double r3[3];
-    int use_r3;
-    uint32_t nq;
-    uint32_t *__restrict__ counts;
-    uint32_t *__restrict__ ecounts;
-    const uint64_t *__restrict__ offsets;
-    uint32_t *__restrict__ entries;
-    unsigned long long *__restrict__ stats;
-    double rad[22][3];
-};
-
-struct Query {
-    double x, y, z, r, R;
-    double lox, loy, loz, hix, hiy, hiz;  // kernel box: center -/+ radius (kernel_abstract.hpp:41-45)
-};
-
-__device__ inline double dmax0(double a) { return (a < 0.0) ? 0.0 : a; }  // std::max(a, 0.0)
-
-// Cube / square kernels built from a Vector of half extents (KernelAbstract(center, radii) and
-// makeBox, kernel_abstract.hpp:39-45): box = centre -/+ radii per axis.
-__device__ inline void box_radii(const SearchArgs &a, bool valid, uint32_t qid, Query &q) {
-    if (!a.use_r3) return;
-    double rx = a.r3[0], ry = a.r3[1], rz = a.r3[2];
-    if (a.qrad3 && valid) rx = a.qrad3[3 * (size_t)qid], ry = a.qrad3[3 * (size_t)qid + 1], rz = a.qrad3[3 * (size_t)qid + 2];
-    q.lox = q.x - rx, q.loy = q.y - ry, q.loz = q.z - rz;
-    q.hix = q.x + rx, q.hiy = q.y + ry, q.hiz = q.z + rz;
-}
-
-// boxIntersect(center, Vector radii) of the four kernels (kernel_3d.hpp:181-223, 277-302;
-// kernel_2d.hpp:107-144, 195-218)
-template <int KERNEL>
-__device__ inline int classify64(const Query &q, double cx, double cy, double cz, double rx, double ry, double rz) {
-    if (KERNEL == OCT_KERNEL_SPHERE) {
-        double dx = fabs(q.x - cx), dy = fabs(q.y - cy), dz = fabs(q.z - cz);
-        const double mdx = q.r + rx, mdy = q.r + ry, mdz = q.r + rz;
-        if (dx > mdx || dy > mdy || dz > mdz) return J_OUTSIDE;
-        double ex = dx + rx, ey = dy + ry, ez = dz + rz;
-        if ((ex * ex + ey * ey + ez * ez) <= q.R) return J_INSIDE;
-        int ni = (dx < rx) + (dy < ry) + (dz < rz);
-        if (ni > 1) return J_OVERLAP;
-        dx = dmax0(dx - rx), dy = dmax0(dy - ry), dz = dmax0(dz - rz);
-        if ((dx * dx + dy * dy + dz * dz) <= q.R) return J_OVERLAP;
-        return J_OUTSIDE;
-    } else if (KERNEL == OCT_KERNEL_CIRCLE) {
-        double dx = fabs(q.x - cx), dy = fabs(q.y - cy);
-        const double mdx = q.r + rx, mdy = q.r + ry;
-        if (dx > mdx || dy > mdy) return J_OUTSIDE;
-        double ex = dx + rx, ey = dy + ry;
-        if ((ex * ex + ey * ey) <= q.R) return J_INSIDE;
-        int ni = (dx < rx) + (dy < ry);
-        if (ni > 1) return J_OVERLAP;
-        dx = dmax0(dx - rx), dy = dmax0(dy - ry);
-        if ((dx * dx + dy * dy) <= q.R) return J_OVERLAP;
-        return J_OUTSIDE;
-    } else if (KERNEL == OCT_KERNEL_CUBE) {
-        const double hx = cx + rx, lx = cx - rx, hy = cy + ry, ly = cy - ry, hz = cz + rz, lz = cz - rz;
-        if (hx < q.lox || hy < q.loy || hz < q.loz || lx > q.hix || ly > q.hiy || lz > q.hiz) return J_OUTSIDE;
-        if (hx <= q.hix && hy <= q.hiy && hz <= q.hiz && lx >= q.lox && ly >= q.loy && lz >= q.loz) return J_INSIDE;
-        return J_OVERLAP;
-    } else {
-        const double hx = cx + rx, lx = cx - rx, hy = cy + ry, ly = cy - ry;
-        if (hx < q.lox || hy < q.loy || lx > q.hix || ly > q.hiy) return J_OUTSIDE;
-        if (hx <= q.hix && hy <= q.hiy && lx >= q.lox && ly >= q.loy) return J_INSIDE;
-        return J_OVERLAP;
-    }
-}
-
-// boxOverlap(center, Vector radii): Kernel3D (kernel_3d.hpp:100-115) / Kernel2D (kernel_2d.hpp:29-41)
-template <int KERNEL>
-__device__ inline bool overlap64(const Query &q, double cx, double cy, double cz, double rx, double ry, double rz) {
-    constexpr bool is3d = (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CUBE);
-    if (cx + rx < q.lox || cy + ry < q.loy || (is3d && cz + rz < q.loz)) return false;
-    if (cx - rx > q.hix || cy - ry > q.hiy || (is3d && cz - rz > q.hiz)) return false;
-    return true;
-}
-
-// isInside (kernel_3d.hpp:133-138, 238-242; kernel_2d.hpp:59-63, 159-162; util.hpp:133-137)
-template <int KERNEL>
-__device__ inline bool inside64(const Query &q, double px, double py, double pz) {
-    if (KERNEL == OCT_KERNEL_SPHERE) {
-        double d1 = px - q.x, d2 = py - q.y, d3 = pz - q.z;
-        return d1 * d1 + d2 * d2 + d3 * d3 < q.R;
-    } else if (KERNEL == OCT_KERNEL_CIRCLE) {
-        double d1 = px - q.x, d2 = py - q.y;
-        return d1 * d1 + d2 * d2 <= q.R;
-    } else if (KERNEL == OCT_KERNEL_CUBE) {
-        return px >= q.lox && px <= q.hix && py >= q.loy && py <= q.hiy && pz >= q.loz && pz <= q.hiz;
-    } else {
-        return px >= q.lox && px <= q.hix && py >= q.loy && py <= q.hiy;
-    }
-}
 
 // Per-lane output sink. MODE 0: neighbour counts only; 1: counts + list-entry counts; 2: fill lists.
 template <int ALGO, int MODE>
@@ -416,16 +309,15 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
 // ------------------------------------------------------------------ count-mode kernel
 // Per-query neighbour counts only (the harness keeps result.size(), neighbor_benchmarks.hpp:509).
 // G queries per wave, S = 64 / G lanes per query: lane = g * G + j serves query j with "group" g.
-//  - the 8 children of a popped node are classified S at a time (group g takes octant k*S + g), so
-//    no classification is duplicated; results are consumed in octant order as in singleTraversal;
+//  - the 8 children of a popped node are judged together from per-axis terms (judge8 below);
 //  - a leaf's points are staged once per wave in LDS and split across the S lanes of each query
 //    (lane group g takes points g, g + S, ...); each staged chunk is padded with sentinel points
-//    that evaluate to +inf (never inside, never in the FP32 band), so every lane runs the same
-//    4x-unrolled trip count with immediate LDS offsets; partial counts are summed at the end;
-//  - a smaller G shrinks the union of the wave's traversals (fewer masked-off lane slots).
-// Empty children are skipped: they add nothing to a count whatever their class.
-constexpr int kStageBytes = 2048;  // per-wave LDS staging (128 FP32 or 64 FP64 points) + padding
-constexpr int kUnroll = 4;
+//    that are never inside and never in the FP32 band, so every lane runs the same 4x-unrolled
+//    trip count with immediate LDS offsets; partial counts are summed at the end;
+//  - a smaller G shrinks the union of the wave's traversals (fewer masked-off lane slots), a
+//    larger one shares each judgement of 8 children among fewer lanes.
+// Empty leaves are skipped: they add nothing to a count whatever their class. Counts do not
+// depend on the order of the traversal, which is therefore free.
 
 template <int KERNEL, int ALGO, int PREC, int G, bool STATS, bool MX>
 __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_count(const SearchArgs a) {
@@ -470,6 +362,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     q.lox = q.x - q.r, q.loy = q.y - q.r, q.loz = q.z - q.r;
     q.hix = q.x + q.r, q.hiy = q.y + q.r, q.hiz = q.z + q.r;
     if (KERNEL == OCT_KERNEL_CUBE || KERNEL == OCT_KERNEL_SQUARE) box_radii(a, valid, qid, q);
+    const float qRf = (float)q.R;
 
     uint32_t cnt = 0;
     uint64_t n_class = 0, n_test = 0, n_recheck = 0, n_cslots = 0, n_tslots = 0;
@@ -479,6 +372,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     // of this wave. mq[h] is the centre and squared radius of query 16h + c16.
     const uint32_t c16 = lane & 15, kk = lane >> 4;
     double mqx[NH], mqy[NH], mqz[NH], mqR[NH];
+    float mqRf[NH];
     uint32_t mcnt[NH];
 #pragma unroll
     for (int h = 0; h < NH; h++) {
@@ -498,6 +392,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
                 mqR[h] = r * r;
             }
         }
+        mqRf[h] = (float)mqR[h];
     }
 
     auto judge = [&](bool act, const NodeRec &nr, uint32_t d) -> int {
@@ -513,7 +408,11 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         const uint32_t slots = iters * S;
         for (uint32_t t = lane; t < slots; t += 64) {
             if (kFp32) {
-                sp32[t] = t < P ? a.p32[base + t] : make_float4(1e30f, 1e30f, 1e30f, __builtin_huge_valf());
+                // sentinel: t = 1e30 + C (sphere: w = 1e30) or +inf (circle: x = y = 1e30), never
+                // inside and above the band
+                const float4 sent = KERNEL == OCT_KERNEL_CIRCLE ? make_float4(1e30f, 1e30f, 0.0f, 0.0f)
+                                                                : make_float4(0.0f, 0.0f, 0.0f, 1e30f);
+                sp32[t] = t < P ? a.p32[base + t] : sent;
             } else {
                 sp64[t] = t < P ? a.p64[base + t] : make_double4(1e300, 1e300, 1e300, 0.0);
             }
@@ -526,18 +425,22 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
 
     // Matrix-core leaf (FP32 sphere): 16 points x 16 queries per v_mfma_f32_16x16x4_f32,
     // D = A B + C with A = the points' [ax, ay, az, w] rows, B = the queries' [-2b; 1] columns and
-    // C = fl32(hw - thr) per column, so D approximates u + hw with u = |p - q|^2 - R.
-    // The tile's summation order is not specified, so the error is bounded for any association with
-    // every product and sum rounded (to nearest or toward zero), plus the rounding of a, b, w and C:
-    // |D - (u + hw)| <= 2^-24 (25 amax + 25 |b|^2 + 11 R + hw) < hw = 2^-19 (amax + |b|^2 + R)(1 + 2^-10).
-    // Hence D <= 0 (sign bit set, -0 included) is inside, D > 2 hw is outside, and 0 <= D <= 2 hw -
-    // an unsigned compare of the bit patterns, which also sends +0 there - is re-tested in FP64.
-    // Rows past the leaf are sentinels (0, 0, 0, 1e30): D ~ 1e30, positive. A leaf/query pair whose
-    // terms could overflow FP32 (2 amax + 3 |b|^2 + 2 R >= 1e38) is re-tested in FP64 outright.
-    auto leaf_mx = [&](uint64_t m, uint32_t c, const NodeRec &nr, uint32_t first, uint32_t last) {
+    // C = fl32(fl32(hw - fl32(R)) + fl32(|b|^2)) per column, so D approximates u + hw with
+    // u = |p - q|^2 - R. Only b = fl32(q - c) is formed in FP64; the rest of the set-up is FP32.
+    // The tile's summation order is not specified, so its error is bounded for any association
+    // with every product and sum rounded to nearest or toward zero (u' = 2^-23, 5 terms); with the
+    // representation errors of a, b, w and the FP32 set-up (unit u = 2^-24):
+    //   |D - (u + hw)| <= u (25 amax + 28 |b|^2 + 13 R + 12 hw) < 32 u (amax + |b|^2 + R).
+    // hw = 2^-18 fl32(fl32(amax + fl32|b|^2) + fl32 R) >= 2 * 32 u (amax + |b|^2 + R)(1 - 5u) covers
+    // it with a factor-2 margin. Hence D <= 0 (sign bit set, -0 included) is inside, D > 2 hw is
+    // outside, and 0 <= D <= 2 hw - an unsigned compare of the bit patterns, which also sends +0
+    // there - is re-tested in FP64. Rows past the leaf are sentinels (0, 0, 0, 1e30): D ~ 1e30,
+    // positive and above 2 hw. A leaf/query pair with amax + |b|^2 + R >= 1e30 (or not finite: a
+    // non-finite point makes amax = inf) is re-tested in FP64 outright, so no term can overflow.
+    // The four blocks of a group are multiplied back to back and then counted.
+    auto leaf_mx = [&](uint64_t m, float amf, const NodeRec &nr, uint32_t first, uint32_t last) {
         typedef float f4 __attribute__((ext_vector_type(4)));
         const float *pf = reinterpret_cast<const float *>(a.p32);
-        const double am = (double)a.amax[c];
         const uint32_t nblk = (last - first + 15) >> 4;
         float bop[NH], ci[NH];
         uint32_t lim[NH], mnb[NH], c8[NH];
@@ -552,12 +455,13 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
             force[h] = false;
             if (on[h]) {
                 const float bx = (float)(mqx[h] - nr.cx), by = (float)(mqy[h] - nr.cy), bz = (float)(mqz[h] - nr.cz);
-                const double b2 = (double)bx * (double)bx + (double)by * (double)by + (double)bz * (double)bz;
-                const double thr = mqR[h] - b2;
-                const float hw = __double2float_ru(1.9073486328125e-06 * (am + b2 + mqR[h]) * 1.0009765625 + 1e-30);
-                ci[h] = (float)((double)hw - thr);
+                const float b2 = __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz));
+                const float Rf = mqRf[h];
+                const float sum = (amf + b2) + Rf;
+                const float hw = 3.814697265625e-06f * sum;  // 2^-18
+                ci[h] = (hw - Rf) + b2;
                 lim[h] = __float_as_uint(2.0f * hw);
-                force[h] = !(2.0 * am + 3.0 * b2 + 2.0 * mqR[h] < 1e38);
+                force[h] = !(sum < 1e30f);
                 bop[h] = kk == 0 ? -2.0f * bx : kk == 1 ? -2.0f * by : kk == 2 ? -2.0f * bz : 1.0f;
             }
         }
@@ -569,21 +473,23 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
                 av[t] = row < last ? pf[4 * (size_t)row + kk] : (kk == 3 ? 1e30f : 0.0f);
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                if (b0 + t < nblk) {
+            for (int h = 0; h < NH; h++) {
+                if (!on[h]) continue;
+                const f4 cv = {ci[h], ci[h], ci[h], ci[h]};
+                f4 d[4];
 #pragma unroll
-                    for (int h = 0; h < NH; h++) {
-                        if (on[h]) {
-                            const f4 cv = {ci[h], ci[h], ci[h], ci[h]};
-                            const f4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bop[h], cv, 0, 0, 0);
-                            const uint32_t d0 = __float_as_uint(d[0]), d1 = __float_as_uint(d[1]);
-                            const uint32_t d2 = __float_as_uint(d[2]), d3 = __float_as_uint(d[3]);
-                            // v_perm selector 9 / 11: byte = 8 copies of bit 31 of src1 / src0
-                            c8[h] += __builtin_popcount(__builtin_amdgcn_perm(d0, d1, 0x0c0c0b09u));
-                            c8[h] += __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u));
-                            mnb[h] = min(mnb[h], min(d0, d1));
-                            mnb[h] = min(mnb[h], min(d2, d3));
-                        }
+                for (int t = 0; t < 4; t++)
+                    if (b0 + t < nblk) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bop[h], cv, 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (b0 + t < nblk) {
+                        const uint32_t d0 = __float_as_uint(d[t][0]), d1 = __float_as_uint(d[t][1]);
+                        const uint32_t d2 = __float_as_uint(d[t][2]), d3 = __float_as_uint(d[t][3]);
+                        // v_perm selector 9 / 11: a byte of 8 copies of bit 31 of src1 / src0
+                        c8[h] = __builtin_popcount(__builtin_amdgcn_perm(d0, d1, 0x0c0c0b09u)) + c8[h];
+                        c8[h] = __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u)) + c8[h];
+                        mnb[h] = min(min(mnb[h], d0), d1);
+                        mnb[h] = min(min(mnb[h], d2), d3);
                     }
                 }
             }
@@ -614,7 +520,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     };
 
     // leaf: queries in m count the leaf's points; lane group g takes points g, g + S, ...
-    auto leaf = [&](uint64_t m, uint32_t c, const NodeRec &nr) {
+    auto leaf = [&](uint64_t m, float am, const NodeRec &nr) {
         const uint32_t first = nr.first, last = nr.last;
         const bool act = (m & qbit) != 0 && valid;
         if (STATS) {
@@ -622,52 +528,53 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
             if (!MX) n_tslots += 64ull * ((last - first + S - 1) / S);
         }
         if constexpr (MX) {
-            leaf_mx(m, c, nr, first, last);
+            leaf_mx(m, am, nr, first, last);
             return;
         }
         uint32_t lc = 0;
         if (kFp32) {
+            // t = w - 2 a.b + C approximates u + hw (the band of leaf_mx, same set-up): t <= 0 (sign
+            // bit set) is inside, 0 <= t <= 2 hw (unsigned compare of the bit patterns) is re-tested
             const float bx = (float)(q.x - nr.cx), by = (float)(q.y - nr.cy), bz = (float)(q.z - nr.cz);
-            double b2 = (double)bx * (double)bx + (double)by * (double)by;
-            if (KERNEL == OCT_KERNEL_SPHERE) b2 += (double)bz * (double)bz;
-            const double thr = q.R - b2;
-            const float M = (float)thr;
-            const double delta = 9.5367431640625e-07 * ((double)a.amax[c] + b2 + q.R) * 1.0009765625 + 1e-30;
-            const float hw = __double2float_ru(delta + fabs((double)M - thr)), nhw = -hw;
-            const bool force = !(2.0 * (double)a.amax[c] + 3.0 * b2 + 2.0 * q.R < 1e38);  // FP32 terms could overflow
+            const float b2 = KERNEL == OCT_KERNEL_SPHERE ? __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz))
+                                                         : __builtin_fmaf(bx, bx, by * by);
+            const float sum = (am + b2) + qRf;
+            const float hw = 3.814697265625e-06f * sum;  // 2^-18
+            const float C = (hw - qRf) + b2;
+            const uint32_t lim = __float_as_uint(2.0f * hw);
+            const bool force = !(sum < 1e30f);
             const float ux = -2.0f * bx, uy = -2.0f * by, uz = -2.0f * bz;
-            float mn = __builtin_huge_valf();
+            uint32_t c8 = 0, mnb = 0xffffffffu;
             for (uint32_t base = first; base < last; base += CAP) {
                 const uint32_t P = (last - base) < (uint32_t)CAP ? (last - base) : (uint32_t)CAP;
                 const uint32_t iters = stage(base, P);
                 const float4 *pl = sp32 + g;
-                uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
                 for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * S) {
                     const float4 p0 = pl[0], p1 = pl[S], p2 = pl[2 * S], p3 = pl[3 * S];
-                    float u0, u1, u2, u3;
+                    float t0, t1, t2, t3;
                     if (KERNEL == OCT_KERNEL_SPHERE) {
-                        u0 = __builtin_fmaf(p0.x, ux, __builtin_fmaf(p0.y, uy, __builtin_fmaf(p0.z, uz, p0.w - M)));
-                        u1 = __builtin_fmaf(p1.x, ux, __builtin_fmaf(p1.y, uy, __builtin_fmaf(p1.z, uz, p1.w - M)));
-                        u2 = __builtin_fmaf(p2.x, ux, __builtin_fmaf(p2.y, uy, __builtin_fmaf(p2.z, uz, p2.w - M)));
-                        u3 = __builtin_fmaf(p3.x, ux, __builtin_fmaf(p3.y, uy, __builtin_fmaf(p3.z, uz, p3.w - M)));
+                        t0 = __builtin_fmaf(p0.x, ux, __builtin_fmaf(p0.y, uy, __builtin_fmaf(p0.z, uz, p0.w + C)));
+                        t1 = __builtin_fmaf(p1.x, ux, __builtin_fmaf(p1.y, uy, __builtin_fmaf(p1.z, uz, p1.w + C)));
+                        t2 = __builtin_fmaf(p2.x, ux, __builtin_fmaf(p2.y, uy, __builtin_fmaf(p2.z, uz, p2.w + C)));
+                        t3 = __builtin_fmaf(p3.x, ux, __builtin_fmaf(p3.y, uy, __builtin_fmaf(p3.z, uz, p3.w + C)));
                     } else {
-                        u0 = __builtin_fmaf(p0.x, p0.x + ux, __builtin_fmaf(p0.y, p0.y + uy, -M));
-                        u1 = __builtin_fmaf(p1.x, p1.x + ux, __builtin_fmaf(p1.y, p1.y + uy, -M));
-                        u2 = __builtin_fmaf(p2.x, p2.x + ux, __builtin_fmaf(p2.y, p2.y + uy, -M));
-                        u3 = __builtin_fmaf(p3.x, p3.x + ux, __builtin_fmaf(p3.y, p3.y + uy, -M));
+                        t0 = __builtin_fmaf(p0.x, p0.x + ux, __builtin_fmaf(p0.y, p0.y + uy, C));
+                        t1 = __builtin_fmaf(p1.x, p1.x + ux, __builtin_fmaf(p1.y, p1.y + uy, C));
+                        t2 = __builtin_fmaf(p2.x, p2.x + ux, __builtin_fmaf(p2.y, p2.y + uy, C));
+                        t3 = __builtin_fmaf(p3.x, p3.x + ux, __builtin_fmaf(p3.y, p3.y + uy, C));
                     }
-                    c0 += (u0 < nhw) ? 1u : 0u;
-                    c1 += (u1 < nhw) ? 1u : 0u;
-                    c2 += (u2 < nhw) ? 1u : 0u;
-                    c3 += (u3 < nhw) ? 1u : 0u;
-                    mn = fminf(mn, fminf(fabsf(u0), fabsf(u1)));
-                    mn = fminf(mn, fminf(fabsf(u2), fabsf(u3)));
+                    const uint32_t d0 = __float_as_uint(t0), d1 = __float_as_uint(t1);
+                    const uint32_t d2 = __float_as_uint(t2), d3 = __float_as_uint(t3);
+                    c8 = __builtin_popcount(__builtin_amdgcn_perm(d0, d1, 0x0c0c0b09u)) + c8;
+                    c8 = __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u)) + c8;
+                    mnb = min(min(mnb, d0), d1);
+                    mnb = min(min(mnb, d2), d3);
                 }
-                lc += (c0 + c1) + (c2 + c3);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
-            const bool band = act && (!(mn > hw) || force);
+            lc = c8 >> 3;
+            const bool band = act && (mnb <= lim || force);
             if (ballot64(band)) {  // rare: exact FP64 re-count of this lane's share of the leaf
                 if (band) {
                     lc = 0;
@@ -699,9 +606,73 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         if (act) cnt += lc;
     };
 
+    // The 8 children of a popped node are judged from per-axis terms: they take two centre values
+    // per axis (NodeExt), so |q - c|, the kernel-box bounds and the squared extents are computed
+    // once per axis value and combined per child, in the reference's operation order, e.g. for the
+    // sphere (ex*ex + ey*ey) + ez*ez with ex = |q.x - cx| + rx (kernel_3d.hpp:181-223). Each lane
+    // judges all 8 children for its own query (the S lanes of a query agree); results are lane
+    // masks, so the per-child decisions are scalar-unit work.
+    auto judge8 = [&](const NodeExt &E, uint32_t d, bool act, bool (&in)[8], bool (&ov)[8]) {
+        const double rx = a.rad[d][0], ry = a.rad[d][1], rz = a.rad[d][2];
+        constexpr bool is3d = (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CUBE);
+        if (ALGO == OCT_ALGO_NEIGHBORS || KERNEL == OCT_KERNEL_CUBE || KERNEL == OCT_KERNEL_SQUARE) {
+            // boxOverlap (kernel_3d.hpp:100-115, kernel_2d.hpp:29-41) and the box kernels'
+            // boxIntersect (kernel_3d.hpp:277-302, kernel_2d.hpp:195-218): per-axis comparisons
+            bool ox[2], oy[2], oz[2], ix[2], iy[2], iz[2];
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const double hx = E.ax[b] + rx, lx = E.ax[b] - rx, hy = E.ax[2 + b] + ry, ly = E.ax[2 + b] - ry;
+                const double hz = E.ax[4 + b] + rz, lz = E.ax[4 + b] - rz;
+                ox[b] = hx < q.lox || lx > q.hix;
+                oy[b] = hy < q.loy || ly > q.hiy;
+                oz[b] = is3d && (hz < q.loz || lz > q.hiz);
+                ix[b] = hx <= q.hix && lx >= q.lox;
+                iy[b] = hy <= q.hiy && ly >= q.loy;
+                iz[b] = !is3d || (hz <= q.hiz && lz >= q.loz);
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+                const bool out = ox[bx] || oy[by] || oz[bz];
+                const bool inside = ALGO != OCT_ALGO_NEIGHBORS && !out && ix[bx] && iy[by] && iz[bz];
+                in[c] = act && inside;
+                ov[c] = act && !out && !inside;
+            }
+        } else {
+            // sphere / circle boxIntersect (kernel_3d.hpp:181-223, kernel_2d.hpp:107-144)
+            double e2x[2], e2y[2], e2z[2], m2x[2], m2y[2], m2z[2];
+            bool ox[2], oy[2], oz[2], nx[2], ny[2], nz[2];
+            const double mdx = q.r + rx, mdy = q.r + ry, mdz = q.r + rz;
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const double dx = fabs(q.x - E.ax[b]), dy = fabs(q.y - E.ax[2 + b]), dz = fabs(q.z - E.ax[4 + b]);
+                ox[b] = dx > mdx, oy[b] = dy > mdy, oz[b] = is3d && dz > mdz;
+                const double ex = dx + rx, ey = dy + ry, ez = dz + rz;
+                e2x[b] = ex * ex, e2y[b] = ey * ey, e2z[b] = ez * ez;
+                nx[b] = dx < rx, ny[b] = dy < ry, nz[b] = is3d && dz < rz;
+                const double fx = dmax0(dx - rx), fy = dmax0(dy - ry), fz = dmax0(dz - rz);
+                m2x[b] = fx * fx, m2y[b] = fy * fy, m2z[b] = fz * fz;
+            }
+            double sxy[4], txy[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) sxy[k] = e2x[k & 1] + e2y[k >> 1], txy[k] = m2x[k & 1] + m2y[k >> 1];
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+                const bool out = ox[bx] || oy[by] || oz[bz];
+                const double smax = is3d ? sxy[c & 3] + e2z[bz] : sxy[c & 3];
+                const double smin = is3d ? txy[c & 3] + m2z[bz] : txy[c & 3];
+                const bool inside = !out && smax <= q.R;
+                const bool two = is3d ? ((nx[bx] && ny[by]) || (nx[bx] && nz[bz]) || (ny[by] && nz[bz])) : (nx[bx] && ny[by]);
+                in[c] = act && inside;
+                ov[c] = act && !out && !inside && (two || smin <= q.R);
+            }
+        }
+    };
+
     // root (singleTraversal, linear_octree.hpp:655-662); group 0 owns the root's INSIDE count.
-    // Stack entries hold the FIRST CHILD of an OVERLAP internal node (known when it is pushed), so a
-    // pop needs one round of memory traffic: the 8 children's records and child indices together.
+    // Stack entries hold the FIRST CHILD of an OVERLAP internal node; its NodeExt record is one
+    // uniform load.
     uint32_t sp = 0;
     {
         const NodeRec nr = a.nodes[0];
@@ -715,7 +686,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
                 NodeRec ur;
                 ur.cx = nr.cx, ur.cy = nr.cy, ur.cz = nr.cz;
                 ur.first = uniform_u32(nr.first), ur.last = uniform_u32(nr.last);
-                leaf(ov, 0, ur);
+                if (ur.last > ur.first) leaf(ov, a.amax[0], ur);
             } else {
                 s_node[w][0] = c0, s_mask[w][0] = ov, s_depth[w][0] = 0;
                 sp = 1;
@@ -726,40 +697,28 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         sp--;
         const uint32_t fc = uniform_u32(s_node[w][sp]);
         const uint64_t m = uniform_u64(s_mask[w][sp]);
-        const uint32_t d = uniform_u32(s_depth[w][sp]) + 1;
+        const uint32_t d = uniform_u32(s_depth[w][sp]) + 1;  // the children's level
+        const NodeExt &E = a.next[(fc - 1) >> 3];
         const bool inm = (m & qbit) != 0;
-        int jv[8 / S];
-        NodeRec nv[8 / S];
-        uint32_t cv[8 / S];
+        bool in[8], ovl[8];
+        judge8(E, d, inm, in, ovl);
+        if (STATS) n_class += 8ull * __popcll(m), n_cslots += 64ull * 8;
 #pragma unroll
-        for (int k = 0; k < 8 / S; k++) {
-            nv[k] = a.nodes[fc + k * S + g];
-            cv[k] = a.child[fc + k * S + g];
-        }
-#pragma unroll
-        for (int k = 0; k < 8 / S; k++) {
-            const bool act = inm && nv[k].first != nv[k].last;
-            jv[k] = judge(act, nv[k], d);
-            if (jv[k] == J_INSIDE) cnt += nv[k].last - nv[k].first;
-            if (STATS) n_class += __popcll(ballot64(act));
-        }
-        if (STATS) n_cslots += 64ull * (8 / S);
-#pragma unroll
-        for (int oct = 0; oct < 8; oct++) {
-            const uint64_t ov = (ballot64(jv[oct / S] == J_OVERLAP) >> ((oct % S) * G)) & gmask;
+        for (int c = 0; c < 8; c++) {
+            const uint32_t f = E.first[c], l = E.last[c];
+            if (g == 0 && in[c]) cnt += l - f;
+            const uint64_t ov = ballot64(ovl[c]) & gmask;
             if (ov) {
-                const int src = (oct % S) * G;  // a lane of the group that loaded child oct
-                const uint32_t cc = readlane_u32(cv[oct / S], src);
-                if (cc == 0) {
-                    NodeRec ur;
-                    ur.cx = readlane_f64(nv[oct / S].cx, src);
-                    ur.cy = readlane_f64(nv[oct / S].cy, src);
-                    ur.cz = readlane_f64(nv[oct / S].cz, src);
-                    ur.first = readlane_u32(nv[oct / S].first, src);
-                    ur.last = readlane_u32(nv[oct / S].last, src);
-                    leaf(ov, fc + oct, ur);
+                const uint32_t gc = E.gc[c];
+                if (gc == 0) {
+                    if (l > f) {  // an empty leaf adds nothing to a count
+                        NodeRec ur;
+                        ur.cx = E.ax[c & 1], ur.cy = E.ax[2 + ((c >> 1) & 1)], ur.cz = E.ax[4 + ((c >> 2) & 1)];
+                        ur.first = f, ur.last = l;
+                        leaf(ov, E.amax[c], ur);
+                    }
                 } else if (sp < kStack) {
-                    s_node[w][sp] = cc, s_mask[w][sp] = ov, s_depth[w][sp] = (uint8_t)d;
+                    s_node[w][sp] = gc, s_mask[w][sp] = ov, s_depth[w][sp] = (uint8_t)d;
                     sp++;
                 }
             }
@@ -863,6 +822,7 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     a.p32 = ctx->p32.p;
     a.amax = ctx->amax.p;
     a.nodes = ctx->nodes.p;
+    a.next = ctx->next.p;
     a.child = ctx->child.p;
     a.order = L.order;
     a.qidx = L.qidx;

@@ -24,7 +24,8 @@ EXPORTS = (
     "oct_create", "oct_destroy", "oct_last_error", "oct_synchronize", "oct_version", "oct_set_points",
     "oct_set_points_dev", "oct_bbox", "oct_encode", "oct_sort", "oct_sort_points", "oct_set_sorted", "oct_build",
     "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_stage_times",
-    "oct_set_tuning", "oct_autotune_radius", "oct_radius_box", "oct_encode_coords", "oct_decode_keys",
+    "oct_set_tuning", "oct_autotune_radius", "oct_tuned_radius", "oct_radius_box", "oct_encode_coords",
+    "oct_decode_keys",
 )
 
 
@@ -83,6 +84,7 @@ def lib():
         L.oct_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
         L.oct_autotune_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_size_t,
                                           C.POINTER(C.c_int), _vp]
+        L.oct_tuned_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.oct_radius_box.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, C.c_size_t, C.c_int, _vp,
                                      C.POINTER(SearchStats)]
         L.oct_encode_coords.argtypes = [_vp, C.c_int, _vp, C.c_size_t, _vp]
@@ -321,8 +323,15 @@ class Octree:
         self._chk(self.L.oct_set_tuning(self.h, 1, g), "oct_set_tuning")
 
     def set_mfma(self, on):
-        """FP32 sphere count-mode point tests on the matrix cores (True, default) or vector ALUs."""
-        self._chk(self.L.oct_set_tuning(self.h, 2, 1 if on else 0), "oct_set_tuning")
+        """FP32 sphere count-mode point tests on the matrix cores (True), the vector ALUs (False), or
+        autotuned / vector ALUs (None, the default)."""
+        self._chk(self.L.oct_set_tuning(self.h, 2, -1 if on is None else (1 if on else 0)), "oct_set_tuning")
+
+    def tuned_radius(self, kernel, algo, precision):
+        """(queries per wave, matrix cores?) a count-mode search would use now."""
+        g, m = C.c_int(0), C.c_int(0)
+        self._chk(self.L.oct_tuned_radius(self.h, kernel, algo, precision, C.byref(g), C.byref(m)), "oct_tuned_radius")
+        return g.value, bool(m.value)
 
     def autotune_radius(self, kernel, algo, precision, radius, sample_nq=1 << 20):
         best = C.c_int(0)

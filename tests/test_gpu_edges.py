@@ -44,7 +44,7 @@ def check_counts(ctx, kernel, algo, centers, radius, ref_sizes):
                     np.testing.assert_array_equal(counts, ref_sizes, err_msg=f"mx={mx} G={g} prec={prec}")
     finally:
         ctx.set_group(-1)
-        ctx.set_mfma(True)
+        ctx.set_mfma(None)
 
 
 OCT8 = [[0, 0, 0], [0.1, 0, 0], [0, 0.1, 0], [10, 10, 10], [10.1, 10, 10], [10, 10.1, 10], [5, 5, 5], [5, 5, 5.1]]

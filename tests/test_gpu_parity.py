@@ -141,22 +141,22 @@ def test_radius_counts_every_group(case, ctx, group, mx):
                     np.testing.assert_array_equal(counts, ref, err_msg=f"G={group} k={kernel} a={algo} p={prec}")
     finally:
         ctx.set_group(-1)
-        ctx.set_mfma(True)
+        ctx.set_mfma(None)
 
 
-@pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32, "fp32-valu"])
+@pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32, "fp32-mfma"])
 def test_radius_all_points_counts(case, ctx, prec):
     """All-points counts (the harness' -s all mode) against the oracle on a strided sample."""
     name, xyz, enc, ml, (sx, keys, perm, c, r, tree), _ = case
     ensure(ctx, case)
     n = len(xyz)
     say(f"[gpu] all-points {name} prec={prec}")
-    if prec == "fp32-valu":
-        ctx.set_mfma(False)
+    if prec == "fp32-mfma":  # FP32 point tests on the matrix cores
+        ctx.set_mfma(True)
         try:
             counts, st = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, radius=2.5, nq=n)
         finally:
-            ctx.set_mfma(True)
+            ctx.set_mfma(None)
     else:
         counts, st = ctx.radius(A.K_SPHERE, A.A_STRUCT, prec, radius=2.5, nq=n)
     sample = np.arange(0, n, 97)
