@@ -29,6 +29,8 @@
 //  - A leaf/query pair whose FP32 terms could overflow, or a leaf holding a non-finite FP32 point
 //    (then amax = inf), is decided in FP64 throughout.
 //  - Cube/square tests are comparisons only and run on the FP64 coordinates in both modes.
+#include <type_traits>
+
 #include "search_common.hpp"
 
 namespace oct {
@@ -330,6 +332,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     __shared__ uint32_t s_node[kWaves][kStack];
     __shared__ uint64_t s_mask[kWaves][kStack];
     __shared__ uint8_t s_depth[kWaves][kStack];
+    __shared__ uint64_t s_leafm[kWaves][8];  // OVERLAP leaf children of the popped node (query masks)
     __shared__ __attribute__((aligned(16))) unsigned char s_stage[kWaves][kStageBytes + PAD * (kFp32 ? 16 : 32)];
 
     const uint32_t w = threadIdx.x >> 6;
@@ -369,31 +372,21 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
 
     // Matrix-core layout (MX): lane l owns column c16 = l % 16 and row group kk = l / 16 of each
     // 16 x 16 tile D = A B + C of v_mfma_f32_16x16x4_f32; column block h holds queries 16h .. 16h+15
-    // of this wave. mq[h] is the centre and squared radius of query 16h + c16.
+    // of this wave, whose per-leaf terms are computed by the query's own lane (lane 16h + c16) and
+    // fetched with ds_bpermute. mcnt[h] is this lane's share of query 16h + c16's count.
     const uint32_t c16 = lane & 15, kk = lane >> 4;
-    double mqx[NH], mqy[NH], mqz[NH], mqR[NH];
-    float mqRf[NH];
     uint32_t mcnt[NH];
 #pragma unroll
-    for (int h = 0; h < NH; h++) {
-        mqx[h] = mqy[h] = mqz[h] = mqR[h] = 0.0;
-        mcnt[h] = 0;
-        if (MX) {
-            const uint32_t sl = tile * G + 16 * h + c16;
-            if (sl < a.nq) {
-                const uint32_t id = a.order ? a.order[sl] : sl;
-                if (a.qxyz) {
-                    mqx[h] = a.qxyz[3 * (size_t)id], mqy[h] = a.qxyz[3 * (size_t)id + 1], mqz[h] = a.qxyz[3 * (size_t)id + 2];
-                } else {
-                    const double4 p = a.p64[a.qidx ? a.qidx[id] : id];
-                    mqx[h] = p.x, mqy[h] = p.y, mqz[h] = p.z;
-                }
-                const double r = a.qrad ? a.qrad[id] : a.radius;
-                mqR[h] = r * r;
-            }
-        }
-        mqRf[h] = (float)mqR[h];
-    }
+    for (int h = 0; h < NH; h++) mcnt[h] = 0;
+    auto bperm_u32 = [](uint32_t src_lane, uint32_t v) -> uint32_t {
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
+    };
+    auto bperm_f32 = [&](uint32_t src_lane, float v) -> float { return __uint_as_float(bperm_u32(src_lane, __float_as_uint(v))); };
+    auto bperm_f64 = [&](uint32_t src_lane, double v) -> double {
+        const uint64_t b = (uint64_t)__double_as_longlong(v);
+        const uint64_t lo = bperm_u32(src_lane, (uint32_t)b), hi = bperm_u32(src_lane, (uint32_t)(b >> 32));
+        return __longlong_as_double((long long)(lo | (hi << 32)));
+    };
 
     auto judge = [&](bool act, const NodeRec &nr, uint32_t d) -> int {
         if (!act) return -1;
@@ -403,18 +396,42 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     };
 
     // stage points [base, base + P) of the sorted cloud, then sentinels up to a multiple of kUnroll*S
+    // All of a chunk's loads are issued before any LDS write (one round trip); loads read a clamped
+    // in-range point and the sentinel is selected by value.
     auto stage = [&](uint32_t base, uint32_t P) -> uint32_t {
         const uint32_t iters = (P + S * kUnroll - 1) / (S * kUnroll) * kUnroll;  // per-lane iterations
         const uint32_t slots = iters * S;
-        for (uint32_t t = lane; t < slots; t += 64) {
-            if (kFp32) {
-                // sentinel: t = 1e30 + C (sphere: w = 1e30) or +inf (circle: x = y = 1e30), never
-                // inside and above the band
-                const float4 sent = KERNEL == OCT_KERNEL_CIRCLE ? make_float4(1e30f, 1e30f, 0.0f, 0.0f)
-                                                                : make_float4(0.0f, 0.0f, 0.0f, 1e30f);
-                sp32[t] = t < P ? a.p32[base + t] : sent;
-            } else {
-                sp64[t] = t < P ? a.p64[base + t] : make_double4(1e300, 1e300, 1e300, 0.0);
+        constexpr int NI = (CAP + PAD + 63) / 64;  // slots <= CAP + PAD
+        if (kFp32) {
+            // sentinel: t = 1e30 + C (sphere: w = 1e30) or +inf (circle: x = y = 1e30), never
+            // inside and above the band
+            const float4 sent = KERNEL == OCT_KERNEL_CIRCLE ? make_float4(1e30f, 1e30f, 0.0f, 0.0f)
+                                                            : make_float4(0.0f, 0.0f, 0.0f, 1e30f);
+            float4 v[NI];
+#pragma unroll
+            for (int i = 0; i < NI; i++)
+                if (64u * i < slots) v[i] = a.p32[base + min(lane + 64u * i, P - 1)];
+#pragma unroll
+            for (int i = 0; i < NI; i++) {
+                const uint32_t t = lane + 64u * i;
+                if (64u * i < slots && t < slots) {
+                    const bool real = t < P;
+                    sp32[t] = make_float4(real ? v[i].x : sent.x, real ? v[i].y : sent.y, real ? v[i].z : sent.z,
+                                          real ? v[i].w : sent.w);
+                }
+            }
+        } else {
+            double4 v[NI];
+#pragma unroll
+            for (int i = 0; i < NI; i++)
+                if (64u * i < slots) v[i] = a.p64[base + min(lane + 64u * i, P - 1)];
+#pragma unroll
+            for (int i = 0; i < NI; i++) {
+                const uint32_t t = lane + 64u * i;
+                if (64u * i < slots && t < slots) {
+                    const bool real = t < P;
+                    sp64[t] = make_double4(real ? v[i].x : 1e300, real ? v[i].y : 1e300, real ? v[i].z : 1e300, 0.0);
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -442,69 +459,80 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         typedef float f4 __attribute__((ext_vector_type(4)));
         const float *pf = reinterpret_cast<const float *>(a.p32);
         const uint32_t nblk = (last - first + 15) >> 4;
-        float bop[NH], ci[NH];
-        uint32_t lim[NH], mnb[NH], c8[NH];
-        bool on[NH], force[NH];
-#pragma unroll
-        for (int h = 0; h < NH; h++) {
-            on[h] = ((m >> (16 * h)) & 0xffffull) != 0;  // wave-uniform
-            c8[h] = 0;
-            mnb[h] = 0xffffffffu;
-            bop[h] = ci[h] = 0.f;
-            lim[h] = 0;
-            force[h] = false;
-            if (on[h]) {
-                const float bx = (float)(mqx[h] - nr.cx), by = (float)(mqy[h] - nr.cy), bz = (float)(mqz[h] - nr.cz);
-                const float b2 = __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz));
-                const float Rf = mqRf[h];
-                const float sum = (amf + b2) + Rf;
-                const float hw = 3.814697265625e-06f * sum;  // 2^-18
-                ci[h] = (hw - Rf) + b2;
-                lim[h] = __float_as_uint(2.0f * hw);
-                force[h] = !(sum < 1e30f);
-                bop[h] = kk == 0 ? -2.0f * bx : kk == 1 ? -2.0f * by : kk == 2 ? -2.0f * bz : 1.0f;
-            }
-        }
-        for (uint32_t b0 = 0; b0 < nblk; b0 += 4) {
-            float av[4];
+        // per-query terms in the query's own lane; a forced FP64 re-test is lim = ~0 (every D <= lim)
+        const float bx = (float)(q.x - nr.cx), by = (float)(q.y - nr.cy), bz = (float)(q.z - nr.cz);
+        const float b2 = __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz));
+        const float sum = (amf + b2) + qRf;
+        const float hw = 3.814697265625e-06f * sum;  // 2^-18
+        const float ciq = (hw - qRf) + b2;
+        const uint32_t limq = sum < 1e30f ? __float_as_uint(2.0f * hw) : 0xffffffffu;
+        const float nbx = -2.0f * bx, nby = -2.0f * by, nbz = -2.0f * bz;
+        // rows of 16-row blocks b0 .. b0+3: clamped row, then the sentinel (0, 0, 0, 1e30) past the leaf
+        auto load_rows = [&](uint32_t b0, float (&av)[4]) {
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const uint32_t row = first + 16 * (b0 + t) + c16;
-                av[t] = row < last ? pf[4 * (size_t)row + kk] : (kk == 3 ? 1e30f : 0.0f);
+                const float v = pf[4 * (size_t)min(row, last - 1) + kk];
+                av[t] = row < last ? v : (kk == 3 ? 1e30f : 0.0f);
             }
-#pragma unroll
-            for (int h = 0; h < NH; h++) {
-                if (!on[h]) continue;
-                const f4 cv = {ci[h], ci[h], ci[h], ci[h]};
-                f4 d[4];
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-                    if (b0 + t < nblk) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bop[h], cv, 0, 0, 0);
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    if (b0 + t < nblk) {
-                        const uint32_t d0 = __float_as_uint(d[t][0]), d1 = __float_as_uint(d[t][1]);
-                        const uint32_t d2 = __float_as_uint(d[t][2]), d3 = __float_as_uint(d[t][3]);
-                        // v_perm selector 9 / 11: a byte of 8 copies of bit 31 of src1 / src0
-                        c8[h] = __builtin_popcount(__builtin_amdgcn_perm(d0, d1, 0x0c0c0b09u)) + c8[h];
-                        c8[h] = __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u)) + c8[h];
-                        mnb[h] = min(min(mnb[h], d0), d1);
-                        mnb[h] = min(min(mnb[h], d2), d3);
-                    }
-                }
-            }
-        }
+        };
+        float av0[4];
+        load_rows(0, av0);
+        // one 16-query column block at a time: N = 1..4 blocks of 16 rows multiplied back to back,
+        // then counted (sign bits) and bounded (unsigned minimum of the bit patterns)
 #pragma unroll
         for (int h = 0; h < NH; h++) {
-            if (!on[h]) continue;
+            if (((m >> (16 * h)) & 0xffffull) == 0) continue;  // no query of this block wants the leaf
+            const uint32_t src = 16 * h + c16;
+            const float ci = bperm_f32(src, ciq);
+            const uint32_t lim = bperm_u32(src, limq);
+            const float sx = bperm_f32(src, nbx), sy = bperm_f32(src, nby), sz = bperm_f32(src, nbz);
+            const float bop = kk == 0 ? sx : kk == 1 ? sy : kk == 2 ? sz : 1.0f;
+            uint32_t c8 = 0, mnb = 0xffffffffu;
+            auto tiles = [&](auto NB, const float (&av)[4]) {
+                constexpr int N = decltype(NB)::value;
+                const f4 cv = {ci, ci, ci, ci};
+                f4 d[N];
+#pragma unroll
+                for (int t = 0; t < N; t++) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bop, cv, 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < N; t++) {
+                    const uint32_t d0 = __float_as_uint(d[t][0]), d1 = __float_as_uint(d[t][1]);
+                    const uint32_t d2 = __float_as_uint(d[t][2]), d3 = __float_as_uint(d[t][3]);
+                    // v_perm selector 9 / 11: a byte of 8 copies of bit 31 of src1 / src0
+                    c8 = __builtin_popcount(__builtin_amdgcn_perm(d0, d1, 0x0c0c0b09u)) + c8;
+                    c8 = __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u)) + c8;
+                    mnb = min(min(mnb, d0), d1);
+                    mnb = min(min(mnb, d2), d3);
+                }
+            };
+            for (uint32_t b0 = 0; b0 < nblk; b0 += 4) {
+                const uint32_t nb = (nblk - b0) < 4u ? (nblk - b0) : 4u;  // wave-uniform
+                float av[4];
+                if (b0 == 0) {
+#pragma unroll
+                    for (int t = 0; t < 4; t++) av[t] = av0[t];
+                } else {
+                    load_rows(b0, av);
+                }
+                if (nb == 4)
+                    tiles(std::integral_constant<int, 4>{}, av);
+                else if (nb == 3)
+                    tiles(std::integral_constant<int, 3>{}, av);
+                else if (nb == 2)
+                    tiles(std::integral_constant<int, 2>{}, av);
+                else
+                    tiles(std::integral_constant<int, 1>{}, av);
+            }
             const bool act = ((m >> (16 * h + c16)) & 1ull) != 0;
-            uint32_t lc = c8[h] >> 3;
-            const uint64_t bm = ballot64(act && (mnb[h] <= lim[h] || force[h]));
+            uint32_t lc = c8 >> 3;
+            const uint64_t bm = ballot64(act && mnb <= lim);
             const uint32_t cols = (uint32_t)((bm | (bm >> 16) | (bm >> 32) | (bm >> 48)) & 0xffffull);
             if (cols) {  // rare: exact FP64 re-count of this column's rows for queries with a band hit
+                Query qh;
+                qh.x = bperm_f64(src, q.x), qh.y = bperm_f64(src, q.y), qh.z = bperm_f64(src, q.z);
+                qh.R = bperm_f64(src, q.R);
                 if ((cols >> c16) & 1u) {
-                    Query qh;
-                    qh.x = mqx[h], qh.y = mqy[h], qh.z = mqz[h], qh.R = mqR[h];
                     lc = 0;
                     for (uint32_t i = first + 4 * kk; i < last; i += 16)
                         for (uint32_t r = 0; r < 4 && i + r < last; r++) {
@@ -653,19 +681,25 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
                 const double fx = dmax0(dx - rx), fy = dmax0(dy - ry), fz = dmax0(dz - rz);
                 m2x[b] = fx * fx, m2y[b] = fy * fy, m2z[b] = fz * fz;
             }
-            double sxy[4], txy[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) sxy[k] = e2x[k & 1] + e2y[k >> 1], txy[k] = m2x[k & 1] + m2y[k >> 1];
+            for (int k = 0; k < 4; k++) {  // (bx, by) pairs; each partial sum serves both z halves
+                const int bx = k & 1, by = k >> 1;
+                const double sxy = e2x[bx] + e2y[by], txy = m2x[bx] + m2y[by];
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-                const bool out = ox[bx] || oy[by] || oz[bz];
-                const double smax = is3d ? sxy[c & 3] + e2z[bz] : sxy[c & 3];
-                const double smin = is3d ? txy[c & 3] + m2z[bz] : txy[c & 3];
-                const bool inside = !out && smax <= q.R;
-                const bool two = is3d ? ((nx[bx] && ny[by]) || (nx[bx] && nz[bz]) || (ny[by] && nz[bz])) : (nx[bx] && ny[by]);
-                in[c] = act && inside;
-                ov[c] = act && !out && !inside && (two || smin <= q.R);
+                for (int bz = 0; bz < (is3d ? 2 : 1); bz++) {
+                    const int c = k | (bz << 2);
+                    const bool out = ox[bx] || oy[by] || oz[bz];
+                    const double smax = is3d ? sxy + e2z[bz] : sxy;
+                    const double smin = is3d ? txy + m2z[bz] : txy;
+                    const bool inside = !out && smax <= q.R;
+                    const bool two = is3d ? ((nx[bx] && ny[by]) || (nx[bx] && nz[bz]) || (ny[by] && nz[bz])) : (nx[bx] && ny[by]);
+                    in[c] = act && inside;
+                    ov[c] = act && !out && !inside && (two || smin <= q.R);
+                    if (!is3d) {  // 2-D kernels: the z half of the children repeats the judgement
+                        in[c | 4] = in[c];
+                        ov[c | 4] = ov[c];
+                    }
+                }
             }
         }
     };
@@ -703,6 +737,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         bool in[8], ovl[8];
         judge8(E, d, inm, in, ovl);
         if (STATS) n_class += 8ull * __popcll(m), n_cslots += 64ull * 8;
+        uint32_t leaves = 0;  // combos of the OVERLAP non-empty leaf children (bit c)
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const uint32_t f = E.first[c], l = E.last[c];
@@ -712,16 +747,29 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
                 const uint32_t gc = E.gc[c];
                 if (gc == 0) {
                     if (l > f) {  // an empty leaf adds nothing to a count
-                        NodeRec ur;
-                        ur.cx = E.ax[c & 1], ur.cy = E.ax[2 + ((c >> 1) & 1)], ur.cz = E.ax[4 + ((c >> 2) & 1)];
-                        ur.first = f, ur.last = l;
-                        leaf(ov, E.amax[c], ur);
+                        if (lane == 0) s_leafm[w][c] = ov;
+                        leaves |= 1u << c;
                     }
                 } else if (sp < kStack) {
                     s_node[w][sp] = gc, s_mask[w][sp] = ov, s_depth[w][sp] = (uint8_t)d;
                     sp++;
                 }
             }
+        }
+        // the leaves, one code copy: their records are scalar loads from E at a uniform index
+        if (leaves) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        while (leaves) {
+            const uint32_t c = (uint32_t)__builtin_ctz(leaves);
+            leaves &= leaves - 1;
+            const uint64_t ov = uniform_u64(s_leafm[w][c]);
+            NodeRec ur;
+            ur.cx = E.ax[c & 1], ur.cy = E.ax[2 + ((c >> 1) & 1)], ur.cz = E.ax[4 + ((c >> 2) & 1)];
+            ur.first = E.first[c], ur.last = E.last[c];
+            leaf(ov, E.amax[c], ur);
         }
     }
 #pragma unroll
