@@ -42,6 +42,48 @@ def sync_all(ctx):
         pass
 
 
+def profile(args):
+    """Small fixed workloads for rocprofv3 runs (kernel statistics and counters)."""
+    if args.profile == "copy":
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record()
+            dst.copy_(src)
+            t1.record()
+            torch.cuda.synchronize()
+            ms = t0.elapsed_time(t1)
+            log(f"[profile] copy 1 GiB: {ms:.3f} ms = {2 * (1 << 30) / ms / 1e6:.0f} GB/s (read + write)")
+        return
+    n = args.n
+    extent = args.extent if n == 50_000_000 else args.extent * (n / 50_000_000) ** 0.5
+    ctx = A.Octree(0)
+    ctx.set_points(A.lidar_tile(n, seed=2, extent=extent))
+    if args.profile == "sort":
+        for _ in range(3):
+            ctx.sort_points(A.ENC_HILBERT)
+            ctx.build(args.max_leaf)
+            log(f"[profile] {ctx.stage_times()}")
+        ctx.close()
+        return
+    kind, _, spec = args.profile.partition(":")
+    if kind != "search" or not spec:
+        sys.exit(f"unknown --profile {args.profile!r}")
+    ctx.sort_points(A.ENC_HILBERT)
+    ctx.build(args.max_leaf)
+    for v in spec.split(","):
+        prec = A.PREC_FP64 if v[0] == "d" else A.PREC_FP32
+        ctx.set_mfma(v[0] == "m")
+        ctx.set_group(int(v[1:]))
+        ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, prec, args.radius)
+        ctx.synchronize()
+        log(f"[profile] {v}: {ctx.stage_times()['search']:.2f} ms")
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,14 +93,22 @@ def main():
     ap.add_argument("--extent", type=float, default=1000.0)
     ap.add_argument("--radius", type=float, default=2.5)
     ap.add_argument("--max-leaf", type=int, default=128)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample duration")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--profile", default=None,
+                    help="profiling driver instead of the benchmark: 'search:v64,d64' (one all-points search per "
+                         "configuration: v = FP32 vector ALUs, m = FP32 matrix cores, d = FP64, then queries per "
+                         "wave), 'sort' (bbox + encode + sort + gather + build, 3 times) or 'copy' (a 1 GiB copy "
+                         "kernel, the counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
               "(the multi-GPU sharding is a design note, not an executable path)", file=sys.stderr)
         sys.exit(2)
+    if args.profile:
+        profile(args)
+        return
 
     n = args.n
     extent = args.extent if args.n == 50_000_000 else args.extent * (n / 50_000_000) ** 0.5
@@ -167,15 +217,31 @@ def main():
                              f"(build each range list, keep its size), OpenMP dynamic schedule, {dt:.1f} s"}
             log(f"[bench] cpu baseline {cpu}")
 
-    # roofline of the dominant kernel (FP32 search): 9 flops per point test
+    # Roofline of the dominant kernel (the FP32 search): 9 flops (3 sub + 3 mul + 2 add + 1 compare) per
+    # point test of the reference traversal (T, identical for both precisions), over the kernel's
+    # device time (HIP events on the context's stream), against the FP32 compute roof. traffic: the
+    # kernel's HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    # this workload (calibrated on a copy kernel), when the measured configuration matches it.
     T = fp32["stats"]["points_tested"]
     kern_s = fp32["kernel_ms"] * 1e-3
     achieved = 9.0 * T / kern_s / 1e12
-    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None}
+    traffic = None
+    try:
+        tr = json.load(open(os.path.join(ROOT, "profiles", "search_c2_50M_traffic.json")))
+        cfg = f"k_count<1, 3, 1, {tuned['fp32']['queries_per_wave']}, false, {str(tuned['fp32']['mfma']).lower()}>"
+        if n == 50_000_000 and tr["kernel"].startswith(cfg):
+            traffic = tr["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        traffic = None
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                "kernel": "radius count (FP32 + FP64 re-check)",
+                "compute_unit": "matrix cores" if tuned["fp32"]["mfma"] else "vector ALUs (the FP32 roof is the same 157.3 TFLOP/s figure)"}
 
     es_ms = med(bbox_ms) + med(enc_ms) + med(sort_ms) + med(gather_ms)
-    es_bytes = (116 + 24 * 8) * n  # SURVEY 8(d), d = 8 bits, P = 8 passes
+    # algorithmic bytes of bbox + encode + sort + gather: 24 + 32 + 8 (histogram) + P * 24 (key + index read
+    # and written per pass) + 52 (gather) B/pt with 8-bit digits, P = 8 passes
+    es_bytes = (116 + 24 * 8) * n
     out = {
         "metric": "radius queries/s @50M pts, FP32+recheck vs FP64 GPU (exact sets); build ms",
         "value": qps,
