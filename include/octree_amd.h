@@ -154,9 +154,21 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
 
 /* ---- H6 kNN: LinearOctree::knn (linear_octree.hpp:899-942), benchmark label KNNV2. Exact k
  * nearest under the (squared FP64 distance, sorted index) order; returns min(n, k) per query,
- * ascending. idx_out: nq x k int32 (-1 pads), d2_out: nq x k FP64 squared distances (NaN pads). */
+ * ascending; k <= 1024. idx_out: nq x k int32 (-1 pads), d2_out: nq x k FP64 squared distances
+ * (NaN pads). OCT_PREC_FP32_RECHECK screens points with leaf-recentred FP32 distances and computes
+ * the FP64 distance only for points that may beat the current k-th; results are identical. */
 int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *center_idx, size_t nq, uint32_t k,
             int32_t *idx_out, double *d2_out, oct_search_stats *stats);
+/* Device-resident kNN for throughput runs, same neighbour sets in the compact output form: int32
+ * indices and FP32 squared distances (the exact FP64 distance rounded to nearest), nq x k each,
+ * -1 / NaN past min(n, k). Queries are the sorted points first .. first + nq - 1
+ * (center_idx_dev NULL), or nq device-resident indices into the sorted cloud (each < n; first = 0),
+ * processed in the given order: ascending indices keep each wave's queries together. idx_dev /
+ * d2_dev NULL: the results stay in the context and are read back with oct_knn_fetch (queries
+ * [q0, q0 + nq) of the last such call; either output may be NULL). */
+int oct_knn_dev(oct_ctx *ctx, int precision, const uint32_t *center_idx_dev, size_t first, size_t nq, uint32_t k,
+                int32_t *idx_dev, float *d2_dev, oct_search_stats *stats);
+int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d2_out);
 
 /* ---- tuning (no counterpart in the reference: its per-query loop has no tile shape to choose).
  * OCT_TUNE_RADIUS_GROUP: queries per wave of the count-mode radius kernel (16, 32 or 64 queries

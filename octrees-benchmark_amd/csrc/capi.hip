@@ -660,15 +660,71 @@ int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *
     HCHK(ctx->kd2.reserve((size_t)nq * k + 1), "alloc knn");
     HCHK(ctx->stats.reserve(8), "alloc stats");
     HCHK(hipMemsetAsync(ctx->stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream), "clear stats");
+    KnnLaunch L;
+    L.precision = precision;
+    L.order = d_order;
+    L.qidx = center_idx ? d_idx : nullptr;
+    L.qxyz = centers ? ctx->qxyz.p : nullptr;
+    L.nq = nq, L.k = k;
+    L.idx = ctx->kidx.p, L.d2 = ctx->kd2.p;
+    L.stats = ctx->stats.p;
     std::string err;
-    hipError_t e = launch_knn(ctx, precision, center_idx ? d_idx : nullptr, centers ? ctx->qxyz.p : nullptr, nq, k,
-                              ctx->kidx.p, ctx->kd2.p, ctx->stats.p, err);
-    if (e != hipSuccess) return err.empty() ? hfail(ctx, e, "knn") : fail(ctx, OCT_ERR_STATE, err);
-    (void)d_order;
+    hipError_t e = launch_knn(ctx, L, err);
+    if (e != hipSuccess) return err.empty() ? hfail(ctx, e, "knn") : fail(ctx, OCT_ERR_ARG, err);
     if (idx_out && nq) HCHK(hipMemcpyAsync(idx_out, ctx->kidx.p, nq * k * 4, hipMemcpyDeviceToHost, ctx->stream), "copy idx");
     if (d2_out && nq) HCHK(hipMemcpyAsync(d2_out, ctx->kd2.p, nq * k * 8, hipMemcpyDeviceToHost, ctx->stream), "copy d2");
     HCHK(hipStreamSynchronize(ctx->stream), "knn");
     if (stats) return read_stats(ctx, stats, stage_ms(ctx, ST_KNN));
+    return OCT_OK;
+}
+
+int oct_knn_dev(oct_ctx *ctx, int precision, const uint32_t *center_idx_dev, size_t first, size_t nq, uint32_t k,
+                int32_t *idx_dev, float *d2_dev, oct_search_stats *stats) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (precision != OCT_PREC_FP64 && precision != OCT_PREC_FP32_RECHECK) return fail(ctx, OCT_ERR_ARG, "unknown precision");
+    if (!ctx->has_tree) return fail(ctx, OCT_ERR_STATE, "knn before oct_build");
+    if (k > 1024) return fail(ctx, OCT_ERR_ARG, "knn: k above 1024 is not supported");
+    if (!center_idx_dev && (first > ctx->n || nq > ctx->n - first))
+        return fail(ctx, OCT_ERR_ARG, "all-points kNN range beyond the cloud");
+    if (center_idx_dev && first) return fail(ctx, OCT_ERR_ARG, "first applies to all-points kNN only");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    const bool keep = !idx_dev || !d2_dev;
+    if (keep) {
+        HCHK(ctx->kidx.reserve((size_t)nq * k + 1), "alloc knn");
+        HCHK(ctx->kd2f.reserve((size_t)nq * k + 1), "alloc knn");
+        ctx->knn_nq = 0, ctx->knn_k = 0;
+    }
+    KnnLaunch L;
+    L.precision = precision;
+    L.qidx = center_idx_dev;
+    L.qoff = (uint32_t)first;
+    L.nq = nq, L.k = k;
+    L.idx = idx_dev ? idx_dev : ctx->kidx.p;
+    L.d2f = d2_dev ? d2_dev : ctx->kd2f.p;
+    if (stats) {
+        HCHK(ctx->stats.reserve(8), "alloc stats");
+        HCHK(hipMemsetAsync(ctx->stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream), "clear stats");
+        L.stats = ctx->stats.p;
+    }
+    std::string err;
+    hipError_t e = launch_knn(ctx, L, err);
+    if (e != hipSuccess) return err.empty() ? hfail(ctx, e, "knn") : fail(ctx, OCT_ERR_ARG, err);
+    if (keep) ctx->knn_nq = nq, ctx->knn_k = k;
+    if (stats) {
+        HCHK(hipStreamSynchronize(ctx->stream), "knn");
+        return read_stats(ctx, stats, stage_ms(ctx, ST_KNN));
+    }
+    return OCT_OK;
+}
+
+int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d2_out) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (q0 > ctx->knn_nq || nq > ctx->knn_nq - q0) return fail(ctx, OCT_ERR_ARG, "knn fetch beyond the kept results");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    const size_t k = ctx->knn_k, off = q0 * k, cnt = nq * k;
+    if (idx_out && cnt) HCHK(hipMemcpyAsync(idx_out, ctx->kidx.p + off, cnt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy idx");
+    if (d2_out && cnt) HCHK(hipMemcpyAsync(d2_out, ctx->kd2f.p + off, cnt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy d2");
+    HCHK(hipStreamSynchronize(ctx->stream), "knn fetch");
     return OCT_OK;
 }
 

@@ -128,6 +128,9 @@ struct oct_ctx {
     // kNN
     oct::DevBuf<int32_t> kidx;
     oct::DevBuf<double> kd2;
+    oct::DevBuf<float> kd2f;        // oct_knn_dev results kept in the context
+    size_t knn_nq = 0;              // rows of the kept oct_knn_dev results
+    uint32_t knn_k = 0;
 
     hipEvent_t ev0[oct::ST_N] = {}, ev1[oct::ST_N] = {};
     bool ev_used[oct::ST_N] = {};
@@ -165,8 +168,23 @@ struct RadiusLaunch {
 };
 hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L);
 hipError_t launch_key_codec(oct_ctx *ctx, int encoder, bool decode, uint32_t *xyz_dev, uint64_t *keys_dev, size_t n);
-hipError_t launch_knn(oct_ctx *ctx, int precision, const uint32_t *qidx_dev, const double *qxyz_dev, size_t nq,
-                      uint32_t k, int32_t *idx_dev, double *d2_dev, unsigned long long *stats_dev, std::string &err);
+// One kNN launch. Query slot s (0..nq-1) processes query id order[s] (or s); the centre is qxyz[id]
+// if given, else the sorted point qidx[id] (or qoff + id); outputs (nq x k, row id) go to idx and
+// to d2 (FP64) and / or d2f (FP32), either of which may be null.
+struct KnnLaunch {
+    int precision = 0;
+    const uint32_t *order = nullptr;
+    const uint32_t *qidx = nullptr;
+    const double *qxyz = nullptr;
+    uint32_t qoff = 0;
+    size_t nq = 0;
+    uint32_t k = 0;
+    int32_t *idx = nullptr;
+    double *d2 = nullptr;
+    float *d2f = nullptr;
+    unsigned long long *stats = nullptr;
+};
+hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err);
 hipError_t exclusive_scan_u32_to_u64(oct_ctx *ctx, const uint32_t *in, uint64_t *out, size_t n);
 hipError_t exclusive_scan_u32(oct_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev);
 

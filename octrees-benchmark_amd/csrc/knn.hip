@@ -5,227 +5,375 @@
 // dx*dx + dy*dy + dz*dz, no contraction), ordered by (distance, sorted index). The reference's
 // heap returns the same distances; only the order inside a tie class is unspecified there.
 //
-// One wave64 per query. Depth-first traversal, children visited nearest-first; a node is skipped
-// when its box distance exceeds the current k-th distance by a margin (1e-6 m) that covers FP64
-// rounding of node boxes and of points sitting on cell faces, so pruning never drops a true
-// neighbour. Leaf points are tested 64 at a time; candidates better than the current k-th go to an
-// LDS buffer of capacity C >= k + 64, which is bitonic-sorted under (d2, index) and truncated to k when
-// it fills up and at the end.
+// One wave serves G = 64 / S queries that are consecutive in SFC order (S lanes per query) and
+// walks the tree once for all of them, depth first, children pushed so that the octant of the
+// wave's middle query comes off the stack first. A child is skipped for a query when a lower
+// bound of its box distance exceeds the query's current k-th distance; the bound widens every box
+// by 2^-40 of the cloud's magnitude, which covers the FP64 rounding of node centres and of points
+// on cell faces, and compares with a relative margin, so pruning never drops a true neighbour.
+// Leaf points are staged in LDS once per wave. Each query appends every point that beats its
+// current k-th entry to an unsorted LDS buffer of 2 KC entries; when a buffer could overflow, the
+// wave sorts every buffer (bitonic, S lanes per query) and keeps the k best, which sets the new
+// k-th distance. FP32_RECHECK filters the points with the leaf-recentred FP32 form
+// e = w - 2 a.b + |b|^2 (the count kernel's representation), whose error is below
+// hw / 2 = 2^-19 (amax + |b|^2); only points with e <= k-th + hw get their exact FP64 distance.
 #include <cfloat>
 
 #include "device.hpp"
 
 namespace oct {
 
-constexpr int kKnnStack = 8 * 23;
+constexpr int kKnnStack = 8 * 22 + 8;  // depth first: at most 7 entries per level + 8 live
+constexpr int kKnnChunk = 128;         // leaf points staged per round
 
 struct KnnArgs {
     const double4 *__restrict__ p64;
+    const float4 *__restrict__ p32;
+    const float *__restrict__ amax;
     const NodeRec *__restrict__ nodes;
+    const NodeExt *__restrict__ next;
     const uint32_t *__restrict__ child;
     const uint32_t *__restrict__ order;
     const uint32_t *__restrict__ qidx;
     const double *__restrict__ qxyz;
     uint32_t nq, k;
+    uint32_t qoff;  // all-points queries: slot s is sorted point qoff + s
     int32_t *__restrict__ idx_out;
     double *__restrict__ d2_out;
+    float *__restrict__ d2f_out;
     unsigned long long *__restrict__ stats;
     double rad[22][3];
+    double mag[3];  // per axis |centre| + radius of the cloud's box
 };
 
 __device__ inline bool knn_less(double da, uint32_t ia, double db, uint32_t ib) { return da < db || (da == db && ia < ib); }
 
-// squared distance from q to the box c +- r (distPointOctantSquared, linear_octree.hpp:854-878)
-__device__ inline double box_dist2(double qx, double qy, double qz, const NodeRec &n, const double *r) {
-    double mnx = n.cx - r[0], mxx = n.cx + r[0];
-    double mny = n.cy - r[1], mxy = n.cy + r[1];
-    double mnz = n.cz - r[2], mxz = n.cz + r[2];
-    double cx = fmax(mnx, fmin(qx, mxx)), cy = fmax(mny, fmin(qy, mxy)), cz = fmax(mnz, fmin(qz, mxz));
-    double dx = qx - cx, dy = qy - cy, dz = qz - cz;
-    return dx * dx + dy * dy + dz * dz;
+// Lower bound of the squared distance from q to any point of the box c +- r, each axis widened by
+// dl; the caller compares it against tau (1 + 2^-40), which covers the rounding of this sum.
+__device__ inline double gap2(double q, double c, double r, double dl) {
+    const double gap = fabs(q - c) - r - dl;
+    return gap > 0.0 ? gap * gap : 0.0;
 }
 
-template <int C>
-__device__ inline void bitonic_sort(double *bd, uint32_t *bi) {
-    const uint32_t lane = lane_id();
-    for (uint32_t size = 2; size <= (uint32_t)C; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            __syncthreads();
-            for (uint32_t t = lane; t < (uint32_t)C / 2; t += 64) {
-                uint32_t i = 2 * stride * (t / stride) + (t % stride);
-                uint32_t j = i + stride;
-                bool up = (i & size) == 0;
-                double di = bd[i], dj = bd[j];
-                uint32_t ii = bi[i], ij = bi[j];
-                bool sw = up ? knn_less(dj, ij, di, ii) : knn_less(di, ii, dj, ij);
-                if (sw) {
-                    bd[i] = dj, bd[j] = di;
-                    bi[i] = ij, bi[j] = ii;
-                }
-            }
+template <int S, int KC, int PREC>
+__global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
+    constexpr int G = 64 / S;                           // queries per wave
+    constexpr int CAP = 2 * KC;                         // buffer entries per query
+    constexpr int U = S >= 16 ? 1 : 16 / S;             // points per lane per step
+    constexpr int STEP = S * U;                         // points per query per step
+    constexpr bool F32 = PREC == OCT_PREC_FP32_RECHECK;
+    constexpr int STG = kKnnChunk + STEP;               // staged slots incl. the last step's tail
+    static_assert(STEP <= KC, "a step must fit after a selection");
+    __shared__ double s_bd[G * CAP];
+    __shared__ uint32_t s_bi[G * CAP];
+    __shared__ uint32_t s_node[kKnnStack];
+    __shared__ uint8_t s_lev[kKnnStack];
+    __shared__ __attribute__((aligned(16))) unsigned char s_stage[STG * (F32 ? 16 : 32)];
+    float4 *s32 = reinterpret_cast<float4 *>(s_stage);
+    double4 *s64 = reinterpret_cast<double4 *>(s_stage);
+
+    const uint32_t lane = threadIdx.x;
+    const uint32_t qs = lane / S, g = lane % S;
+    const uint32_t slot0 = blockIdx.x * G;
+    const uint32_t slot = slot0 + qs;
+    const bool valid = slot < a.nq;
+    const uint64_t segmask = (S == 64 ? ~0ull : ((1ull << S) - 1ull)) << (qs * S);
+    const uint32_t k = a.k;
+    double *bd = s_bd + qs * CAP;
+    uint32_t *bi = s_bi + qs * CAP;
+
+    auto wsync = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    uint32_t qid = 0;
+    double qx = 0.0, qy = 0.0, qz = 0.0;
+    if (valid) {
+        qid = a.order ? a.order[slot] : slot;
+        if (a.qxyz) {
+            qx = a.qxyz[3 * (size_t)qid], qy = a.qxyz[3 * (size_t)qid + 1], qz = a.qxyz[3 * (size_t)qid + 2];
+        } else {
+            const double4 p = a.p64[a.qidx ? a.qidx[qid] : a.qoff + qid];
+            qx = p.x, qy = p.y, qz = p.z;
         }
     }
-    __syncthreads();
-}
+    // box widening per axis: 2^-40 of the magnitudes involved (cloud box and query)
+    const double dlx = 0x1p-40 * (a.mag[0] + fabs(qx)), dly = 0x1p-40 * (a.mag[1] + fabs(qy));
+    const double dlz = 0x1p-40 * (a.mag[2] + fabs(qz));
 
-template <int C>
-__global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
-    __shared__ double bd[C];
-    __shared__ uint32_t bi[C];
-    __shared__ uint32_t s_node[kKnnStack];
-    __shared__ double s_md[kKnnStack];
-    __shared__ uint8_t s_dep[kKnnStack];
-
-    const uint32_t slot = blockIdx.x;
-    const uint32_t lane = threadIdx.x;
-    if (slot >= a.nq) return;
-    const uint32_t qid = a.order ? a.order[slot] : slot;
-    double qx, qy, qz;
-    if (a.qxyz) {
-        qx = a.qxyz[3 * (size_t)qid], qy = a.qxyz[3 * (size_t)qid + 1], qz = a.qxyz[3 * (size_t)qid + 2];
-    } else {
-        double4 p = a.p64[a.qidx ? a.qidx[qid] : qid];
-        qx = p.x, qy = p.y, qz = p.z;
-    }
-    const uint32_t k = a.k;
-    uint32_t cnt = 0;                 // entries in the buffer (uniform)
-    double tau = DBL_MAX;             // current k-th distance (valid once full)
+    for (uint32_t t = g; t < (uint32_t)CAP; t += S) bd[t] = __builtin_huge_val(), bi[t] = 0xffffffffu;
+    uint32_t cnt = 0;        // buffer entries of this query (same in its S lanes)
+    bool full = false;       // k entries seen: tau is the k-th
+    double tau = __builtin_huge_val(), tau_pr = __builtin_huge_val();
     uint32_t tau_i = 0xffffffffu;
-    bool full = false;
-    uint64_t n_test = 0, n_nodes = 0;
+    float tau32 = __builtin_huge_valf();  // tau rounded up to FP32
+    uint64_t n_nodes = 0, n_test = 0, n_recheck = 0;
 
-    auto merge = [&]() {
-        for (uint32_t t = cnt + lane; t < (uint32_t)C; t += 64) bd[t] = DBL_MAX, bi[t] = 0xffffffffu;
-        bitonic_sort<C>(bd, bi);
-        if (cnt >= k) {
-            cnt = k;
+    // sort every query's buffer, keep its k best
+    auto select = [&]() {
+        wsync();
+#pragma unroll 1
+        for (uint32_t size = 2; size <= (uint32_t)CAP; size <<= 1) {
+#pragma unroll 1
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = g; t < (uint32_t)CAP / 2; t += S) {
+                    const uint32_t i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
+                    const uint32_t j = i + stride;
+                    const bool up = (i & size) == 0;
+                    const double di = bd[i], dj = bd[j];
+                    const uint32_t ii = bi[i], ij = bi[j];
+                    const bool sw = up ? knn_less(dj, ij, di, ii) : knn_less(di, ii, dj, ij);
+                    if (sw) {
+                        bd[i] = dj, bd[j] = di;
+                        bi[i] = ij, bi[j] = ii;
+                    }
+                }
+                wsync();
+            }
+        }
+        const uint32_t keep = cnt < k ? cnt : k;
+        for (uint32_t t = keep + g; t < cnt; t += S) bd[t] = __builtin_huge_val(), bi[t] = 0xffffffffu;
+        cnt = keep;
+        if (keep == k && k > 0) {
             full = true;
             tau = bd[k - 1];
             tau_i = bi[k - 1];
+            tau_pr = tau * (1.0 + 0x1p-40);
+            tau32 = __double2float_ru(tau);
+        }
+        wsync();
+    };
+
+    // leaf [first, last) with centre c and FP32 bound amx, for the queries with need set
+    auto leaf = [&](bool need, uint32_t first, uint32_t last, double cx, double cy, double cz, float amx) {
+        float m2x = 0.f, m2y = 0.f, m2z = 0.f, bb = 0.f, hw = 0.f;
+        bool force = false;
+        if (F32) {
+            const float bx = (float)(qx - cx), by = (float)(qy - cy), bz = (float)(qz - cz);
+            bb = bx * bx + by * by + bz * bz;
+            const float s = amx + bb;
+            force = !(s < 1e30f);
+            hw = s * 0x1p-18f;
+            m2x = -2.0f * bx, m2y = -2.0f * by, m2z = -2.0f * bz;
+        }
+        if (need && g == 0) n_nodes++;
+        for (uint32_t base = first; base < last; base += kKnnChunk) {
+            const uint32_t P = (last - base) < (uint32_t)kKnnChunk ? (last - base) : (uint32_t)kKnnChunk;
+            const uint32_t nsteps = (P + STEP - 1) / STEP;
+            const uint32_t slots = nsteps * STEP;
+            for (uint32_t t = lane; t < slots; t += 64) {
+                if (F32)
+                    s32[t] = t < P ? a.p32[base + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+                else
+                    s64[t] = t < P ? a.p64[base + t] : make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+            wsync();
+            for (uint32_t s = 0; s < nsteps; s++) {
+                if (ballot64(need && cnt + STEP > (uint32_t)CAP)) select();
+                float thr = __builtin_huge_valf();
+                if (F32 && full && !force) thr = ((tau32 + hw) * (1.0f + 0x1p-20f));
+                double d[U];
+                uint32_t id[U];
+                bool c[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const uint32_t t = s * STEP + u * S + g;
+                    id[u] = base + t;
+                    const bool real = need && t < P;
+                    if (F32) {
+                        const float4 p = s32[t];
+                        const float e = __builtin_fmaf(p.x, m2x, __builtin_fmaf(p.y, m2y, __builtin_fmaf(p.z, m2z, p.w + bb)));
+                        c[u] = real && !(e > thr);
+                        d[u] = 0.0;
+                    } else {
+                        const double4 p = s64[t];
+                        const double dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+                        d[u] = dx * dx + dy * dy + dz * dz;
+                        c[u] = real && (!full || knn_less(d[u], id[u], tau, tau_i));
+                    }
+                    if (real) n_test++;
+                }
+                if (F32) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (c[u]) {
+                            const double4 p = a.p64[id[u]];
+                            const double dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+                            d[u] = dx * dx + dy * dy + dz * dz;
+                            c[u] = !full || knn_less(d[u], id[u], tau, tau_i);
+                            n_recheck++;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const uint64_t m = ballot64(c[u]) & segmask;
+                    if (c[u]) {
+                        const uint32_t pos = cnt + (uint32_t)__popcll(m & lanemask_lt());
+                        bd[pos] = d[u];
+                        bi[pos] = id[u];
+                    }
+                    cnt += (uint32_t)__popcll(m);
+                }
+            }
+            wsync();  // the next chunk overwrites the staging area
         }
     };
-    auto prune_bound = [&]() -> double {  // (sqrt(tau) + 1e-6)^2
-        if (!full) return DBL_MAX;
-        double s = sqrt(tau) + 1e-6;
-        return s * s;
-    };
+
+    // reference query for the child order: the middle valid query of the wave
+    const uint32_t nvalid = a.nq - slot0 < (uint32_t)G ? a.nq - slot0 : (uint32_t)G;
+    const int refl = (int)(S * (nvalid / 2));
 
     uint32_t sp = 0;
-    s_node[0] = 0, s_md[0] = 0.0, s_dep[0] = 0;
-    sp = 1;
-    while (sp > 0) {
-        __syncthreads();
-        sp--;
-        const uint32_t node = s_node[sp];
-        const double md = s_md[sp];
-        const uint32_t dep = s_dep[sp];
-        __syncthreads();
-        if (md > prune_bound()) continue;
-        n_nodes++;
-        const uint32_t fc = a.child[node];
-        if (fc == 0) {
-            const NodeRec nr = a.nodes[node];
-            for (uint32_t base = nr.first; base < nr.last; base += 64) {
-                const uint32_t i = base + lane;
-                const bool v = i < nr.last;
-                double d = DBL_MAX;
-                if (v) {
-                    const double4 p = a.p64[i];
-                    double dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
-                    d = dx * dx + dy * dy + dz * dz;
-                }
-                const bool cand = v && (!full || knn_less(d, i, tau, tau_i));
-                uint64_t m = ballot64(cand);
-                uint32_t nc = (uint32_t)__popcll(m);
-                n_test += (uint64_t)__popcll(ballot64(v));
-                if (nc == 0) continue;
-                if (cnt + nc > (uint32_t)C) {
-                    merge();
-                    const bool cand2 = cand && (!full || knn_less(d, i, tau, tau_i));
-                    m = ballot64(cand2);
-                    nc = (uint32_t)__popcll(m);
-                    if (nc == 0) continue;
-                }
-                if (m & (1ull << lane)) {
-                    uint32_t pos = cnt + (uint32_t)__popcll(m & lanemask_lt());
-                    bd[pos] = d;
-                    bi[pos] = i;
-                }
-                cnt += nc;
-            }
+    {
+        const uint32_t c0 = uniform_u32(a.child[0]);
+        if (c0 == 0) {
+            const NodeRec nr = a.nodes[0];
+            leaf(valid, uniform_u32(nr.first), uniform_u32(nr.last), nr.cx, nr.cy, nr.cz, a.amax[0]);
         } else {
-            // children: lane c < 8 computes its child's box distance; push farthest first
-            double cd = DBL_MAX;
-            if (lane < 8) cd = box_dist2(qx, qy, qz, a.nodes[fc + lane], a.rad[dep + 1]);
-            uint32_t rankc = 0;
-            for (uint32_t o = 0; o < 8; o++) {
-                double od = __shfl(cd, (int)o);
-                if (lane < 8 && (od > cd || (od == cd && o > lane))) rankc++;
-            }
-            // rankc = number of children to push before this one (farther children first)
-            __syncthreads();
-            if (lane < 8 && sp + rankc < (uint32_t)kKnnStack) {
-                s_node[sp + rankc] = fc + lane;
-                s_md[sp + rankc] = cd;
-                s_dep[sp + rankc] = (uint8_t)(dep + 1);
-            }
-            sp = (sp + 8 <= (uint32_t)kKnnStack) ? sp + 8 : (uint32_t)kKnnStack;
+            s_node[0] = (c0 - 1) >> 3, s_lev[0] = 0;
+            sp = 1;
         }
     }
-    __syncthreads();
-    merge();
-    const uint32_t found = cnt < k ? cnt : k;
-    for (uint32_t t = lane; t < k; t += 64) {
-        a.idx_out[(size_t)qid * k + t] = t < found ? (int32_t)bi[t] : -1;
-        a.d2_out[(size_t)qid * k + t] = t < found ? bd[t] : __builtin_nan("");
+    wsync();
+    while (sp > 0) {
+        sp--;
+        const uint32_t ent = uniform_u32(s_node[sp]);
+        const uint32_t lv = uniform_u32(s_lev[sp]);
+        if (lv & 0x80u) {
+            // a leaf: entry = NodeExt index * 8 + combo; re-test it against the current k-th distances
+            const NodeExt &E = a.next[ent >> 3];
+            const uint32_t c = ent & 7u, L = lv & 31u;
+            const double cx = E.ax[c & 1u], cy = E.ax[2 + ((c >> 1) & 1u)], cz = E.ax[4 + (c >> 2)];
+            const double lb = (gap2(qx, cx, a.rad[L][0], dlx) + gap2(qy, cy, a.rad[L][1], dly)) + gap2(qz, cz, a.rad[L][2], dlz);
+            const bool need = valid && !(lb > tau_pr);
+            if (!ballot64(need)) continue;
+            leaf(need, E.first[c], E.last[c], cx, cy, cz, E.amax[c]);
+            continue;
+        }
+        // an internal node at level lv: judge its 8 children (level lv + 1) from per-axis terms
+        const NodeExt &E = a.next[ent];
+        const uint32_t d = lv + 1;
+        const double rx = a.rad[d][0], ry = a.rad[d][1], rz = a.rad[d][2];
+        double gx[2], gy[2], gz[2];
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            gx[b] = gap2(qx, E.ax[b], rx, dlx);
+            gy[b] = gap2(qy, E.ax[2 + b], ry, dly);
+            gz[b] = gap2(qz, E.ax[4 + b], rz, dlz);
+        }
+        uint32_t needbits = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const double lb = (gx[c & 1] + gy[(c >> 1) & 1]) + gz[c >> 2];
+            const bool nd = valid && E.last[c] > E.first[c] && !(lb > tau_pr);
+            if (ballot64(nd)) needbits |= 1u << c;
+        }
+        if (g == 0) n_nodes += 8 * (valid ? 1 : 0);
+        if (needbits == 0) continue;
+        // push the farthest octants (relative to the reference query) first
+        const uint32_t oc = (qx >= 0.5 * (E.ax[0] + E.ax[1]) ? 1u : 0u) | (qy >= 0.5 * (E.ax[2] + E.ax[3]) ? 2u : 0u) |
+                            (qz >= 0.5 * (E.ax[4] + E.ax[5]) ? 4u : 0u);
+        const uint32_t qc = readlane_u32(oc, refl);
+        constexpr uint32_t kOrder = 0x02416537u;  // nibbles, low first: 7 3 5 6 1 4 2 0
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const uint32_t c = ((kOrder >> (4 * t)) & 7u) ^ qc;
+            if (((needbits >> c) & 1u) && sp < (uint32_t)kKnnStack) {
+                const uint32_t gc = E.gc[c];
+                s_node[sp] = gc ? (gc - 1) >> 3 : ent * 8 + c;
+                s_lev[sp] = (uint8_t)(gc ? d : (d | 0x80u));
+                sp++;
+            }
+        }
+        wsync();
     }
-    if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[0], (unsigned long long)n_nodes);
-        atomicAdd(&a.stats[1], (unsigned long long)n_test);
-        atomicAdd(&a.stats[3], (unsigned long long)found);
+    select();
+
+    if (valid) {
+        for (uint32_t t = g; t < k; t += S) {
+            const bool have = t < cnt;
+            a.idx_out[(size_t)qid * k + t] = have ? (int32_t)bi[t] : -1;
+            if (a.d2_out) a.d2_out[(size_t)qid * k + t] = have ? bd[t] : __builtin_nan("");
+            if (a.d2f_out) a.d2f_out[(size_t)qid * k + t] = have ? (float)bd[t] : __builtin_nanf("");
+        }
+    }
+    if (a.stats) {
+        uint64_t found = (valid && g == 0) ? cnt : 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            n_nodes += __shfl_xor(n_nodes, off);
+            n_test += __shfl_xor(n_test, off);
+            n_recheck += __shfl_xor(n_recheck, off);
+            found += __shfl_xor(found, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats[0], (unsigned long long)n_nodes);
+            atomicAdd(&a.stats[1], (unsigned long long)n_test);
+            atomicAdd(&a.stats[2], (unsigned long long)n_recheck);
+            atomicAdd(&a.stats[3], (unsigned long long)found);
+        }
     }
 }
 
-template <int C>
-static void launch_c(const KnnArgs &a, hipStream_t s) {
-    k_knn<C><<<a.nq, 64, 0, s>>>(a);
+template <int S, int KC>
+static void launch_sk(const KnnArgs &a, int precision, hipStream_t s) {
+    const unsigned nb = (unsigned)((a.nq + (64 / S) - 1) / (64 / S));
+    if (precision == OCT_PREC_FP32_RECHECK)
+        k_knn<S, KC, OCT_PREC_FP32_RECHECK><<<nb, 64, 0, s>>>(a);
+    else
+        k_knn<S, KC, OCT_PREC_FP64><<<nb, 64, 0, s>>>(a);
 }
 
-hipError_t launch_knn(oct_ctx *ctx, int precision, const uint32_t *qidx_dev, const double *qxyz_dev, size_t nq, uint32_t k,
-                      int32_t *idx_dev, double *d2_dev, unsigned long long *stats_dev, std::string &err) {
-    (void)precision;
+hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err) {
+    const size_t nq = L.nq;
+    const uint32_t k = L.k;
+    const int precision = L.precision;
     if (k > 1024) {
         err = "knn: k above 1024 is not supported";
         return hipErrorInvalidValue;
     }
-    if (nq == 0) return hipSuccess;
+    if (nq > 0xffffffffull) {
+        err = "knn: more than 2^32 - 1 queries in one call";
+        return hipErrorInvalidValue;
+    }
+    if (nq == 0 || k == 0) return hipSuccess;
     KnnArgs a;
     a.p64 = ctx->p64.p;
+    a.p32 = ctx->p32.p;
+    a.amax = ctx->amax.p;
     a.nodes = ctx->nodes.p;
+    a.next = ctx->next.p;
     a.child = ctx->child.p;
-    a.order = ctx->qidx.p;  // query order staged by the C ABI in qidx[0, nq)
-    a.qidx = qidx_dev;
-    a.qxyz = qxyz_dev;
+    a.order = L.order;
+    a.qidx = L.qidx;
+    a.qxyz = L.qxyz;
     a.nq = (uint32_t)nq;
     a.k = k;
-    a.idx_out = idx_dev;
-    a.d2_out = d2_dev;
-    a.stats = stats_dev;
+    a.qoff = L.qoff;
+    a.idx_out = L.idx;
+    a.d2_out = L.d2;
+    a.d2f_out = L.d2f;
+    a.stats = L.stats;
     for (int l = 0; l < 22; l++)
         for (int c = 0; c < 3; c++) a.rad[l][c] = ctx->level_radii[l][c];
-    // capacity: after a merge the buffer holds k entries and must still take a full 64-lane batch
-    uint32_t need = k + 64;
+    for (int c = 0; c < 3; c++) a.mag[c] = fabs(ctx->center[c]) + fabs(ctx->radii[c]);
     stage_begin(ctx, ST_KNN);
-    if (need <= 128)
-        launch_c<128>(a, ctx->stream);
-    else if (need <= 256)
-        launch_c<256>(a, ctx->stream);
-    else if (need <= 512)
-        launch_c<512>(a, ctx->stream);
-    else if (need <= 1024)
-        launch_c<1024>(a, ctx->stream);
+    if (k <= 16)
+        launch_sk<4, 16>(a, precision, ctx->stream);
+    else if (k <= 32)
+        launch_sk<4, 32>(a, precision, ctx->stream);
+    else if (k <= 64)
+        launch_sk<8, 64>(a, precision, ctx->stream);
+    else if (k <= 128)
+        launch_sk<16, 128>(a, precision, ctx->stream);
+    else if (k <= 256)
+        launch_sk<32, 256>(a, precision, ctx->stream);
+    else if (k <= 512)
+        launch_sk<64, 512>(a, precision, ctx->stream);
     else
-        launch_c<2048>(a, ctx->stream);
+        launch_sk<64, 1024>(a, precision, ctx->stream);
     stage_end(ctx, ST_KNN);
     return hipGetLastError();
 }

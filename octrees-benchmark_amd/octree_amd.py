@@ -23,7 +23,7 @@ STAGES = ("bbox", "encode", "sort", "gather", "build", "search", "knn")
 EXPORTS = (
     "oct_create", "oct_destroy", "oct_last_error", "oct_synchronize", "oct_version", "oct_set_points",
     "oct_set_points_dev", "oct_bbox", "oct_encode", "oct_sort", "oct_sort_points", "oct_set_sorted", "oct_build",
-    "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_stage_times",
+    "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_knn_dev", "oct_knn_fetch", "oct_stage_times",
     "oct_set_tuning", "oct_autotune_radius", "oct_tuned_radius", "oct_radius_box", "oct_encode_coords",
     "oct_decode_keys",
 )
@@ -80,6 +80,8 @@ def lib():
         L.oct_radius_dev.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_double, C.c_size_t, _vp,
                                      C.POINTER(SearchStats)]
         L.oct_knn.argtypes = [_vp, C.c_int, _vp, _vp, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
+        L.oct_knn_dev.argtypes = [_vp, C.c_int, _vp, C.c_size_t, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
+        L.oct_knn_fetch.argtypes = [_vp, C.c_size_t, C.c_size_t, _vp, _vp]
         L.oct_stage_times.argtypes = [_vp, _vp]
         L.oct_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
         L.oct_autotune_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_size_t,
@@ -317,6 +319,24 @@ class Octree:
         self._chk(self.L.oct_knn(self.h, precision, _ptr(centers), _ptr(idx), nq, k, _ptr(oi), _ptr(od), C.byref(st)),
                   "oct_knn")
         return oi, od, st.as_dict()
+
+    def knn_dev(self, k, precision=PREC_FP32, first=0, nq=None, stats=False):
+        """All-points kNN for sorted points first .. first + nq - 1; results (int32 indices, FP32 d^2)
+        stay on the device until knn_fetch."""
+        nq = self.n - first if nq is None else nq
+        st = SearchStats() if stats else None
+        self._chk(self.L.oct_knn_dev(self.h, precision, None, first, nq, k, None, None, C.byref(st) if stats else None),
+                  "oct_knn_dev")
+        self._knn_shape = (nq, k)
+        return st.as_dict() if stats else None
+
+    def knn_fetch(self, q0=0, nq=None):
+        total, k = self._knn_shape
+        nq = total - q0 if nq is None else nq
+        oi = np.empty((nq, k), np.int32)
+        od = np.empty((nq, k), np.float32)
+        self._chk(self.L.oct_knn_fetch(self.h, q0, nq, _ptr(oi), _ptr(od)), "oct_knn_fetch")
+        return oi, od
 
     def set_group(self, g):
         """Queries per wave of the count-mode radius kernel (0, 16, 32, 64; -1 = tuned/default)."""

@@ -128,6 +128,28 @@ def test_knn_duplicates_and_k_above_n(ctx):  # test_octree_advanced.cpp:49-67
     assert list(gd[0, :3]) == [0.0, 0.0, 3.0] and list(gi[0, 3:]) == [-1, -1]
 
 
+@pytest.mark.parametrize("offset", [0.0, 1e6, 1e12])
+def test_knn_offsets_and_far_queries(ctx, offset):
+    """kNN with large coordinate offsets (the pruning margin scales with the magnitudes) and with
+    centres outside the cloud, both precisions, against brute force."""
+    print(f"[gpu] knn offset={offset}", flush=True)
+    ctx.current = None
+    xyz = A.uniform_box(20_000, seed=3) * 0.01 + offset
+    ctx.set_points(xyz)
+    _, _, perm, _ = ctx.sort_points(A.ENC_HILBERT)
+    ctx.build(16)
+    sx = xyz[perm.astype(np.int64)]
+    rng = np.random.default_rng(4)
+    centers = np.vstack([sx[rng.integers(0, len(sx), 200)],
+                         offset + rng.uniform(-2.0, 3.0, (50, 3)),
+                         [[offset - 1e3, offset, offset], [offset, offset, offset + 7.0]]])
+    bi, bd = O.knn_brute(sx, centers, 12)
+    for prec in (A.PREC_FP64, A.PREC_FP32):
+        gi, gd, _ = ctx.knn(12, precision=prec, centers=centers)
+        np.testing.assert_array_equal(gi.astype(np.int64), bi)
+        assert np.array_equal(gd.view(np.uint64), bd.view(np.uint64))
+
+
 def test_single_point_and_small_leaf(ctx):
     for n in (1, 2, 7, 33, 129, 1000):
         pts = A.uniform_box(n, seed=n)

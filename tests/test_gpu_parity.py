@@ -165,17 +165,51 @@ def test_radius_all_points_counts(case, ctx, prec):
     assert st["total_neighbors"] == int(counts.astype(np.int64).sum())
 
 
-@pytest.mark.parametrize("k", [1, 10, 50, 250, 1000])
-def test_knn_exact(ctx, k):
-    say(f"[gpu] knn k={k}")
-    xyz = A.lidar_tile(60_000, seed=5, extent=40.0)
-    sx, keys, perm, c, r, tree = oracle_pipeline(xyz, A.ENC_HILBERT, 32)
-    gpu_pipeline(ctx, xyz, A.ENC_HILBERT, 32, tag="knn")
-    idx = A.random_indices(500, len(xyz), seed=13)
-    gi, gd, st = ctx.knn(k, idx=idx)
+_KNN_CASE = {}
+
+
+def knn_case(ctx):
+    """60 k-point tile (Hilbert, maxLeaf 32) with its oracle pipeline, built on the GPU once."""
+    if not _KNN_CASE:
+        xyz = A.lidar_tile(60_000, seed=5, extent=40.0)
+        _KNN_CASE["xyz"] = xyz
+        _KNN_CASE["oracle"] = oracle_pipeline(xyz, A.ENC_HILBERT, 32)
+    if getattr(ctx, "current", None) != "knn":
+        gpu_pipeline(ctx, _KNN_CASE["xyz"], A.ENC_HILBERT, 32, tag="knn")
+    return _KNN_CASE["oracle"]
+
+
+@pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("k", [1, 10, 17, 50, 100, 250, 600, 1000])
+def test_knn_exact(ctx, k, prec):
+    say(f"[gpu] knn k={k} prec={prec}")
+    sx, keys, perm, c, r, tree = knn_case(ctx)
+    idx = A.random_indices(500, len(sx), seed=13)
+    gi, gd, st = ctx.knn(k, precision=prec, idx=idx)
     bi, bd = O.knn_brute(sx, sx[idx.astype(np.int64)], k)
     np.testing.assert_array_equal(gi.astype(np.int64), bi)
     assert np.array_equal(gd.view(np.uint64), bd.view(np.uint64))
-    # the reference's best-first order returns the same squared distances
-    hi, hd = O.knn(tree, sx, sx[idx.astype(np.int64)], k)
-    assert np.array_equal(hd.view(np.uint64), bd.view(np.uint64))
+    assert st["total_neighbors"] == 500 * k
+    if k in (10, 250):
+        # the reference's best-first order returns the same squared distances
+        hi, hd = O.knn(tree, sx, sx[idx.astype(np.int64)], k)
+        assert np.array_equal(hd.view(np.uint64), bd.view(np.uint64))
+
+
+@pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("k", [10, 50])
+def test_knn_dev_all_points(ctx, k, prec):
+    """Device-resident all-points kNN over a chunk of sorted points: int32 indices equal the brute
+    force, FP32 distances are the FP64 distances rounded to nearest."""
+    say(f"[gpu] knn_dev k={k} prec={prec}")
+    sx, keys, perm, c, r, tree = knn_case(ctx)
+    first, nq = 20_011, 3_000
+    st = ctx.knn_dev(k, precision=prec, first=first, nq=nq, stats=True)
+    gi, gf = ctx.knn_fetch()
+    assert gi.shape == (nq, k) and st["total_neighbors"] == nq * k
+    bi, bd = O.knn_brute(sx, sx[first:first + nq], k)
+    np.testing.assert_array_equal(gi.astype(np.int64), bi)
+    assert np.array_equal(gf.view(np.uint32), bd.astype(np.float32).view(np.uint32))
+    # a slice of the kept results
+    si, sf = ctx.knn_fetch(100, 7)
+    np.testing.assert_array_equal(si, gi[100:107])
