@@ -67,8 +67,9 @@ typedef struct oct_search_stats {
     uint64_t nodes_classified; /* node predicate evaluations */
     uint64_t rechecked;        /* FP32 decisions re-evaluated in FP64 (points + nodes) */
     double kernel_ms;          /* device time of the search kernel(s) */
-    uint64_t class_slots;      /* wave lane-slots spent on node tests (64 per wave-level test) */
-    uint64_t test_slots;       /* wave lane-slots spent on point tests */
+    uint64_t class_slots;      /* wave lane-slots spent on node tests (64 per wave-level test);
+                                  kNN: wave-wide candidate-buffer selections */
+    uint64_t test_slots;       /* wave lane-slots spent on point tests; kNN: wave-wide leaf steps */
 } oct_search_stats;
 
 /* ---- context -------------------------------------------------------------------------------- */
@@ -180,12 +181,23 @@ int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d
  * units unless fixed by oct_set_tuning) on sample_nq queries taken from the sorted cloud (16 equal
  * strided chunks of consecutive SFC points; sample_nq >= n uses every point), stores the fastest for
  * (kernel, algo, precision), returns its G in *best and the best milliseconds per G in ms_out[0..3].
- * oct_tuned_radius: the G and point-test unit a count-mode search would use now. */
-enum oct_tune_knob { OCT_TUNE_RADIUS_GROUP = 1, OCT_TUNE_RADIUS_MFMA = 2 };
+ * oct_tuned_radius: the G and point-test unit a count-mode search would use now.
+ * OCT_TUNE_KNN_LANES: lanes per query of the kNN kernel (64 / lanes queries share one wave's
+ * traversal); each k size class (k <= 16, 32, 64, ... 1024) offers up to three shapes, the
+ * default first (k <= 32: 4, 2, 8; <= 64: 8, 4, 16; <= 128: 16, 8, 32; <= 256: 64, 32, 16;
+ * above: 64); a value a class does not offer makes its kNN calls fail with OCT_ERR_ARG; -1
+ * restores "autotuned, else the default". oct_autotune_knn times the offered shapes on sample_nq
+ * all-points queries (16 strided chunks, as above; 0 = 2^18), stores the fastest for (precision,
+ * size class of k), returns it in *best and the best milliseconds per shape in ms_out[0..2].
+ * oct_tuned_knn: the lanes per query a kNN call with this k would use now. Results never depend on
+ * the tuning. */
+enum oct_tune_knob { OCT_TUNE_RADIUS_GROUP = 1, OCT_TUNE_RADIUS_MFMA = 2, OCT_TUNE_KNN_LANES = 3 };
 int oct_set_tuning(oct_ctx *ctx, int knob, int value);
 int oct_autotune_radius(oct_ctx *ctx, int kernel, int algo, int precision, double radius, size_t sample_nq, int *best,
                         double ms_out[4]);
 int oct_tuned_radius(oct_ctx *ctx, int kernel, int algo, int precision, int *group, int *mfma);
+int oct_autotune_knn(oct_ctx *ctx, int precision, uint32_t k, size_t sample_nq, int *best, double ms_out[3]);
+int oct_tuned_knn(oct_ctx *ctx, int precision, uint32_t k, int *lanes);
 
 /* ---- timing: device milliseconds of the stages of the last calls (HIP events on the context's
  * stream): [0] bbox, [1] encode, [2] sort, [3] gather, [4] build, [5] search, [6] knn. */

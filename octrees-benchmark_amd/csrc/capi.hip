@@ -138,6 +138,8 @@ int oct_create(oct_ctx **out, int device) {
     for (auto &a : ctx->mx_tuned)
         for (auto &b : a)
             for (auto &c : b) c = -1;
+    for (auto &a : ctx->knn_lanes_tuned)
+        for (auto &b : a) b = -1;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return OCT_ERR_HIP;
@@ -565,6 +567,12 @@ int oct_set_tuning(oct_ctx *ctx, int knob, int value) {
         ctx->mx_override = value;
         return OCT_OK;
     }
+    if (knob == OCT_TUNE_KNN_LANES) {
+        if (value != -1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32 && value != 64)
+            return fail(ctx, OCT_ERR_ARG, "knn lanes per query must be -1, 2, 4, 8, 16, 32 or 64");
+        ctx->knn_lanes_override = value;
+        return OCT_OK;
+    }
     if (knob != OCT_TUNE_RADIUS_GROUP) return fail(ctx, OCT_ERR_ARG, "unknown tuning knob");
     if (value != -1 && value != 0 && value != 16 && value != 32 && value != 64)
         return fail(ctx, OCT_ERR_ARG, "radius group must be -1, 0, 16, 32 or 64");
@@ -635,6 +643,19 @@ int oct_tuned_radius(oct_ctx *ctx, int kernel, int algo, int precision, int *gro
     return OCT_OK;
 }
 
+// kNN size class of k: 0 for k <= 16, 1 for <= 32, ... 6 for <= 1024
+static int knn_class(uint32_t k) {
+    int c = 0;
+    while (c < 6 && (16u << c) < k) c++;
+    return c;
+}
+// lanes per query for a kNN launch: explicit override, else autotuned, else the size class default
+static int knn_lanes(const oct_ctx *ctx, int precision, uint32_t k) {
+    if (ctx->knn_lanes_override > 0) return ctx->knn_lanes_override;
+    const int t = ctx->knn_lanes_tuned[precision & 1][knn_class(k)];
+    return t > 0 ? t : 0;
+}
+
 int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *center_idx, size_t nq, uint32_t k,
             int32_t *idx_out, double *d2_out, oct_search_stats *stats) {
     if (!ctx) return OCT_ERR_ARG;
@@ -668,6 +689,7 @@ int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *
     L.nq = nq, L.k = k;
     L.idx = ctx->kidx.p, L.d2 = ctx->kd2.p;
     L.stats = ctx->stats.p;
+    L.lanes = knn_lanes(ctx, precision, k);
     std::string err;
     hipError_t e = launch_knn(ctx, L, err);
     if (e != hipSuccess) return err.empty() ? hfail(ctx, e, "knn") : fail(ctx, OCT_ERR_ARG, err);
@@ -701,6 +723,7 @@ int oct_knn_dev(oct_ctx *ctx, int precision, const uint32_t *center_idx_dev, siz
     L.nq = nq, L.k = k;
     L.idx = idx_dev ? idx_dev : ctx->kidx.p;
     L.d2f = d2_dev ? d2_dev : ctx->kd2f.p;
+    L.lanes = knn_lanes(ctx, precision, k);
     if (stats) {
         HCHK(ctx->stats.reserve(8), "alloc stats");
         HCHK(hipMemsetAsync(ctx->stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream), "clear stats");
@@ -725,6 +748,67 @@ int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d
     if (idx_out && cnt) HCHK(hipMemcpyAsync(idx_out, ctx->kidx.p + off, cnt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy idx");
     if (d2_out && cnt) HCHK(hipMemcpyAsync(d2_out, ctx->kd2f.p + off, cnt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy d2");
     HCHK(hipStreamSynchronize(ctx->stream), "knn fetch");
+    return OCT_OK;
+}
+
+int oct_autotune_knn(oct_ctx *ctx, int precision, uint32_t k, size_t sample_nq, int *best, double ms_out[3]) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (precision != OCT_PREC_FP64 && precision != OCT_PREC_FP32_RECHECK) return fail(ctx, OCT_ERR_ARG, "unknown precision");
+    if (!ctx->has_tree) return fail(ctx, OCT_ERR_STATE, "knn autotune before oct_build");
+    if (k == 0 || k > 1024) return fail(ctx, OCT_ERR_ARG, "knn: k must be 1 .. 1024");
+    if (ctx->n == 0) return fail(ctx, OCT_ERR_STATE, "autotune on an empty cloud");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    const size_t nq = std::min(sample_nq ? sample_nq : (size_t)1 << 18, ctx->n);
+    // 16 chunks of consecutive SFC positions spread over the cloud
+    std::vector<uint32_t> idx(nq);
+    const size_t chunks = 16, per = (nq + chunks - 1) / chunks;
+    size_t n = 0;
+    for (size_t c = 0; c < chunks && n < nq; c++) {
+        const size_t start = (ctx->n / chunks) * c;
+        for (size_t i = 0; i < per && n < nq; i++) idx[n++] = (uint32_t)std::min(start + i, ctx->n - 1);
+    }
+    HCHK(ctx->qidx.reserve(nq + 1), "alloc queries");
+    HCHK(ctx->kidx.reserve(nq * k + 1), "alloc knn");
+    HCHK(ctx->kd2f.reserve(nq * k + 1), "alloc knn");
+    ctx->knn_nq = 0, ctx->knn_k = 0;
+    HCHK(hipMemcpyAsync(ctx->qidx.p, idx.data(), nq * 4, hipMemcpyHostToDevice, ctx->stream), "copy sample");
+    int opts[3];
+    const int no = knn_lane_options(k, opts);
+    double bestms = 1e300;
+    int bests = opts[0];
+    for (int o = 0; o < 3; o++) {
+        if (ms_out) ms_out[o] = 1e300;
+        if (o >= no) continue;
+        KnnLaunch L;
+        L.precision = precision;
+        L.qidx = ctx->qidx.p;
+        L.nq = nq, L.k = k;
+        L.idx = ctx->kidx.p, L.d2f = ctx->kd2f.p;
+        L.lanes = opts[o];
+        double t = 1e300;
+        for (int rep = 0; rep < 3; rep++) {  // first repetition warms caches and code objects
+            std::string err;
+            hipError_t e = launch_knn(ctx, L, err);
+            if (e != hipSuccess) return err.empty() ? hfail(ctx, e, "knn autotune") : fail(ctx, OCT_ERR_ARG, err);
+            HCHK(hipStreamSynchronize(ctx->stream), "knn autotune");
+            if (rep) t = std::min(t, stage_ms(ctx, ST_KNN));
+        }
+        if (ms_out) ms_out[o] = t;
+        if (t < bestms) bestms = t, bests = opts[o];
+    }
+    ctx->knn_lanes_tuned[precision & 1][knn_class(k)] = bests;
+    if (best) *best = bests;
+    return OCT_OK;
+}
+
+int oct_tuned_knn(oct_ctx *ctx, int precision, uint32_t k, int *lanes) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (precision != OCT_PREC_FP64 && precision != OCT_PREC_FP32_RECHECK) return fail(ctx, OCT_ERR_ARG, "unknown precision");
+    if (k == 0 || k > 1024) return fail(ctx, OCT_ERR_ARG, "knn: k must be 1 .. 1024");
+    int opts[3];
+    knn_lane_options(k, opts);
+    const int l = knn_lanes(ctx, precision, k);
+    if (lanes) *lanes = l > 0 ? l : opts[0];
     return OCT_OK;
 }
 

@@ -131,6 +131,8 @@ struct oct_ctx {
     oct::DevBuf<float> kd2f;        // oct_knn_dev results kept in the context
     size_t knn_nq = 0;              // rows of the kept oct_knn_dev results
     uint32_t knn_k = 0;
+    int knn_lanes_override = -1;    // OCT_TUNE_KNN_LANES (-1: tuned / default)
+    int knn_lanes_tuned[2][8];      // autotuned lanes per query per (precision, size class), -1 = not tuned
 
     hipEvent_t ev0[oct::ST_N] = {}, ev1[oct::ST_N] = {};
     bool ev_used[oct::ST_N] = {};
@@ -183,8 +185,11 @@ struct KnnLaunch {
     double *d2 = nullptr;
     float *d2f = nullptr;
     unsigned long long *stats = nullptr;
+    int lanes = 0;  // lanes per query (0: the size class's default)
 };
 hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err);
+// lanes-per-query shapes offered for k (default first); returns how many
+int knn_lane_options(uint32_t k, int out[3]);
 hipError_t exclusive_scan_u32_to_u64(oct_ctx *ctx, const uint32_t *in, uint64_t *out, size_t n);
 hipError_t exclusive_scan_u32(oct_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev);
 

@@ -109,10 +109,11 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
     double tau = __builtin_huge_val(), tau_pr = __builtin_huge_val();
     uint32_t tau_i = 0xffffffffu;
     float tau32 = __builtin_huge_valf();  // tau rounded up to FP32
-    uint64_t n_nodes = 0, n_test = 0, n_recheck = 0;
+    uint64_t n_nodes = 0, n_test = 0, n_recheck = 0, n_select = 0, n_steps = 0;
 
     // sort every query's buffer, keep its k best
     auto select = [&]() {
+        n_select++;
         wsync();
 #pragma unroll 1
         for (uint32_t size = 2; size <= (uint32_t)CAP; size <<= 1) {
@@ -172,6 +173,7 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
             wsync();
             for (uint32_t s = 0; s < nsteps; s++) {
                 if (ballot64(need && cnt + STEP > (uint32_t)CAP)) select();
+                n_steps++;
                 float thr = __builtin_huge_valf();
                 if (F32 && full && !force) thr = ((tau32 + hw) * (1.0f + 0x1p-20f));
                 double d[U];
@@ -313,6 +315,8 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
             atomicAdd(&a.stats[1], (unsigned long long)n_test);
             atomicAdd(&a.stats[2], (unsigned long long)n_recheck);
             atomicAdd(&a.stats[3], (unsigned long long)found);
+            atomicAdd(&a.stats[4], (unsigned long long)n_select);  // wave-wide selections
+            atomicAdd(&a.stats[5], (unsigned long long)n_steps);   // wave-wide leaf steps
         }
     }
 }
@@ -324,6 +328,42 @@ static void launch_sk(const KnnArgs &a, int precision, hipStream_t s) {
         k_knn<S, KC, OCT_PREC_FP32_RECHECK><<<nb, 64, 0, s>>>(a);
     else
         k_knn<S, KC, OCT_PREC_FP64><<<nb, 64, 0, s>>>(a);
+}
+
+// buffer size class of k (entries per query kept between selections)
+static int knn_kc(uint32_t k) {
+    int kc = 16;
+    while ((uint32_t)kc < k) kc <<= 1;
+    return kc;
+}
+
+// lanes per query offered for each size class, default first
+int knn_lane_options(uint32_t k, int out[3]) {
+    switch (knn_kc(k)) {
+        case 16: out[0] = 4, out[1] = 2, out[2] = 8; return 3;
+        case 32: out[0] = 4, out[1] = 2, out[2] = 8; return 3;
+        case 64: out[0] = 8, out[1] = 4, out[2] = 16; return 3;
+        case 128: out[0] = 16, out[1] = 8, out[2] = 32; return 3;
+        case 256: out[0] = 64, out[1] = 32, out[2] = 16; return 3;
+        default: out[0] = 64; return 1;
+    }
+}
+
+static bool launch_knn_shape(const KnnArgs &a, int precision, int lanes, hipStream_t s) {
+#define OCT_KNN_CASE(KCV, SV)                     \
+    if (kc == KCV && lanes == SV) {               \
+        launch_sk<SV, KCV>(a, precision, s);      \
+        return true;                              \
+    }
+    const int kc = knn_kc(a.k);
+    OCT_KNN_CASE(16, 2) OCT_KNN_CASE(16, 4) OCT_KNN_CASE(16, 8)
+    OCT_KNN_CASE(32, 2) OCT_KNN_CASE(32, 4) OCT_KNN_CASE(32, 8)
+    OCT_KNN_CASE(64, 4) OCT_KNN_CASE(64, 8) OCT_KNN_CASE(64, 16)
+    OCT_KNN_CASE(128, 8) OCT_KNN_CASE(128, 16) OCT_KNN_CASE(128, 32)
+    OCT_KNN_CASE(256, 16) OCT_KNN_CASE(256, 32) OCT_KNN_CASE(256, 64)
+    OCT_KNN_CASE(512, 64) OCT_KNN_CASE(1024, 64)
+#undef OCT_KNN_CASE
+    return false;
 }
 
 hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err) {
@@ -359,21 +399,14 @@ hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err) {
     for (int l = 0; l < 22; l++)
         for (int c = 0; c < 3; c++) a.rad[l][c] = ctx->level_radii[l][c];
     for (int c = 0; c < 3; c++) a.mag[c] = fabs(ctx->center[c]) + fabs(ctx->radii[c]);
+    int opts[3];
+    knn_lane_options(k, opts);
+    const int lanes = L.lanes > 0 ? L.lanes : opts[0];
     stage_begin(ctx, ST_KNN);
-    if (k <= 16)
-        launch_sk<4, 16>(a, precision, ctx->stream);
-    else if (k <= 32)
-        launch_sk<4, 32>(a, precision, ctx->stream);
-    else if (k <= 64)
-        launch_sk<8, 64>(a, precision, ctx->stream);
-    else if (k <= 128)
-        launch_sk<16, 128>(a, precision, ctx->stream);
-    else if (k <= 256)
-        launch_sk<32, 256>(a, precision, ctx->stream);
-    else if (k <= 512)
-        launch_sk<64, 512>(a, precision, ctx->stream);
-    else
-        launch_sk<64, 1024>(a, precision, ctx->stream);
+    if (!launch_knn_shape(a, precision, lanes, ctx->stream)) {
+        err = "knn: lanes per query not offered for this k";
+        return hipErrorInvalidValue;
+    }
     stage_end(ctx, ST_KNN);
     return hipGetLastError();
 }

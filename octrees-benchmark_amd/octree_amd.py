@@ -23,7 +23,7 @@ STAGES = ("bbox", "encode", "sort", "gather", "build", "search", "knn")
 EXPORTS = (
     "oct_create", "oct_destroy", "oct_last_error", "oct_synchronize", "oct_version", "oct_set_points",
     "oct_set_points_dev", "oct_bbox", "oct_encode", "oct_sort", "oct_sort_points", "oct_set_sorted", "oct_build",
-    "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_knn_dev", "oct_knn_fetch", "oct_stage_times",
+    "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_knn_dev", "oct_knn_fetch", "oct_autotune_knn", "oct_tuned_knn", "oct_stage_times",
     "oct_set_tuning", "oct_autotune_radius", "oct_tuned_radius", "oct_radius_box", "oct_encode_coords",
     "oct_decode_keys",
 )
@@ -82,6 +82,8 @@ def lib():
         L.oct_knn.argtypes = [_vp, C.c_int, _vp, _vp, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_knn_dev.argtypes = [_vp, C.c_int, _vp, C.c_size_t, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_knn_fetch.argtypes = [_vp, C.c_size_t, C.c_size_t, _vp, _vp]
+        L.oct_autotune_knn.argtypes = [_vp, C.c_int, C.c_uint32, C.c_size_t, C.POINTER(C.c_int), _vp]
+        L.oct_tuned_knn.argtypes = [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_int)]
         L.oct_stage_times.argtypes = [_vp, _vp]
         L.oct_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
         L.oct_autotune_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_size_t,
@@ -337,6 +339,21 @@ class Octree:
         od = np.empty((nq, k), np.float32)
         self._chk(self.L.oct_knn_fetch(self.h, q0, nq, _ptr(oi), _ptr(od)), "oct_knn_fetch")
         return oi, od
+
+    def set_knn_lanes(self, lanes):
+        """Lanes per query of the kNN kernel (None: autotuned / default)."""
+        self._chk(self.L.oct_set_tuning(self.h, 3, -1 if lanes is None else int(lanes)), "oct_set_tuning")
+
+    def autotune_knn(self, k, precision=PREC_FP32, sample_nq=0):
+        best = C.c_int(0)
+        ms = np.zeros(3)
+        self._chk(self.L.oct_autotune_knn(self.h, precision, k, sample_nq, C.byref(best), _ptr(ms)), "oct_autotune_knn")
+        return best.value, ms
+
+    def tuned_knn(self, k, precision=PREC_FP32):
+        lanes = C.c_int(0)
+        self._chk(self.L.oct_tuned_knn(self.h, precision, k, C.byref(lanes)), "oct_tuned_knn")
+        return lanes.value
 
     def set_group(self, g):
         """Queries per wave of the count-mode radius kernel (0, 16, 32, 64; -1 = tuned/default)."""

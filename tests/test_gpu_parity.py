@@ -196,6 +196,41 @@ def test_knn_exact(ctx, k, prec):
         assert np.array_equal(hd.view(np.uint64), bd.view(np.uint64))
 
 
+@pytest.mark.parametrize("k,lanes", [(10, 2), (10, 8), (30, 2), (30, 8), (50, 4), (50, 16), (100, 8), (100, 32),
+                                     (250, 16), (250, 32)])
+def test_knn_every_shape(ctx, k, lanes):
+    """Every lanes-per-query shape the kNN kernel offers returns the same exact sets."""
+    say(f"[gpu] knn shape k={k} lanes={lanes}")
+    sx, keys, perm, c, r, tree = knn_case(ctx)
+    idx = A.random_indices(300, len(sx), seed=k + lanes)
+    bi, bd = O.knn_brute(sx, sx[idx.astype(np.int64)], k)
+    try:
+        ctx.set_knn_lanes(lanes)
+        for prec in (A.PREC_FP64, A.PREC_FP32):
+            gi, gd, _ = ctx.knn(k, precision=prec, idx=idx)
+            np.testing.assert_array_equal(gi.astype(np.int64), bi)
+            assert np.array_equal(gd.view(np.uint64), bd.view(np.uint64))
+    finally:
+        ctx.set_knn_lanes(None)
+
+
+def test_knn_autotune_keeps_results(ctx):
+    sx, keys, perm, c, r, tree = knn_case(ctx)
+    best, ms = ctx.autotune_knn(20, A.PREC_FP32, sample_nq=20_000)
+    assert best in (2, 4, 8) and ctx.tuned_knn(20, A.PREC_FP32) == best
+    assert np.all(np.isfinite(ms))
+    idx = A.random_indices(300, len(sx), seed=3)
+    gi, gd, _ = ctx.knn(20, precision=A.PREC_FP32, idx=idx)
+    bi, bd = O.knn_brute(sx, sx[idx.astype(np.int64)], 20)
+    np.testing.assert_array_equal(gi.astype(np.int64), bi)
+    with pytest.raises(A.OctError):
+        ctx.set_knn_lanes(64)
+        try:
+            ctx.knn(20, precision=A.PREC_FP32, idx=idx)  # 64 lanes is not offered for k <= 32
+        finally:
+            ctx.set_knn_lanes(None)
+
+
 @pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("k", [10, 50])
 def test_knn_dev_all_points(ctx, k, prec):
