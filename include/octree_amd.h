@@ -182,6 +182,9 @@ int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d
  * strided chunks of consecutive SFC points; sample_nq >= n uses every point), stores the fastest for
  * (kernel, algo, precision), returns its G in *best and the best milliseconds per G in ms_out[0..3].
  * oct_tuned_radius: the G and point-test unit a count-mode search would use now.
+ * OCT_TUNE_RADIUS_REMAP: 1 (default, also -1) lets the count-mode vector point tests move the
+ * queries of a leaf that at most half of a wave's queries want onto all 64 lanes; 0 keeps every
+ * query on its own lanes.
  * OCT_TUNE_KNN_LANES: lanes per query of the kNN kernel (64 / lanes queries share one wave's
  * traversal); each k size class (k <= 16, 32, 64, ... 1024) offers up to three shapes, the
  * default first (k <= 32: 4, 2, 8; <= 64: 8, 4, 16; <= 128: 16, 8, 32; <= 256: 64, 32, 16;
@@ -191,7 +194,7 @@ int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d
  * size class of k), returns it in *best and the best milliseconds per shape in ms_out[0..2].
  * oct_tuned_knn: the lanes per query a kNN call with this k would use now. Results never depend on
  * the tuning. */
-enum oct_tune_knob { OCT_TUNE_RADIUS_GROUP = 1, OCT_TUNE_RADIUS_MFMA = 2, OCT_TUNE_KNN_LANES = 3 };
+enum oct_tune_knob { OCT_TUNE_RADIUS_GROUP = 1, OCT_TUNE_RADIUS_MFMA = 2, OCT_TUNE_KNN_LANES = 3, OCT_TUNE_RADIUS_REMAP = 4 };
 int oct_set_tuning(oct_ctx *ctx, int knob, int value);
 int oct_autotune_radius(oct_ctx *ctx, int kernel, int algo, int precision, double radius, size_t sample_nq, int *best,
                         double ms_out[4]);

@@ -418,6 +418,7 @@ static int radius_impl(oct_ctx *ctx, int kernel, int algo, int precision, const 
     L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = want_lists ? 1 : 0;
     L.group = radius_group(ctx, kernel, algo, precision);
     L.mx = radius_mx(ctx, kernel, algo, precision);
+    L.remap = ctx->remap_override != 0;
     L.order = d_order;
     L.qidx = center_idx ? d_idx : nullptr;
     L.qxyz = centers ? ctx->qxyz.p : nullptr;
@@ -542,6 +543,7 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
     L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0;
     L.group = radius_group(ctx, kernel, algo, precision);
     L.mx = radius_mx(ctx, kernel, algo, precision);
+    L.remap = ctx->remap_override != 0;
     L.qidx = center_idx_dev;
     L.radius = radius;
     L.nq = nq;
@@ -565,6 +567,11 @@ int oct_set_tuning(oct_ctx *ctx, int knob, int value) {
     if (knob == OCT_TUNE_RADIUS_MFMA) {
         if (value != -1 && value != 0 && value != 1) return fail(ctx, OCT_ERR_ARG, "radius mfma switch must be -1, 0 or 1");
         ctx->mx_override = value;
+        return OCT_OK;
+    }
+    if (knob == OCT_TUNE_RADIUS_REMAP) {
+        if (value != -1 && value != 0 && value != 1) return fail(ctx, OCT_ERR_ARG, "radius remap switch must be -1, 0 or 1");
+        ctx->remap_override = value;
         return OCT_OK;
     }
     if (knob == OCT_TUNE_KNN_LANES) {
@@ -613,6 +620,7 @@ int oct_autotune_radius(oct_ctx *ctx, int kernel, int algo, int precision, doubl
             RadiusLaunch L;
             L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = cand[ci];
             L.mx = mx != 0;
+            L.remap = ctx->remap_override != 0;
             L.qidx = ctx->qidx.p;
             L.radius = radius;
             L.nq = nq;

@@ -20,6 +20,7 @@
 #include <cfloat>
 
 #include "device.hpp"
+#include "search_common.hpp"
 
 namespace oct {
 
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
         const uint32_t lv = uniform_u32(s_lev[sp]);
         if (lv & 0x80u) {
             // a leaf: entry = NodeExt index * 8 + combo; re-test it against the current k-th distances
-            const NodeExt &E = a.next[ent >> 3];
+            ConstNodeExt &E = node_ext(a.next, ent >> 3);
             const uint32_t c = ent & 7u, L = lv & 31u;
             const double cx = E.ax[c & 1u], cy = E.ax[2 + ((c >> 1) & 1u)], cz = E.ax[4 + (c >> 2)];
             const double lb = (gap2(qx, cx, a.rad[L][0], dlx) + gap2(qy, cy, a.rad[L][1], dly)) + gap2(qz, cz, a.rad[L][2], dlz);
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
             continue;
         }
         // an internal node at level lv: judge its 8 children (level lv + 1) from per-axis terms
-        const NodeExt &E = a.next[ent];
+        ConstNodeExt &E = node_ext(a.next, ent);
         const uint32_t d = lv + 1;
         const double rx = a.rad[d][0], ry = a.rad[d][1], rz = a.rad[d][2];
         double gx[2], gy[2], gz[2];

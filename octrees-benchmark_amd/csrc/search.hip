@@ -328,11 +328,13 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     static_assert(!MX || (kFp32 && KERNEL == OCT_KERNEL_SPHERE), "matrix-core leaf tests are FP32 sphere only");
     constexpr int NH = MX ? G / 16 : 1;  // 16-query column blocks per wave
     constexpr int CAP = kFp32 ? kStageBytes / 16 : kStageBytes / 32;  // real points per chunk
-    constexpr int PAD = kUnroll * S;                                   // sentinel slots after a chunk
+    constexpr int PAD = kUnroll * (S > 16 ? S : 16);                   // sentinel slots after a chunk (stride <= max(S, 16))
     __shared__ uint32_t s_node[kWaves][kStack];
     __shared__ uint64_t s_mask[kWaves][kStack];
     __shared__ uint8_t s_depth[kWaves][kStack];
     __shared__ uint64_t s_leafm[kWaves][8];  // OVERLAP leaf children of the popped node (query masks)
+    __shared__ uint8_t s_rmap[kWaves][64];   // remapped leaf: home slot of compacted query r
+    __shared__ uint32_t s_rtot[kWaves][64];  // remapped leaf: count of compacted query r
     __shared__ __attribute__((aligned(16))) unsigned char s_stage[kWaves][kStageBytes + PAD * (kFp32 ? 16 : 32)];
 
     const uint32_t w = threadIdx.x >> 6;
@@ -395,12 +397,12 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         return classify64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]);
     };
 
-    // stage points [base, base + P) of the sorted cloud, then sentinels up to a multiple of kUnroll*S
+    // stage points [base, base + P) of the sorted cloud, then sentinels up to a multiple of kUnroll*st
     // All of a chunk's loads are issued before any LDS write (one round trip); loads read a clamped
     // in-range point and the sentinel is selected by value.
-    auto stage = [&](uint32_t base, uint32_t P) -> uint32_t {
-        const uint32_t iters = (P + S * kUnroll - 1) / (S * kUnroll) * kUnroll;  // per-lane iterations
-        const uint32_t slots = iters * S;
+    auto stage = [&](uint32_t base, uint32_t P, uint32_t st) -> uint32_t {
+        const uint32_t iters = (P + st * kUnroll - 1) / (st * kUnroll) * kUnroll;  // per-lane iterations
+        const uint32_t slots = iters * st;
         constexpr int NI = (CAP + PAD + 63) / 64;  // slots <= CAP + PAD
         if (kFp32) {
             // sentinel: t = 1e30 + C (sphere: w = 1e30) or +inf (circle: x = y = 1e30), never
@@ -547,38 +549,72 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         }
     };
 
-    // leaf: queries in m count the leaf's points; lane group g takes points g, g + S, ...
+    // leaf: the queries in m count the leaf's points. Normally lane group g of each query takes
+    // points g, g + S, ... . A leaf wanted by at most half of the wave's queries is remapped: its mc
+    // queries are compacted onto groups of Sp = 64 / Gp lanes (Gp = max(4, mc rounded up to a power
+    // of 2), so Sp <= 16); lane l serves compacted query r = l % Gp and takes points l / Gp, + Sp, ...;
+    // the query's terms are computed in its own lane and fetched with ds_bpermute, and each group's
+    // count returns to its query's first lane through LDS. Same tests and band either way.
     auto leaf = [&](uint64_t m, float am, const NodeRec &nr) {
         const uint32_t first = nr.first, last = nr.last;
-        const bool act = (m & qbit) != 0 && valid;
-        if (STATS) {
-            n_test += (uint64_t)__popcll(m) * (last - first);
-            if (!MX) n_tslots += 64ull * ((last - first + S - 1) / S);
-        }
+        if (STATS) n_test += (uint64_t)__popcll(m) * (last - first);
         if constexpr (MX) {
             leaf_mx(m, am, nr, first, last);
             return;
         }
+        const uint32_t mc = (uint32_t)__popcll(m);
+        const bool rm = a.remap && 2 * mc <= (uint32_t)G;  // wave-uniform
+        uint32_t lg = 0, Sp = S, gp = g, src = lane, rank = 0;
+        bool ract = (m & qbit) != 0 && valid;
+        const bool home = rm && g == 0 && ((m >> j) & 1ull) != 0;
+        if (rm) {
+            lg = mc <= 4 ? 2u : 32u - (uint32_t)__builtin_clz(mc - 1);
+            Sp = 64u >> lg;
+            gp = lane >> lg;
+            const uint32_t r = lane & ((1u << lg) - 1u);
+            rank = (uint32_t)__popcll(m & (qbit - 1ull));
+            if (home) s_rmap[w][rank] = (uint8_t)j;  // slot j's first lane is lane j
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            ract = r < mc;
+            src = ract ? (uint32_t)s_rmap[w][r] : lane;
+        }
+        // a value of the served query: its own lane's, or fetched from the query's lane
+        auto qf = [&](float v) -> float { return rm ? bperm_f32(src, v) : v; };
+        auto qd = [&](double v) -> double { return rm ? bperm_f64(src, v) : v; };
+        auto fetch_query = [&]() {
+            Query qr = q;
+            if (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE) {
+                qr.x = qd(q.x), qr.y = qd(q.y), qr.R = qd(q.R);
+                if (KERNEL == OCT_KERNEL_SPHERE) qr.z = qd(q.z);
+            } else {
+                qr.lox = qd(q.lox), qr.hix = qd(q.hix), qr.loy = qd(q.loy), qr.hiy = qd(q.hiy);
+                if (KERNEL == OCT_KERNEL_CUBE) qr.loz = qd(q.loz), qr.hiz = qd(q.hiz);
+            }
+            return qr;
+        };
         uint32_t lc = 0;
         if (kFp32) {
             // t = w - 2 a.b + C approximates u + hw (the band of leaf_mx, same set-up): t <= 0 (sign
-            // bit set) is inside, 0 <= t <= 2 hw (unsigned compare of the bit patterns) is re-tested
+            // bit set) is inside, 0 <= t <= 2 hw (unsigned compare of the bit patterns) is re-tested;
+            // a forced FP64 re-test is lim = ~0
             const float bx = (float)(q.x - nr.cx), by = (float)(q.y - nr.cy), bz = (float)(q.z - nr.cz);
             const float b2 = KERNEL == OCT_KERNEL_SPHERE ? __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz))
                                                          : __builtin_fmaf(bx, bx, by * by);
             const float sum = (am + b2) + qRf;
             const float hw = 3.814697265625e-06f * sum;  // 2^-18
-            const float C = (hw - qRf) + b2;
-            const uint32_t lim = __float_as_uint(2.0f * hw);
-            const bool force = !(sum < 1e30f);
-            const float ux = -2.0f * bx, uy = -2.0f * by, uz = -2.0f * bz;
+            const float C = qf((hw - qRf) + b2);
+            const uint32_t lim = __float_as_uint(qf(__uint_as_float(sum < 1e30f ? __float_as_uint(2.0f * hw) : 0xffffffffu)));
+            const float ux = qf(-2.0f * bx), uy = qf(-2.0f * by);
+            const float uz = KERNEL == OCT_KERNEL_SPHERE ? qf(-2.0f * bz) : 0.0f;
             uint32_t c8 = 0, mnb = 0xffffffffu;
             for (uint32_t base = first; base < last; base += CAP) {
                 const uint32_t P = (last - base) < (uint32_t)CAP ? (last - base) : (uint32_t)CAP;
-                const uint32_t iters = stage(base, P);
-                const float4 *pl = sp32 + g;
-                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * S) {
-                    const float4 p0 = pl[0], p1 = pl[S], p2 = pl[2 * S], p3 = pl[3 * S];
+                const uint32_t iters = stage(base, P, Sp);
+                const float4 *pl = sp32 + gp;
+                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * Sp) {
+                    const float4 p0 = pl[0], p1 = pl[Sp], p2 = pl[2 * Sp], p3 = pl[3 * Sp];
                     float t0, t1, t2, t3;
                     if (KERNEL == OCT_KERNEL_SPHERE) {
                         t0 = __builtin_fmaf(p0.x, ux, __builtin_fmaf(p0.y, uy, __builtin_fmaf(p0.z, uz, p0.w + C)));
@@ -602,36 +638,48 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
                 __builtin_amdgcn_wave_barrier();
             }
             lc = c8 >> 3;
-            const bool band = act && (mnb <= lim || force);
+            const bool band = ract && mnb <= lim;
             if (ballot64(band)) {  // rare: exact FP64 re-count of this lane's share of the leaf
+                const Query qr = fetch_query();
                 if (band) {
                     lc = 0;
-                    for (uint32_t i = first + g; i < last; i += S) {
+                    for (uint32_t i = first + gp; i < last; i += Sp) {
                         const double4 pd = a.p64[i];
-                        lc += inside64<KERNEL>(q, pd.x, pd.y, pd.z);
+                        lc += inside64<KERNEL>(qr, pd.x, pd.y, pd.z);
                         if (STATS) n_recheck++;
                     }
                 }
             }
         } else {
+            const Query qr = fetch_query();
             for (uint32_t base = first; base < last; base += CAP) {
                 const uint32_t P = (last - base) < (uint32_t)CAP ? (last - base) : (uint32_t)CAP;
-                const uint32_t iters = stage(base, P);
-                const double4 *pl = sp64 + g;
+                const uint32_t iters = stage(base, P, Sp);
+                const double4 *pl = sp64 + gp;
                 uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * S) {
-                    const double4 p0 = pl[0], p1 = pl[S], p2 = pl[2 * S], p3 = pl[3 * S];
-                    c0 += inside64<KERNEL>(q, p0.x, p0.y, p0.z) ? 1u : 0u;
-                    c1 += inside64<KERNEL>(q, p1.x, p1.y, p1.z) ? 1u : 0u;
-                    c2 += inside64<KERNEL>(q, p2.x, p2.y, p2.z) ? 1u : 0u;
-                    c3 += inside64<KERNEL>(q, p3.x, p3.y, p3.z) ? 1u : 0u;
+                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * Sp) {
+                    const double4 p0 = pl[0], p1 = pl[Sp], p2 = pl[2 * Sp], p3 = pl[3 * Sp];
+                    c0 += inside64<KERNEL>(qr, p0.x, p0.y, p0.z) ? 1u : 0u;
+                    c1 += inside64<KERNEL>(qr, p1.x, p1.y, p1.z) ? 1u : 0u;
+                    c2 += inside64<KERNEL>(qr, p2.x, p2.y, p2.z) ? 1u : 0u;
+                    c3 += inside64<KERNEL>(qr, p3.x, p3.y, p3.z) ? 1u : 0u;
                 }
                 lc += (c0 + c1) + (c2 + c3);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
         }
-        if (act) cnt += lc;
+        if (!ract) lc = 0;
+        if (rm) {
+            for (uint32_t off = 1u << lg; off < 64; off <<= 1) lc += __shfl_xor(lc, (int)off);
+            if (ract && gp == 0) s_rtot[w][lane] = lc;  // lane = r for the group's first lane
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            lc = home ? s_rtot[w][rank] : 0u;
+        }
+        cnt += lc;
+        if (STATS) n_tslots += 64ull * ((last - first + Sp - 1) / Sp);
     };
 
     // The 8 children of a popped node are judged from per-axis terms: they take two centre values
@@ -640,7 +688,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     // sphere (ex*ex + ey*ey) + ez*ez with ex = |q.x - cx| + rx (kernel_3d.hpp:181-223). Each lane
     // judges all 8 children for its own query (the S lanes of a query agree); results are lane
     // masks, so the per-child decisions are scalar-unit work.
-    auto judge8 = [&](const NodeExt &E, uint32_t d, bool act, bool (&in)[8], bool (&ov)[8]) {
+    auto judge8 = [&](ConstNodeExt &E, uint32_t d, bool act, bool (&in)[8], bool (&ov)[8]) {
         const double rx = a.rad[d][0], ry = a.rad[d][1], rz = a.rad[d][2];
         constexpr bool is3d = (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CUBE);
         if (ALGO == OCT_ALGO_NEIGHBORS || KERNEL == OCT_KERNEL_CUBE || KERNEL == OCT_KERNEL_SQUARE) {
@@ -732,7 +780,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         const uint32_t fc = uniform_u32(s_node[w][sp]);
         const uint64_t m = uniform_u64(s_mask[w][sp]);
         const uint32_t d = uniform_u32(s_depth[w][sp]) + 1;  // the children's level
-        const NodeExt &E = a.next[(fc - 1) >> 3];
+        ConstNodeExt &E = node_ext(a.next, (fc - 1) >> 3);
         const bool inm = (m & qbit) != 0;
         bool in[8], ovl[8];
         judge8(E, d, inm, in, ovl);
@@ -880,6 +928,7 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     a.qrad3 = L.qrad3;
     for (int k = 0; k < 3; k++) a.r3[k] = L.radius3[k];
     a.use_r3 = L.use_r3 ? 1 : 0;
+    a.remap = L.remap ? 1 : 0;
     a.nq = (uint32_t)L.nq;
     a.counts = L.counts;
     a.ecounts = L.ecounts;

@@ -27,6 +27,7 @@ struct SearchArgs {
     const double *__restrict__ qrad3;
     double r3[3];
     int use_r3;
+    int remap;  // count kernel: compact the queries of a leaf wanted by at most half the wave
     uint32_t nq;
     uint32_t *__restrict__ counts;
     uint32_t *__restrict__ ecounts;
@@ -40,6 +41,13 @@ struct Query {
     double x, y, z, r, R;
     double lox, loy, loz, hix, hiy, hiz;  // kernel box: center -/+ radius (kernel_abstract.hpp:41-45)
 };
+
+// Read-only tree records through the constant address space: a wave-uniform index then compiles
+// to scalar loads (the records land in SGPRs, not in 64 copies across the vector registers).
+typedef const __attribute__((address_space(4))) NodeExt ConstNodeExt;
+__device__ inline ConstNodeExt &node_ext(const NodeExt *p, uint32_t i) {
+    return ((ConstNodeExt *)(uintptr_t)p)[i];
+}
 
 __device__ inline double dmax0(double a) { return (a < 0.0) ? 0.0 : a; }  // std::max(a, 0.0)
 

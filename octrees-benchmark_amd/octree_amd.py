@@ -340,6 +340,10 @@ class Octree:
         self._chk(self.L.oct_knn_fetch(self.h, q0, nq, _ptr(oi), _ptr(od)), "oct_knn_fetch")
         return oi, od
 
+    def set_remap(self, on):
+        """Count-mode leaf remapping of sparsely wanted leaves (None: default, on)."""
+        self._chk(self.L.oct_set_tuning(self.h, 4, -1 if on is None else (1 if on else 0)), "oct_set_tuning")
+
     def set_knn_lanes(self, lanes):
         """Lanes per query of the kNN kernel (None: autotuned / default)."""
         self._chk(self.L.oct_set_tuning(self.h, 3, -1 if lanes is None else int(lanes)), "oct_set_tuning")

@@ -136,12 +136,15 @@ def test_radius_counts_every_group(case, ctx, group, mx):
         for kernel in (A.K_SPHERE, A.K_CIRCLE, A.K_CUBE, A.K_SQUARE):
             for algo in (A.A_NEIGHBORS, A.A_PRUNE, A.A_STRUCT):
                 ref = O.radius_count(tree, sx, kernel, algo, sx[idx.astype(np.int64)], RADII[name][-1])
-                for prec in (A.PREC_FP64, A.PREC_FP32):
-                    counts, st = ctx.radius(kernel, algo, prec, idx=idx, radius=RADII[name][-1])
-                    np.testing.assert_array_equal(counts, ref, err_msg=f"G={group} k={kernel} a={algo} p={prec}")
+                for remap in ((None,) if mx else (None, False)):  # leaf remapping on / off (vector path)
+                    ctx.set_remap(remap)
+                    for prec in (A.PREC_FP64, A.PREC_FP32):
+                        counts, st = ctx.radius(kernel, algo, prec, idx=idx, radius=RADII[name][-1])
+                        np.testing.assert_array_equal(counts, ref, err_msg=f"G={group} k={kernel} a={algo} p={prec} remap={remap}")
     finally:
         ctx.set_group(-1)
         ctx.set_mfma(None)
+        ctx.set_remap(None)
 
 
 @pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32, "fp32-mfma"])
