@@ -275,6 +275,7 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
             if (ballot64(nd)) needbits |= 1u << c;
         }
         if (g == 0) n_nodes += 8 * (valid ? 1 : 0);
+        needbits = uniform_u32(needbits);  // built from ballots: the same in every lane
         if (needbits == 0) continue;
         // push the farthest octants (relative to the reference query) first
         const uint32_t oc = (qx >= 0.5 * (E.ax[0] + E.ax[1]) ? 1u : 0u) | (qy >= 0.5 * (E.ax[2] + E.ax[3]) ? 2u : 0u) |

@@ -328,7 +328,8 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     static_assert(!MX || (kFp32 && KERNEL == OCT_KERNEL_SPHERE), "matrix-core leaf tests are FP32 sphere only");
     constexpr int NH = MX ? G / 16 : 1;  // 16-query column blocks per wave
     constexpr int CAP = kFp32 ? kStageBytes / 16 : kStageBytes / 32;  // real points per chunk
-    constexpr int PAD = kUnroll * (S > 16 ? S : 16);                   // sentinel slots after a chunk (stride <= max(S, 16))
+    constexpr int SMX = S > 16 ? S : 16;                               // largest lane-group count of a leaf
+    constexpr int PAD = kUnroll * SMX + SMX;                           // sentinel + row-padding slots after a chunk
     __shared__ uint32_t s_node[kWaves][kStack];
     __shared__ uint64_t s_mask[kWaves][kStack];
     __shared__ uint8_t s_depth[kWaves][kStack];
@@ -397,13 +398,18 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         return classify64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]);
     };
 
-    // stage points [base, base + P) of the sorted cloud, then sentinels up to a multiple of kUnroll*st
-    // All of a chunk's loads are issued before any LDS write (one round trip); loads read a clamped
-    // in-range point and the sentinel is selected by value.
-    auto stage = [&](uint32_t base, uint32_t P, uint32_t st) -> uint32_t {
+    // Stage points [base, base + P) of the sorted cloud for 2^lg lane groups: point t goes to group
+    // t % 2^lg, row slot t / 2^lg, rows padded with sentinels to iters = a multiple of kUnroll and
+    // spaced iters + 1 slots apart when there are several groups (the spare slot turns the rows'
+    // 16-byte columns across LDS banks). Group gp then reads slots gp * (iters + (lg > 0)) + k at
+    // fixed offsets. All of a chunk's loads are issued before any LDS write (one round trip);
+    // loads read a clamped in-range point and the sentinel is selected by value. Returns iters.
+    auto stage = [&](uint32_t base, uint32_t P, uint32_t lg) -> uint32_t {
+        const uint32_t st = 1u << lg;
         const uint32_t iters = (P + st * kUnroll - 1) / (st * kUnroll) * kUnroll;  // per-lane iterations
         const uint32_t slots = iters * st;
-        constexpr int NI = (CAP + PAD + 63) / 64;  // slots <= CAP + PAD
+        const uint32_t rs = iters + (lg ? 1u : 0u);
+        constexpr int NI = (CAP + kUnroll * SMX + 63) / 64;  // slots <= CAP + kUnroll * SMX - 1
         if (kFp32) {
             // sentinel: t = 1e30 + C (sphere: w = 1e30) or +inf (circle: x = y = 1e30), never
             // inside and above the band
@@ -418,8 +424,8 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
                 const uint32_t t = lane + 64u * i;
                 if (64u * i < slots && t < slots) {
                     const bool real = t < P;
-                    sp32[t] = make_float4(real ? v[i].x : sent.x, real ? v[i].y : sent.y, real ? v[i].z : sent.z,
-                                          real ? v[i].w : sent.w);
+                    sp32[(t & (st - 1u)) * rs + (t >> lg)] =
+                        make_float4(real ? v[i].x : sent.x, real ? v[i].y : sent.y, real ? v[i].z : sent.z, real ? v[i].w : sent.w);
                 }
             }
         } else {
@@ -432,7 +438,8 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
                 const uint32_t t = lane + 64u * i;
                 if (64u * i < slots && t < slots) {
                     const bool real = t < P;
-                    sp64[t] = make_double4(real ? v[i].x : 1e300, real ? v[i].y : 1e300, real ? v[i].z : 1e300, 0.0);
+                    sp64[(t & (st - 1u)) * rs + (t >> lg)] =
+                        make_double4(real ? v[i].x : 1e300, real ? v[i].y : 1e300, real ? v[i].z : 1e300, 0.0);
                 }
             }
         }
@@ -564,11 +571,13 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
         }
         const uint32_t mc = (uint32_t)__popcll(m);
         const bool rm = a.remap && 2 * mc <= (uint32_t)G;  // wave-uniform
-        uint32_t lg = 0, Sp = S, gp = g, src = lane, rank = 0;
+        constexpr uint32_t LS = S == 1 ? 0 : S == 2 ? 1 : S == 4 ? 2 : S == 8 ? 3 : S == 16 ? 4 : S == 32 ? 5 : 6;
+        uint32_t lg = 0, lgs = LS, Sp = S, gp = g, src = lane, rank = 0;
         bool ract = (m & qbit) != 0 && valid;
         const bool home = rm && g == 0 && ((m >> j) & 1ull) != 0;
         if (rm) {
             lg = mc <= 4 ? 2u : 32u - (uint32_t)__builtin_clz(mc - 1);
+            lgs = 6u - lg;
             Sp = 64u >> lg;
             gp = lane >> lg;
             const uint32_t r = lane & ((1u << lg) - 1u);
@@ -611,10 +620,10 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
             uint32_t c8 = 0, mnb = 0xffffffffu;
             for (uint32_t base = first; base < last; base += CAP) {
                 const uint32_t P = (last - base) < (uint32_t)CAP ? (last - base) : (uint32_t)CAP;
-                const uint32_t iters = stage(base, P, Sp);
-                const float4 *pl = sp32 + gp;
-                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * Sp) {
-                    const float4 p0 = pl[0], p1 = pl[Sp], p2 = pl[2 * Sp], p3 = pl[3 * Sp];
+                const uint32_t iters = stage(base, P, lgs);
+                const float4 *pl = sp32 + gp * (iters + (lgs ? 1u : 0u));
+                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll) {
+                    const float4 p0 = pl[0], p1 = pl[1], p2 = pl[2], p3 = pl[3];
                     float t0, t1, t2, t3;
                     if (KERNEL == OCT_KERNEL_SPHERE) {
                         t0 = __builtin_fmaf(p0.x, ux, __builtin_fmaf(p0.y, uy, __builtin_fmaf(p0.z, uz, p0.w + C)));
@@ -654,11 +663,11 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
             const Query qr = fetch_query();
             for (uint32_t base = first; base < last; base += CAP) {
                 const uint32_t P = (last - base) < (uint32_t)CAP ? (last - base) : (uint32_t)CAP;
-                const uint32_t iters = stage(base, P, Sp);
-                const double4 *pl = sp64 + gp;
+                const uint32_t iters = stage(base, P, lgs);
+                const double4 *pl = sp64 + gp * (iters + (lgs ? 1u : 0u));
                 uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll * Sp) {
-                    const double4 p0 = pl[0], p1 = pl[Sp], p2 = pl[2 * Sp], p3 = pl[3 * Sp];
+                for (uint32_t k = 0; k < iters; k += kUnroll, pl += kUnroll) {
+                    const double4 p0 = pl[0], p1 = pl[1], p2 = pl[2], p3 = pl[3];
                     c0 += inside64<KERNEL>(qr, p0.x, p0.y, p0.z) ? 1u : 0u;
                     c1 += inside64<KERNEL>(qr, p1.x, p1.y, p1.z) ? 1u : 0u;
                     c2 += inside64<KERNEL>(qr, p2.x, p2.y, p2.z) ? 1u : 0u;
@@ -805,6 +814,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
             }
         }
         // the leaves, one code copy: their records are scalar loads from E at a uniform index
+        leaves = uniform_u32(leaves);  // built from ballots: the same in every lane
         if (leaves) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
