@@ -321,13 +321,23 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
 // Empty leaves are skipped: they add nothing to a count whatever their class. Counts do not
 // depend on the order of the traversal, which is therefore free.
 
+// Occupancy targets (waves per SIMD), measured on the 50 M-point tile for the sphere kernel with the
+// Struct / Prune algorithms: 7 for the FP32 vector path (72 VGPRs), 6 for FP64 (80 VGPRs). The other
+// variants keep the compiler's choice (forcing them spills registers).
+template <int KERNEL, int ALGO, int PREC, bool MX>
+constexpr int count_waves() {
+    if (MX || ALGO == OCT_ALGO_NEIGHBORS || !(KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE)) return 2;
+    return PREC == OCT_PREC_FP32_RECHECK ? 7 : 6;
+}
 template <int KERNEL, int ALGO, int PREC, int G, bool STATS, bool MX>
-__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_count(const SearchArgs a) {
+__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(count_waves<KERNEL, ALGO, PREC, MX>()))) void k_count(const SearchArgs a) {
     constexpr int S = 64 / G;
     constexpr bool kFp32 = PREC == OCT_PREC_FP32_RECHECK && (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE);
     static_assert(!MX || (kFp32 && KERNEL == OCT_KERNEL_SPHERE), "matrix-core leaf tests are FP32 sphere only");
     constexpr int NH = MX ? G / 16 : 1;  // 16-query column blocks per wave
-    constexpr int CAP = kFp32 ? kStageBytes / 16 : kStageBytes / 32;  // real points per chunk
+    // real points per staging chunk: 64 FP32 or 32 FP64 points (leaves average ~30 points; a small
+    // chunk keeps the LDS per block low enough for 7 (FP32) or 6 (FP64) waves per SIMD)
+    constexpr int CAP = kFp32 ? kStageBytes / 32 : kStageBytes / 64;
     constexpr int SMX = S > 16 ? S : 16;                               // largest lane-group count of a leaf
     constexpr int PAD = kUnroll * SMX + SMX;                           // sentinel + row-padding slots after a chunk
     __shared__ uint32_t s_node[kWaves][kStack];
@@ -336,7 +346,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(2)
     __shared__ uint64_t s_leafm[kWaves][8];  // OVERLAP leaf children of the popped node (query masks)
     __shared__ uint8_t s_rmap[kWaves][64];   // remapped leaf: home slot of compacted query r
     __shared__ uint32_t s_rtot[kWaves][64];  // remapped leaf: count of compacted query r
-    __shared__ __attribute__((aligned(16))) unsigned char s_stage[kWaves][kStageBytes + PAD * (kFp32 ? 16 : 32)];
+    __shared__ __attribute__((aligned(16))) unsigned char s_stage[kWaves][(CAP + PAD) * (kFp32 ? 16 : 32)];
 
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63;

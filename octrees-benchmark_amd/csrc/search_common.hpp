@@ -125,7 +125,7 @@ __device__ inline bool inside64(const Query &q, double px, double py, double pz)
     }
 }
 
-constexpr int kStageBytes = 2048;  // per-wave LDS staging (128 FP32 or 64 FP64 points) + padding
+constexpr int kStageBytes = 2048;  // per-wave LDS staging unit of the search kernels (bytes)
 constexpr int kUnroll = 4;
 
 }  // namespace oct
