@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--max-leaf", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--knn-k", type=int, default=10, help="k of the all-points kNN measured beside the search (0: skip)")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--profile", default=None,
                     help="profiling driver instead of the benchmark: 'search:v64,d64' (one all-points search per "
@@ -165,6 +166,31 @@ def main():
 
     fp32, fp64, fp32v = results["fp32"], results["fp64"], results["fp32_alt"]
     ctx.set_mfma(None)
+
+    # kNN (KNNV2) on the same tile: every point as a query, int32 indices + FP32 squared distances kept
+    # on the device; each precision with its autotuned lanes-per-query shape
+    knn = {}
+    if args.knn_k > 0:
+        k = args.knn_k
+        for name, prec in (("fp32", A.PREC_FP32), ("fp64", A.PREC_FP64)):
+            lanes, cand = ctx.autotune_knn(k, prec, sample_nq=n)  # tuned on the full workload
+            ctx.knn_dev(k, prec, nq=n)
+            sync_all(ctx)
+            t = time.perf_counter()
+            ctx.knn_dev(k, prec, nq=n)
+            sync_all(ctx)
+            wall = time.perf_counter() - t
+            kern = ctx.stage_times()["knn"]
+            st = ctx.knn_dev(k, prec, nq=n, stats=True)  # counters in a run of their own (global atomics)
+            knn[name] = {"queries_per_s": n / wall, "kernel_ms": kern, "lanes_per_query": lanes,
+                         "candidate_ms": cand.tolist(), "points_tested_per_query": st["points_tested"] / n,
+                         "rechecked_per_query": st["rechecked"] / n}
+            log(f"[bench] knn k={k} {name}: {n / wall:.3e} queries/s, kernel {kern:.1f} ms, lanes {lanes}")
+            ksamp = np.arange(0, n, max(1, n // 2000), dtype=np.int64)[:2000]
+            rows = [ctx.knn_fetch(int(q), 1) for q in ksamp]
+            knn[name]["_sample"] = (np.vstack([r[0] for r in rows]), np.vstack([r[1] for r in rows]))
+        knn["k"] = k
+        knn["fp32_vs_fp64_speedup"] = knn["fp64"]["kernel_ms"] / knn["fp32"]["kernel_ms"]
     qps = n / fp32["wall_s"]
     verify = {}
     if not args.no_verify:
@@ -198,6 +224,22 @@ def main():
                 ok &= bool(np.array_equal(cnt, ref_sizes)) and all(np.array_equal(a, b) for a, b in zip(lists, ref))
             verify["lists_eq_oracle_sample"] = ok
             verify["sample_queries"] = int(len(samp))
+            if knn:
+                ksamp = np.arange(0, n, max(1, n // 2000), dtype=np.int64)[:2000]
+                t = time.time()
+                oi, od = O.knn(tree, sx, sx[ksamp], knn["k"])  # the reference's best-first search
+                knn["cpu_oracle_queries_per_s"] = len(ksamp) / max(time.time() - t, 1e-9)
+                # distances against the best-first search on the whole sample; index lists against brute
+                # force (order (d^2, index)) on its first 16 queries (the best-first order of ties is free)
+                bi, _ = O.knn_brute(sx, sx[ksamp[:16]], knn["k"])
+                ok, err = True, 0.0
+                for name in ("fp32", "fp64"):
+                    gi, gf = knn[name]["_sample"]
+                    ok &= bool(np.array_equal(gf, od.astype(np.float32)))
+                    ok &= bool(np.array_equal(gi[:16].astype(np.int64), bi))
+                    err = max(err, float(np.max(np.abs(gf.astype(np.float64) - od))))
+                verify["knn_eq_oracle_sample"] = ok
+                knn["max_abs_d2_error_fp32_output"] = err
         if not args.no_cpu:
             threads = O.lib().orc_max_threads()
             # calibrate, then time a sample sized for ~cpu_seconds of CPU work
@@ -277,6 +319,8 @@ def main():
         "autotune": tuned,
         "union_efficiency": {"fp32_test_slots_used": fp32["stats"]["points_tested"] / max(1, fp32["stats"]["test_slots"]),
                              "fp32_class_slots_used": fp32["stats"]["nodes_classified"] / max(1, fp32["stats"]["class_slots"])},
+        "knn": {kk: v for kk, v in knn.items() if kk not in ("fp32", "fp64")} | {
+            name: {kk: v for kk, v in knn[name].items() if kk != "_sample"} for name in ("fp32", "fp64") if name in knn},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
