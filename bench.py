@@ -84,6 +84,65 @@ def profile(args):
     ctx.close()
 
 
+def mls_stress(args):
+    """Stress workload: a mobile-laser-scan sweep (default 400 M points along a 5 km trajectory),
+    Morton against Hilbert, all-points r = 0.5 m sphere counts in both precisions, and neighborsStruct
+    range lists (CSR) for a 10 M-centre subset. One JSON line; not the benchmark line."""
+    n = args.n if args.n != 50_000_000 else 400_000_000
+    radius = 0.5
+    t0 = time.time()
+    xyz = A.mls_sweep(n)
+    log(f"[mls] generated {n} points in {time.time() - t0:.1f} s")
+    ctx = A.Octree(0)
+    ctx.set_points(xyz)
+    out = {"workload": f"synthetic mobile-laser-scan sweep, {n} FP64 points along a 5 km trajectory, maxLeaf "
+                       f"{args.max_leaf}, all-points sphere r={radius} m counts, neighborsStruct CSR for 10 M centres",
+           "points": n, "radius": radius, "encoders": {}}
+    per_point = {}
+    for enc, name in ((A.ENC_MORTON, "morton"), (A.ENC_HILBERT, "hilbert")):
+        for rep in range(2):
+            c, r, perm, keys = ctx.sort_points(enc, fetch=(rep == 1))
+        st = ctx.stage_times()
+        info = ctx.build(args.max_leaf)
+        bst = ctx.stage_times()["build"]
+        e = {"encode_ms": st["encode"], "sort_ms": st["sort"], "gather_ms": st["gather"], "bbox_ms": st["bbox"],
+             "build_ms": bst, "tree": {k: info[k] for k in ("n_leaf", "n_internal", "n_total", "max_depth_seen")},
+             "keys_sorted": bool(np.all(keys[1:] >= keys[:-1]))}
+        log(f"[mls] {name}: sort {st['sort']:.2f} ms build {bst:.2f} ms tree {info}")
+        cnt = {}
+        for pname, prec in (("fp32", A.PREC_FP32), ("fp64", A.PREC_FP64)):
+            ctx.autotune_radius(A.K_SPHERE, A.A_STRUCT, prec, radius, sample_nq=1 << 22)
+            ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, prec, radius)
+            ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, prec, radius)
+            ms = ctx.stage_times()["search"]
+            cnt[pname], stt = ctx.radius(A.K_SPHERE, A.A_STRUCT, prec, radius=radius, nq=n)
+            e[f"search_{pname}_ms"] = ms
+            e[f"{pname}_queries_per_s"] = n / (ms * 1e-3)
+            e["total_neighbors"] = stt["total_neighbors"]
+            log(f"[mls] {name} {pname}: {ms:.1f} ms, {stt['total_neighbors']} neighbours")
+        e["fp32_eq_fp64_all_queries"] = bool(np.array_equal(cnt["fp32"], cnt["fp64"]))
+        e["fp32_vs_fp64_speedup"] = e["search_fp64_ms"] / e["search_fp32_ms"]
+        orig = np.empty(n, np.uint32)
+        orig[perm.astype(np.int64)] = cnt["fp32"]
+        per_point[name] = orig
+        del cnt, keys
+        # neighborsStruct lists for 10 M random centres: CSR of [first, last) ranges
+        nq = min(10_000_000, n)
+        idx = A.random_indices(nq, n, seed=5)
+        counts, offs, ent, stl = ctx.radius_csr(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, idx, radius)
+        rng = ent.reshape(-1, 2).astype(np.int64)
+        cs = np.concatenate([[0], np.cumsum(rng[:, 1] - rng[:, 0])])
+        lens = cs[offs[1:].astype(np.int64)] - cs[offs[:-1].astype(np.int64)]
+        e["csr"] = {"centres": nq, "ranges": int(offs[-1]), "neighbors": int(counts.astype(np.int64).sum()),
+                    "kernel_ms": stl["kernel_ms"], "range_lengths_eq_counts": bool(np.array_equal(lens, counts))}
+        log(f"[mls] {name} csr: {e['csr']}")
+        del rng, ent, offs, counts, perm
+        out["encoders"][name] = e
+    out["counts_eq_across_encoders"] = bool(np.array_equal(per_point["morton"], per_point["hilbert"]))
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +155,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--knn-k", type=int, default=10, help="k of the all-points kNN measured beside the search (0: skip)")
+    ap.add_argument("--workload", default="tile50m", choices=("tile50m", "mls400m"),
+                    help="tile50m: the benchmark line (default); mls400m: the mobile-laser-scan stress run (one JSON "
+                         "line of its own; --n sets its size, default 400 M)")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--profile", default=None,
                     help="profiling driver instead of the benchmark: 'search:v64,d64' (one all-points search per "
@@ -109,6 +171,9 @@ def main():
         sys.exit(2)
     if args.profile:
         profile(args)
+        return
+    if args.workload == "mls400m":
+        mls_stress(args)
         return
 
     n = args.n

@@ -194,6 +194,99 @@ void oct_synth_lidar_tile(uint64_t seed, size_t n, double extent, double ox, dou
     }
 }
 
+// Synthetic mobile-laser-scan sweep along a vehicle trajectory of about `length` metres: a smooth
+// random curve (Catmull-Rom spline through control points 50 m apart, heading a random walk) with a
+// gently rolling road elevation. Returns, in proportion:
+//  - 0.1 %: outliers, uniform in the corridor's bounding box (+-100 m, z in [-20, 60] m);
+//  - 5 %: pole cylinders (radius 0.15 m, height 8 m) every 30 m on both sides, 6.5 m from the path;
+//  - 30 %: facade planes on both sides, 9-12 m from the path (offset and height 6-20 m per 20 m block);
+//  - the rest: ground returns at lateral range d drawn with density proportional to 1/d^2 on
+//    [1, 80] m (inverse CDF d = 1 / (1 - u (1 - 1/80))), height = road elevation + N(0, 0.02 m).
+// Scan rings are not traced; the mix reproduces their range falloff and surface types.
+void oct_synth_mls_sweep(uint64_t seed, size_t n, double length, double ox, double oy, double *xyz) {
+    const int M = (int)(length / 50.0) + 3;  // control points (with one extra at each end)
+    std::vector<double> px((size_t)M), py((size_t)M);
+    {
+        Rng r(seed, 0x5EEDull);
+        double h = 2.0 * kPi * r.uni(), x = 0.0, y = 0.0;
+        for (int i = 0; i < M; i++) {
+            px[(size_t)i] = x, py[(size_t)i] = y;
+            h += 0.25 * r.normal();
+            x += 50.0 * det_sin(h + 0.5 * kPi);
+            y += 50.0 * det_sin(h);
+        }
+    }
+    const int S = M - 3;  // spline segments between control points 1 .. M-2
+    auto curve = [&](double t, double &x, double &y, double &tx, double &ty) {
+        int j = (int)t;
+        if (j > S - 1) j = S - 1;
+        const double u = t - (double)j, u2 = u * u, u3 = u2 * u;
+        const double *X = &px[(size_t)j], *Y = &py[(size_t)j];
+        x = 0.5 * ((2.0 * X[1]) + (-X[0] + X[2]) * u + (2.0 * X[0] - 5.0 * X[1] + 4.0 * X[2] - X[3]) * u2 +
+                   (-X[0] + 3.0 * X[1] - 3.0 * X[2] + X[3]) * u3);
+        y = 0.5 * ((2.0 * Y[1]) + (-Y[0] + Y[2]) * u + (2.0 * Y[0] - 5.0 * Y[1] + 4.0 * Y[2] - Y[3]) * u2 +
+                   (-Y[0] + 3.0 * Y[1] - 3.0 * Y[2] + Y[3]) * u3);
+        tx = 0.5 * ((-X[0] + X[2]) + 2.0 * (2.0 * X[0] - 5.0 * X[1] + 4.0 * X[2] - X[3]) * u +
+                    3.0 * (-X[0] + 3.0 * X[1] - 3.0 * X[2] + X[3]) * u2);
+        ty = 0.5 * ((-Y[0] + Y[2]) + 2.0 * (2.0 * Y[0] - 5.0 * Y[1] + 4.0 * Y[2] - Y[3]) * u +
+                    3.0 * (-Y[0] + 3.0 * Y[1] - 3.0 * Y[2] + Y[3]) * u2);
+        const double nrm = std::sqrt(tx * tx + ty * ty);
+        tx /= nrm, ty /= nrm;
+    };
+    auto road_z = [&](double t) { return 4.0 * det_sin(0.21 * t) + 1.5 * det_sin(0.73 * t + 1.0); };
+    double mnx = px[0], mxx = px[0], mny = py[0], mxy = py[0];
+    for (int i = 0; i < M; i++) {
+        mnx = std::fmin(mnx, px[(size_t)i]), mxx = std::fmax(mxx, px[(size_t)i]);
+        mny = std::fmin(mny, py[(size_t)i]), mxy = std::fmax(mxy, py[(size_t)i]);
+    }
+    const double span = (double)S;           // spline parameter range [0, S)
+    const int n_poles = (int)(length / 30.0);  // per side
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; i++) {
+        Rng r(seed, 0x300000000ull + i);
+        const double u = r.uni();
+        double x, y, z;
+        if (u < 0.001) {
+            x = mnx - 100.0 + (mxx - mnx + 200.0) * r.uni();
+            y = mny - 100.0 + (mxy - mny + 200.0) * r.uni();
+            z = -20.0 + 80.0 * r.uni();
+        } else {
+            const double side = r.uni() < 0.5 ? -1.0 : 1.0;
+            double t, cx, cy, tx, ty, lat;
+            if (u < 0.051) {  // pole
+                const int k = (int)(r.uni() * (double)n_poles);
+                t = ((double)k + 0.5) * span / (double)n_poles;
+                curve(t, cx, cy, tx, ty);
+                const double a = 2.0 * kPi * r.uni();
+                lat = side * 6.5;
+                x = cx - ty * lat + 0.15 * det_sin(a + 0.5 * kPi);
+                y = cy + tx * lat + 0.15 * det_sin(a);
+                z = road_z(t) + 8.0 * r.uni();
+            } else if (u < 0.351) {  // facade
+                t = span * r.uni();
+                curve(t, cx, cy, tx, ty);
+                Rng b(seed, 0xFACADEull + (uint64_t)(t * 2.5) * 2 + (side > 0.0 ? 1 : 0));  // 20 m blocks
+                const double off = 9.0 + 3.0 * b.uni(), h = 6.0 + 14.0 * b.uni();
+                lat = side * off;
+                x = cx - ty * lat;
+                y = cy + tx * lat;
+                z = road_z(t) + h * r.uni();
+            } else {  // ground, range density ~ 1/d^2 on [1, 80] m
+                t = span * r.uni();
+                curve(t, cx, cy, tx, ty);
+                const double d = 1.0 / (1.0 - r.uni() * (1.0 - 1.0 / 80.0));
+                lat = side * d;
+                x = cx - ty * lat;
+                y = cy + tx * lat;
+                z = road_z(t) + 0.02 * r.normal();
+            }
+        }
+        xyz[3 * i + 0] = las_mm(x, ox);
+        xyz[3 * i + 1] = las_mm(y, oy);
+        xyz[3 * i + 2] = las_mm(z, 40.0);
+    }
+}
+
 // nq indices drawn uniformly with replacement from [0, n) (the reference's SearchSet random mode,
 // inc/benchmarking/search_set.hpp:37-41, with a portable generator instead of std::mt19937).
 void oct_synth_indices(uint64_t seed, size_t nq, size_t n, uint32_t *out) {

@@ -107,6 +107,7 @@ def synth():
         S.oct_synth_lidar_tile.argtypes = [C.c_uint64, C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_int,
                                            C.c_int, C.c_double, C.c_double, _vp]
         S.oct_synth_indices.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, _vp]
+        S.oct_synth_mls_sweep.argtypes = [C.c_uint64, C.c_size_t, C.c_double, C.c_double, C.c_double, _vp]
         _SYNTH = S
     return _SYNTH
 
@@ -133,6 +134,14 @@ def lidar_tile(n, seed=2, extent=1000.0, offset=(651000.0, 6861000.0), n_buildin
     xyz = np.empty((n, 3), np.float64)
     synth().oct_synth_lidar_tile(seed, n, extent, offset[0], offset[1], max(nb, 1), max(nv, 1), frac_building,
                                  frac_veg, _ptr(xyz))
+    return xyz
+
+
+def mls_sweep(n, seed=5, length=5000.0, offset=(651000.0, 6861000.0)):
+    """Synthetic mobile-laser-scan sweep along a ~length-metre smooth trajectory: ground returns with
+    range density ~ 1/d^2 on [1, 80] m, facade planes, pole cylinders, 0.1 % outliers."""
+    xyz = np.empty((n, 3), np.float64)
+    synth().oct_synth_mls_sweep(seed, n, length, offset[0], offset[1], _ptr(xyz))
     return xyz
 
 
@@ -296,6 +305,22 @@ class Octree:
             else:
                 out.append(ent[a:b].astype(np.int64))
         return counts, out, st.as_dict()
+
+    def radius_csr(self, kernel, algo, precision, idx, radius):
+        """Result lists of a search over sorted-cloud indices as CSR: counts, offsets (nq + 1) and the flat
+        entries (point indices, or [first, last) pairs for neighborsStruct), without per-query arrays."""
+        idx = np.ascontiguousarray(idx, np.uint32)
+        nq = idx.shape[0]
+        counts = np.empty(nq, np.uint32)
+        st = SearchStats()
+        self._chk(self.L.oct_radius(self.h, kernel, algo, precision, None, _ptr(idx), None, float(radius), nq, 1,
+                                    _ptr(counts), C.byref(st)), "oct_radius")
+        offs = np.empty(nq + 1, np.uint64)
+        self._chk(self.L.oct_radius_fetch(self.h, _ptr(offs), None, 0), "oct_radius_fetch(offsets)")
+        words = int(offs[-1]) * (2 if algo == A_STRUCT else 1)
+        ent = np.empty(max(words, 1), np.uint32)
+        self._chk(self.L.oct_radius_fetch(self.h, _ptr(offs), _ptr(ent), ent.shape[0]), "oct_radius_fetch")
+        return counts, offs, ent[:words], st.as_dict()
 
     def radius_dev(self, kernel, algo, precision, radius, nq=None, stats=False):
         """All-points search on the device-resident sorted cloud; counts stay on the device."""
