@@ -202,6 +202,32 @@ int oct_tuned_radius(oct_ctx *ctx, int kernel, int algo, int precision, int *gro
 int oct_autotune_knn(oct_ctx *ctx, int precision, uint32_t k, size_t sample_nq, int *best, double ms_out[3]);
 int oct_tuned_knn(oct_ctx *ctx, int precision, uint32_t k, int *lanes);
 
+/* ---- batches of independent clouds (no counterpart in the reference: its LinearOctree holds one
+ * cloud and its harness runs clouds one after another). Each added cloud is encoded, sorted and built
+ * exactly as by oct_sort_points + oct_build on its own context (same keys, permutation and tree);
+ * oct_batch_build then lays all clouds out as one forest on the device, and each search call runs
+ * every cloud's queries in one launch per search kind (kNN: one per k size class). Queries name a
+ * cloud and a centre by its index in that cloud's sorted order; results use the same indices.
+ * oct_batch_add copies the points (stride as oct_set_points); encoder OCT_ENC_MORTON3D or
+ * OCT_ENC_HILBERT3D; *cloud_id receives the cloud's number (0, 1, ...). oct_batch_radius: counts of
+ * a count-mode search with a radius per query. oct_batch_knn: exact k nearest with a k per query
+ * (1 .. kmax, kmax <= 1024); rows of kmax entries, -1 / NaN past min(n, k). oct_batch_info: a cloud's
+ * size and tree after oct_batch_build. */
+typedef struct oct_batch oct_batch;
+int oct_batch_create(oct_batch **batch, int device);
+int oct_batch_destroy(oct_batch *batch);
+const char *oct_batch_last_error(const oct_batch *batch);
+int oct_batch_add(oct_batch *batch, const double *xyz, size_t n, size_t stride_bytes, int encoder, uint32_t max_points_leaf,
+                  int *cloud_id);
+int oct_batch_build(oct_batch *batch);
+int oct_batch_info(const oct_batch *batch, int cloud, size_t *n, oct_tree_info *info);
+/* device milliseconds of the last batch searches, as oct_stage_times ([5] radius, [6] kNN: the last class) */
+int oct_batch_stage_times(const oct_batch *batch, double ms[8]);
+int oct_batch_radius(oct_batch *batch, int kernel, int algo, int precision, const uint32_t *cloud, const uint32_t *center_idx,
+                     const double *radii, size_t nq, uint32_t *counts_out, oct_search_stats *stats);
+int oct_batch_knn(oct_batch *batch, int precision, const uint32_t *cloud, const uint32_t *center_idx, const uint32_t *k,
+                  size_t nq, uint32_t kmax, int32_t *idx_out, double *d2_out, oct_search_stats *stats);
+
 /* ---- timing: device milliseconds of the stages of the last calls (HIP events on the context's
  * stream): [0] bbox, [1] encode, [2] sort, [3] gather, [4] build, [5] search, [6] knn. */
 int oct_stage_times(const oct_ctx *ctx, double ms[8]);

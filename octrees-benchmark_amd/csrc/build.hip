@@ -176,6 +176,15 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t M, std::string &err) {
         for (int a = 0; a < 3; a++) g.rad[l][a] = ctx->level_radii[l][a];
 
     hipError_t e;
+    // geometry for the searches: level radii, then per axis |centre| + radius (the kNN pruning margin);
+    // copied from the context's own array, which outlives the copy
+    for (int l = 0; l < 22; l++)
+        for (int a = 0; a < 3; a++) ctx->geo[3 * l + a] = ctx->level_radii[l][a];
+    for (int a = 0; a < 3; a++) ctx->geo[66 + a] = fabs(ctx->center[a]) + fabs(ctx->radii[a]);
+    if ((e = ctx->radtab.reserve(69)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(ctx->radtab.p, ctx->geo, 69 * sizeof(double), hipMemcpyHostToDevice, ctx->stream)) !=
+        hipSuccess)
+        return e;
     size_t est = 64 + 4 * (size_t)n / (M ? M : 1);
     if ((e = grow_preserve(ctx, ctx->nodes, est, 0)) != hipSuccess) return e;
     if ((e = grow_preserve(ctx, ctx->nkey, est, 0)) != hipSuccess) return e;

@@ -97,6 +97,8 @@ struct oct_ctx {
     double boxmin[3] = {0, 0, 0};
     double half_lengths[3] = {0, 0, 0};
     double level_radii[22][3] = {{0}};
+    double geo[69] = {0};           // level_radii (66) then |centre| + radius per axis (3)
+    oct::DevBuf<double> radtab;     // geo on the device
     oct::DevBuf<oct::NodeRec> nodes;
     oct::DevBuf<uint32_t> child;    // first child index, 0 = leaf
     oct::DevBuf<uint64_t> nkey;     // node start key (SFC format)
@@ -155,6 +157,11 @@ struct RadiusLaunch {
     int group = 16;  // count mode: queries per wave (16, 32, 64), or 0 for the one-lane-per-query kernel
     bool mx = false;  // count mode, FP32 sphere: point tests on the matrix cores
     bool remap = true;  // count mode, vector point tests: compact the queries of sparsely wanted leaves
+    // count mode over a forest of clouds (batches): wave tiles of 64 slots, tile_info[tile] = cloud |
+    // valid slots << 20, roots[cloud], radtab[cloud * 66 + 3 * level + axis]; null: the context's cloud
+    const uint32_t *tile_info = nullptr;
+    const uint32_t *roots = nullptr;
+    const double *radtab = nullptr;
     const uint32_t *order = nullptr;
     const uint32_t *qidx = nullptr;
     const double *qxyz = nullptr;
@@ -188,6 +195,14 @@ struct KnnLaunch {
     float *d2f = nullptr;
     unsigned long long *stats = nullptr;
     int lanes = 0;  // lanes per query (0: the size class's default)
+    // forests of clouds (batches): per wave tile cloud | valid slots << 20, roots, per-cloud geometry
+    // (66 level radii each, 3 magnitudes each) and per-query k; null: the context's one cloud
+    const uint32_t *tile_info = nullptr;
+    const uint32_t *roots = nullptr;
+    const double *radtab = nullptr;
+    const double *magtab = nullptr;
+    const uint32_t *kq = nullptr;
+    uint32_t kclass = 0;  // the largest k of the launch when kq is set (0: k)
 };
 hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err);
 // lanes-per-query shapes offered for k (default first); returns how many

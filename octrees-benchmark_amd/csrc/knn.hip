@@ -43,8 +43,14 @@ struct KnnArgs {
     double *__restrict__ d2_out;
     float *__restrict__ d2f_out;
     unsigned long long *__restrict__ stats;
-    double rad[22][3];
-    double mag[3];  // per axis |centre| + radius of the cloud's box
+    // tree geometry per cloud: level radii radtab[cloud * 66 + 3 * level + axis] and the box magnitude
+    // magtab[cloud * 3 + axis] = |centre| + radius; for a forest of clouds, tile_info[wave tile] =
+    // cloud | valid slots << 20 and roots[cloud] (null: one cloud, root node 0)
+    const double *__restrict__ radtab;
+    const double *__restrict__ magtab;
+    const uint32_t *__restrict__ tile_info;
+    const uint32_t *__restrict__ roots;
+    const uint32_t *__restrict__ kq;  // per-query k (null: k for every query); k is the row width
 };
 
 __device__ inline bool knn_less(double da, uint32_t ia, double db, uint32_t ib) { return da < db || (da == db && ia < ib); }
@@ -77,9 +83,17 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
     const uint32_t qs = lane / S, g = lane % S;
     const uint32_t slot0 = blockIdx.x * G;
     const uint32_t slot = slot0 + qs;
-    const bool valid = slot < a.nq;
+    uint32_t cloud = 0, nvt = G, root = 0;
+    if (a.tile_info) {
+        const uint32_t ti = uniform_u32(a.tile_info[blockIdx.x]);
+        cloud = ti & 0xfffffu, nvt = ti >> 20;
+        root = uniform_u32(a.roots[cloud]);
+    }
+    ConstDouble *radp = const_doubles(a.radtab) + 66 * cloud;  // level radii: radp[3 * level + axis]
+    ConstDouble *magp = const_doubles(a.magtab) + 3 * cloud;
+    const bool valid = slot < a.nq && qs < nvt;
     const uint64_t segmask = (S == 64 ? ~0ull : ((1ull << S) - 1ull)) << (qs * S);
-    const uint32_t k = a.k;
+    const uint32_t kw = a.k;  // output row width
     double *bd = s_bd + qs * CAP;
     uint32_t *bi = s_bi + qs * CAP;
 
@@ -89,10 +103,11 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
 
-    uint32_t qid = 0;
+    uint32_t qid = 0, k = kw;
     double qx = 0.0, qy = 0.0, qz = 0.0;
     if (valid) {
         qid = a.order ? a.order[slot] : slot;
+        if (a.kq) k = a.kq[qid];
         if (a.qxyz) {
             qx = a.qxyz[3 * (size_t)qid], qy = a.qxyz[3 * (size_t)qid + 1], qz = a.qxyz[3 * (size_t)qid + 2];
         } else {
@@ -101,8 +116,8 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
         }
     }
     // box widening per axis: 2^-40 of the magnitudes involved (cloud box and query)
-    const double dlx = 0x1p-40 * (a.mag[0] + fabs(qx)), dly = 0x1p-40 * (a.mag[1] + fabs(qy));
-    const double dlz = 0x1p-40 * (a.mag[2] + fabs(qz));
+    const double dlx = 0x1p-40 * (magp[0] + fabs(qx)), dly = 0x1p-40 * (magp[1] + fabs(qy));
+    const double dlz = 0x1p-40 * (magp[2] + fabs(qz));
 
     for (uint32_t t = g; t < (uint32_t)CAP; t += S) bd[t] = __builtin_huge_val(), bi[t] = 0xffffffffu;
     uint32_t cnt = 0;        // buffer entries of this query (same in its S lanes)
@@ -231,10 +246,10 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
 
     uint32_t sp = 0;
     {
-        const uint32_t c0 = uniform_u32(a.child[0]);
+        const uint32_t c0 = uniform_u32(a.child[root]);
         if (c0 == 0) {
-            const NodeRec nr = a.nodes[0];
-            leaf(valid, uniform_u32(nr.first), uniform_u32(nr.last), nr.cx, nr.cy, nr.cz, a.amax[0]);
+            const NodeRec nr = a.nodes[root];
+            leaf(valid, uniform_u32(nr.first), uniform_u32(nr.last), nr.cx, nr.cy, nr.cz, a.amax[root]);
         } else {
             s_node[0] = (c0 - 1) >> 3, s_lev[0] = 0;
             sp = 1;
@@ -250,7 +265,7 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
             ConstNodeExt &E = node_ext(a.next, ent >> 3);
             const uint32_t c = ent & 7u, L = lv & 31u;
             const double cx = E.ax[c & 1u], cy = E.ax[2 + ((c >> 1) & 1u)], cz = E.ax[4 + (c >> 2)];
-            const double lb = (gap2(qx, cx, a.rad[L][0], dlx) + gap2(qy, cy, a.rad[L][1], dly)) + gap2(qz, cz, a.rad[L][2], dlz);
+            const double lb = (gap2(qx, cx, radp[3 * L], dlx) + gap2(qy, cy, radp[3 * L + 1], dly)) + gap2(qz, cz, radp[3 * L + 2], dlz);
             const bool need = valid && !(lb > tau_pr);
             if (!ballot64(need)) continue;
             leaf(need, E.first[c], E.last[c], cx, cy, cz, E.amax[c]);
@@ -259,7 +274,7 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
         // an internal node at level lv: judge its 8 children (level lv + 1) from per-axis terms
         ConstNodeExt &E = node_ext(a.next, ent);
         const uint32_t d = lv + 1;
-        const double rx = a.rad[d][0], ry = a.rad[d][1], rz = a.rad[d][2];
+        const double rx = radp[3 * d], ry = radp[3 * d + 1], rz = radp[3 * d + 2];
         double gx[2], gy[2], gz[2];
 #pragma unroll
         for (int b = 0; b < 2; b++) {
@@ -297,11 +312,11 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
     select();
 
     if (valid) {
-        for (uint32_t t = g; t < k; t += S) {
+        for (uint32_t t = g; t < kw; t += S) {
             const bool have = t < cnt;
-            a.idx_out[(size_t)qid * k + t] = have ? (int32_t)bi[t] : -1;
-            if (a.d2_out) a.d2_out[(size_t)qid * k + t] = have ? bd[t] : __builtin_nan("");
-            if (a.d2f_out) a.d2f_out[(size_t)qid * k + t] = have ? (float)bd[t] : __builtin_nanf("");
+            a.idx_out[(size_t)qid * kw + t] = have ? (int32_t)bi[t] : -1;
+            if (a.d2_out) a.d2_out[(size_t)qid * kw + t] = have ? bd[t] : __builtin_nan("");
+            if (a.d2f_out) a.d2f_out[(size_t)qid * kw + t] = have ? (float)bd[t] : __builtin_nanf("");
         }
     }
     if (a.stats) {
@@ -351,13 +366,12 @@ int knn_lane_options(uint32_t k, int out[3]) {
     }
 }
 
-static bool launch_knn_shape(const KnnArgs &a, int precision, int lanes, hipStream_t s) {
+static bool launch_knn_shape(const KnnArgs &a, int kc, int precision, int lanes, hipStream_t s) {
 #define OCT_KNN_CASE(KCV, SV)                     \
     if (kc == KCV && lanes == SV) {               \
         launch_sk<SV, KCV>(a, precision, s);      \
         return true;                              \
     }
-    const int kc = knn_kc(a.k);
     OCT_KNN_CASE(16, 2) OCT_KNN_CASE(16, 4) OCT_KNN_CASE(16, 8)
     OCT_KNN_CASE(32, 2) OCT_KNN_CASE(32, 4) OCT_KNN_CASE(32, 8)
     OCT_KNN_CASE(64, 4) OCT_KNN_CASE(64, 8) OCT_KNN_CASE(64, 16)
@@ -398,14 +412,18 @@ hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err) {
     a.d2_out = L.d2;
     a.d2f_out = L.d2f;
     a.stats = L.stats;
-    for (int l = 0; l < 22; l++)
-        for (int c = 0; c < 3; c++) a.rad[l][c] = ctx->level_radii[l][c];
-    for (int c = 0; c < 3; c++) a.mag[c] = fabs(ctx->center[c]) + fabs(ctx->radii[c]);
+    a.radtab = ctx->radtab.p;
+    a.magtab = ctx->radtab.p + 66;  // the context's magnitudes follow its level radii
+    a.tile_info = L.tile_info;
+    a.roots = L.roots;
+    a.kq = L.kq;
+    if (L.radtab) a.radtab = L.radtab, a.magtab = L.magtab;
+    const uint32_t kcls = L.kq && L.kclass ? L.kclass : k;  // selection size class of the launch
     int opts[3];
-    knn_lane_options(k, opts);
+    knn_lane_options(kcls, opts);
     const int lanes = L.lanes > 0 ? L.lanes : opts[0];
     stage_begin(ctx, ST_KNN);
-    if (!launch_knn_shape(a, precision, lanes, ctx->stream)) {
+    if (!launch_knn_shape(a, knn_kc(kcls), precision, lanes, ctx->stream)) {
         err = "knn: lanes per query not offered for this k";
         return hipErrorInvalidValue;
     }

@@ -355,7 +355,15 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
     const uint32_t tile = blockIdx.x * kWaves + w;
     if (uniform_u32(tile * G) >= a.nq) return;
     const uint32_t slot = tile * G + j;
-    const bool valid = slot < a.nq;
+    // one cloud (root 0), or the wave tile's cloud of a forest and its number of valid slots
+    uint32_t cloud = 0, nvt = G, root = 0;
+    if (a.tile_info) {
+        const uint32_t ti = uniform_u32(a.tile_info[tile]);
+        cloud = ti & 0xfffffu, nvt = ti >> 20;
+        root = uniform_u32(a.roots[cloud]);
+    }
+    ConstDouble *radp = const_doubles(a.radtab) + 66 * cloud;  // level radii: radp[3 * level + axis]
+    const bool valid = slot < a.nq && j < nvt;
     const uint64_t qbit = 1ull << j;
     float4 *sp32 = reinterpret_cast<float4 *>(&s_stage[w][0]);
     double4 *sp64 = reinterpret_cast<double4 *>(&s_stage[w][0]);
@@ -403,9 +411,9 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
 
     auto judge = [&](bool act, const NodeRec &nr, uint32_t d) -> int {
         if (!act) return -1;
-        if (ALGO == OCT_ALGO_NEIGHBORS)
-            return overlap64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]) ? J_OVERLAP : J_OUTSIDE;
-        return classify64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]);
+        const double rx = radp[3 * d], ry = radp[3 * d + 1], rz = radp[3 * d + 2];
+        if (ALGO == OCT_ALGO_NEIGHBORS) return overlap64<KERNEL>(q, nr.cx, nr.cy, nr.cz, rx, ry, rz) ? J_OVERLAP : J_OUTSIDE;
+        return classify64<KERNEL>(q, nr.cx, nr.cy, nr.cz, rx, ry, rz);
     };
 
     // Stage points [base, base + P) of the sorted cloud for 2^lg lane groups: point t goes to group
@@ -708,7 +716,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
     // judges all 8 children for its own query (the S lanes of a query agree); results are lane
     // masks, so the per-child decisions are scalar-unit work.
     auto judge8 = [&](ConstNodeExt &E, uint32_t d, bool act, bool (&in)[8], bool (&ov)[8]) {
-        const double rx = a.rad[d][0], ry = a.rad[d][1], rz = a.rad[d][2];
+        const double rx = radp[3 * d], ry = radp[3 * d + 1], rz = radp[3 * d + 2];
         constexpr bool is3d = (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CUBE);
         if (ALGO == OCT_ALGO_NEIGHBORS || KERNEL == OCT_KERNEL_CUBE || KERNEL == OCT_KERNEL_SQUARE) {
             // boxOverlap (kernel_3d.hpp:100-115, kernel_2d.hpp:29-41) and the box kernels'
@@ -776,18 +784,18 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
     // uniform load.
     uint32_t sp = 0;
     {
-        const NodeRec nr = a.nodes[0];
+        const NodeRec nr = a.nodes[root];
         const int jr = judge(valid, nr, 0);
         if (STATS) n_class += __popcll(ballot64(valid) & gmask), n_cslots += 64;
         if (g == 0 && jr == J_INSIDE) cnt += nr.last - nr.first;
         const uint64_t ov = ballot64(jr == J_OVERLAP) & gmask;
         if (ov) {
-            const uint32_t c0 = uniform_u32(a.child[0]);
+            const uint32_t c0 = uniform_u32(a.child[root]);
             if (c0 == 0) {
                 NodeRec ur;
                 ur.cx = nr.cx, ur.cy = nr.cy, ur.cz = nr.cz;
                 ur.first = uniform_u32(nr.first), ur.last = uniform_u32(nr.last);
-                if (ur.last > ur.first) leaf(ov, a.amax[0], ur);
+                if (ur.last > ur.first) leaf(ov, a.amax[root], ur);
             } else {
                 s_node[w][0] = c0, s_mask[w][0] = ov, s_depth[w][0] = 0;
                 sp = 1;
@@ -949,6 +957,9 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     for (int k = 0; k < 3; k++) a.r3[k] = L.radius3[k];
     a.use_r3 = L.use_r3 ? 1 : 0;
     a.remap = L.remap ? 1 : 0;
+    a.radtab = L.radtab ? L.radtab : ctx->radtab.p;
+    a.tile_info = L.tile_info;
+    a.roots = L.roots;
     a.nq = (uint32_t)L.nq;
     a.counts = L.counts;
     a.ecounts = L.ecounts;

@@ -28,6 +28,12 @@ struct SearchArgs {
     double r3[3];
     int use_r3;
     int remap;  // count kernel: compact the queries of a leaf wanted by at most half the wave
+    // Count kernel's tree geometry and roots: level radii radtab[cloud * 66 + 3 * level + axis]; for a
+    // forest of clouds (batches), tile_info[wave tile] = cloud | valid slots << 20 and roots[cloud]
+    // (null: one cloud, root node 0, every slot below nq valid).
+    const double *__restrict__ radtab;
+    const uint32_t *__restrict__ tile_info;
+    const uint32_t *__restrict__ roots;
     uint32_t nq;
     uint32_t *__restrict__ counts;
     uint32_t *__restrict__ ecounts;
@@ -45,6 +51,8 @@ struct Query {
 // Read-only tree records through the constant address space: a wave-uniform index then compiles
 // to scalar loads (the records land in SGPRs, not in 64 copies across the vector registers).
 typedef const __attribute__((address_space(4))) NodeExt ConstNodeExt;
+typedef const __attribute__((address_space(4))) double ConstDouble;
+__device__ inline ConstDouble *const_doubles(const double *p) { return (ConstDouble *)(uintptr_t)p; }
 __device__ inline ConstNodeExt &node_ext(const NodeExt *p, uint32_t i) {
     return ((ConstNodeExt *)(uintptr_t)p)[i];
 }

@@ -25,7 +25,8 @@ EXPORTS = (
     "oct_set_points_dev", "oct_bbox", "oct_encode", "oct_sort", "oct_sort_points", "oct_set_sorted", "oct_build",
     "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_knn_dev", "oct_knn_fetch", "oct_autotune_knn", "oct_tuned_knn", "oct_stage_times",
     "oct_set_tuning", "oct_autotune_radius", "oct_tuned_radius", "oct_radius_box", "oct_encode_coords",
-    "oct_decode_keys",
+    "oct_decode_keys", "oct_batch_create", "oct_batch_destroy", "oct_batch_last_error", "oct_batch_add",
+    "oct_batch_build", "oct_batch_info", "oct_batch_stage_times", "oct_batch_radius", "oct_batch_knn",
 )
 
 
@@ -82,6 +83,16 @@ def lib():
         L.oct_knn.argtypes = [_vp, C.c_int, _vp, _vp, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_knn_dev.argtypes = [_vp, C.c_int, _vp, C.c_size_t, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_knn_fetch.argtypes = [_vp, C.c_size_t, C.c_size_t, _vp, _vp]
+        L.oct_batch_create.argtypes = [C.POINTER(_vp), C.c_int]
+        L.oct_batch_destroy.argtypes = [_vp]
+        L.oct_batch_last_error.argtypes = [_vp]
+        L.oct_batch_last_error.restype = C.c_char_p
+        L.oct_batch_add.argtypes = [_vp, _vp, C.c_size_t, C.c_size_t, C.c_int, C.c_uint32, C.POINTER(C.c_int)]
+        L.oct_batch_build.argtypes = [_vp]
+        L.oct_batch_stage_times.argtypes = [_vp, _vp]
+        L.oct_batch_info.argtypes = [_vp, C.c_int, C.POINTER(C.c_size_t), C.POINTER(TreeInfo)]
+        L.oct_batch_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_size_t, _vp, C.POINTER(SearchStats)]
+        L.oct_batch_knn.argtypes = [_vp, C.c_int, _vp, _vp, _vp, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_autotune_knn.argtypes = [_vp, C.c_int, C.c_uint32, C.c_size_t, C.POINTER(C.c_int), _vp]
         L.oct_tuned_knn.argtypes = [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_int)]
         L.oct_stage_times.argtypes = [_vp, _vp]
@@ -152,6 +163,80 @@ def random_indices(nq, n, seed=42):
 
 
 # ------------------------------------------------------------------ context
+class Batch:
+    """Many independent clouds: each is sorted and built as on its own, then all are searched together
+    (one launch per search kind). Query centres are (cloud, index in that cloud's sorted order)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = _vp()
+        rc = self.L.oct_batch_create(C.byref(h), device)
+        if rc != 0:
+            raise OctError(f"oct_batch_create(device={device}) failed with status {rc}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oct_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.oct_batch_last_error(self.h)
+            raise OctError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def add(self, xyz, encoder=ENC_HILBERT, max_points_leaf=128):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        cid = C.c_int(-1)
+        self._chk(self.L.oct_batch_add(self.h, _ptr(xyz), xyz.shape[0], 24, encoder, max_points_leaf, C.byref(cid)),
+                  "oct_batch_add")
+        return cid.value
+
+    def build(self):
+        self._chk(self.L.oct_batch_build(self.h), "oct_batch_build")
+
+    def stage_times(self):
+        ms = np.zeros(8)
+        self._chk(self.L.oct_batch_stage_times(self.h, _ptr(ms)), "oct_batch_stage_times")
+        return dict(zip(STAGES, ms[:len(STAGES)].tolist()))
+
+    def info(self, cloud):
+        n = C.c_size_t(0)
+        ti = TreeInfo()
+        self._chk(self.L.oct_batch_info(self.h, cloud, C.byref(n), C.byref(ti)), "oct_batch_info")
+        return n.value, {f: getattr(ti, f) for f, _ in ti._fields_}
+
+    def radius(self, kernel, algo, precision, cloud, idx, radii):
+        cloud = np.ascontiguousarray(cloud, np.uint32)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        radii = np.ascontiguousarray(radii, np.float64)
+        nq = cloud.shape[0]
+        counts = np.empty(nq, np.uint32)
+        st = SearchStats()
+        self._chk(self.L.oct_batch_radius(self.h, kernel, algo, precision, _ptr(cloud), _ptr(idx), _ptr(radii), nq,
+                                          _ptr(counts), C.byref(st)), "oct_batch_radius")
+        return counts, st.as_dict()
+
+    def knn(self, precision, cloud, idx, k, kmax=None):
+        cloud = np.ascontiguousarray(cloud, np.uint32)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        k = np.ascontiguousarray(k, np.uint32)
+        nq = cloud.shape[0]
+        kmax = int(k.max()) if kmax is None else kmax
+        oi = np.empty((nq, kmax), np.int32)
+        od = np.empty((nq, kmax), np.float64)
+        st = SearchStats()
+        self._chk(self.L.oct_batch_knn(self.h, precision, _ptr(cloud), _ptr(idx), _ptr(k), nq, kmax, _ptr(oi), _ptr(od),
+                                       C.byref(st)), "oct_batch_knn")
+        return oi, od, st.as_dict()
+
+
 class Octree:
     """One device context: points, keys, sorted cloud, tree and search buffers live on the GPU."""
 
