@@ -143,6 +143,88 @@ def mls_stress(args):
     ctx.close()
 
 
+def batch_workload(args):
+    """Batched workload: 256 independent clouds (sizes log-uniform in [1e4, 4e6], rescaled to ~200 M
+    points; half uniform boxes at 5 pts/m^3, half terrain + building tiles at 50 pts/m^2; each at a
+    random origin in [-1e5, 1e5]), 50 k random centres per cloud, radius per query uniform in
+    [0.5, 5] m (float), 20 % of the queries kNN with k uniform in {8..256}. One JSON line of its own."""
+    rng = np.random.default_rng(4)
+    C = 256
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(4e6), C))
+    sizes = np.maximum(1e4, sizes * (200e6 / sizes.sum())).astype(np.int64)
+    t0 = time.time()
+    b = A.Batch(0)
+    clouds = []
+    for i, n in enumerate(sizes):
+        n = int(n)
+        off = rng.uniform(-1e5, 1e5, 3)
+        if i % 2:
+            L = float(n) ** (1.0 / 3.0)
+            xyz = A.uniform_box(n, seed=1000 + i, size=(L, L, L / 5.0), offset=tuple(off))
+        else:
+            xyz = A.lidar_tile(n, seed=1000 + i, extent=1000.0 * (n / 50e6) ** 0.5, offset=(off[0], off[1]))
+        b.add(xyz, A.ENC_HILBERT, args.max_leaf)
+        clouds.append(xyz if i < 3 else None)
+    gen_s = time.time() - t0
+    log(f"[batch] {C} clouds, {int(sizes.sum())} points generated and uploaded in {gen_s:.1f} s")
+    t0 = time.time()
+    b.build()
+    build_s = time.time() - t0
+    log(f"[batch] built every cloud in {build_s:.2f} s")
+    per = 50_000
+    cloud = np.repeat(np.arange(C, dtype=np.uint32), per)
+    idx = (rng.uniform(0.0, 1.0, C * per) * np.repeat(sizes, per)).astype(np.uint32)
+    is_knn = rng.uniform(0.0, 1.0, C * per) < 0.2
+    radii = rng.uniform(0.5, 5.0, C * per).astype(np.float32).astype(np.float64)
+    kk = rng.integers(8, 257, C * per).astype(np.uint32)
+    rq, kq = np.nonzero(~is_knn)[0], np.nonzero(is_knn)[0]
+    out = {"workload": f"{C} clouds, {int(sizes.sum())} points (log-uniform 1e4..4e6), {per} centres each, radius "
+                       f"U[0.5, 5] m, 20 % kNN with k U{{8..256}}, Hilbert, maxLeaf {args.max_leaf}",
+           "clouds": C, "points": int(sizes.sum()), "radius_queries": int(len(rq)), "knn_queries": int(len(kq)),
+           "build_all_clouds_s": build_s}
+    for name, prec in (("fp32", A.PREC_FP32), ("fp64", A.PREC_FP64)):
+        b.radius(A.K_SPHERE, A.A_STRUCT, prec, cloud[rq], idx[rq], radii[rq])
+        t = time.perf_counter()
+        counts, _ = b.radius(A.K_SPHERE, A.A_STRUCT, prec, cloud[rq], idx[rq], radii[rq])
+        wall = time.perf_counter() - t
+        dev = b.stage_times()["search"]
+        out[f"radius_{name}"] = {"device_ms": dev, "wall_s": wall, "queries_per_s_device": len(rq) / (dev * 1e-3),
+                                 "neighbors": int(counts.astype(np.int64).sum())}
+        t = time.perf_counter()
+        ki, kd, st = b.knn(prec, cloud[kq], idx[kq], kk[kq], 256)
+        kwall = time.perf_counter() - t
+        out[f"knn_{name}"] = {"device_ms": st["kernel_ms"], "wall_s": kwall,
+                              "queries_per_s_device": len(kq) / max(st["kernel_ms"] * 1e-3, 1e-12)}
+        log(f"[batch] {name}: radius {dev:.1f} ms device ({wall:.2f} s wall), knn {st['kernel_ms']:.1f} ms device")
+        out[f"_{name}"] = (counts, ki, kd)
+    # exact against the oracle on the first three clouds (each on its own)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    ok = bool(np.array_equal(out["_fp32"][0], out["_fp64"][0]))
+    ok &= bool(np.array_equal(out["_fp32"][1], out["_fp64"][1]))
+    for c in range(3):
+        xyz = clouds[c]
+        sx, keys, perm, cc, rr = O.sort_points(A.ENC_HILBERT, xyz)
+        tree = O.Tree(keys, args.max_leaf, A.ENC_HILBERT, cc, rr)
+        rsel = np.nonzero(cloud[rq] == c)[0][:300]
+        for q in rsel:
+            qq = rq[q]
+            ref = O.radius_count(tree, sx, A.K_SPHERE, A.A_STRUCT, sx[[int(idx[qq])]], float(radii[qq]))[0]
+            ok &= int(out["_fp32"][0][q]) == int(ref)
+        ksel = np.nonzero(cloud[kq] == c)[0][:100]
+        bi, bd = O.knn_brute(sx, sx[idx[kq[ksel]].astype(np.int64)], 256)
+        for row, q in enumerate(ksel):
+            m = int(kk[kq[q]])
+            ok &= bool(np.array_equal(out["_fp32"][1][q, :m].astype(np.int64), bi[row, :m]))
+    out["verify"] = {"fp32_eq_fp64": bool(np.array_equal(out["_fp32"][0], out["_fp64"][0])),
+                     "oracle_sample_three_clouds": ok}
+    for name in ("_fp32", "_fp64"):
+        del out[name]
+    print(json.dumps(out), flush=True)
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,9 +237,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--knn-k", type=int, default=10, help="k of the all-points kNN measured beside the search (0: skip)")
-    ap.add_argument("--workload", default="tile50m", choices=("tile50m", "mls400m"),
-                    help="tile50m: the benchmark line (default); mls400m: the mobile-laser-scan stress run (one JSON "
-                         "line of its own; --n sets its size, default 400 M)")
+    ap.add_argument("--workload", default="tile50m", choices=("tile50m", "mls400m", "batch256"),
+                    help="tile50m: the benchmark line (default); mls400m: the mobile-laser-scan stress run; batch256: "
+                         "256 independent clouds searched together (each prints one JSON line of its own)")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--profile", default=None,
                     help="profiling driver instead of the benchmark: 'search:v64,d64' (one all-points search per "
@@ -174,6 +256,9 @@ def main():
         return
     if args.workload == "mls400m":
         mls_stress(args)
+        return
+    if args.workload == "batch256":
+        batch_workload(args)
         return
 
     n = args.n
