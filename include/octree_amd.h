@@ -45,7 +45,7 @@ enum oct_encoder { OCT_ENC_MORTON3D = 1, OCT_ENC_HILBERT3D = 5 };
 /* Kernel ids = the reference's Kernel_t values (inc/main_options.hpp:97-103). */
 enum oct_kernel { OCT_KERNEL_CIRCLE = 0, OCT_KERNEL_SPHERE = 1, OCT_KERNEL_SQUARE = 2, OCT_KERNEL_CUBE = 3 };
 /* Search algorithms = the reference's SearchAlgo values (inc/main_options.hpp:29-35). */
-enum oct_algo { OCT_ALGO_NEIGHBORS = 1, OCT_ALGO_PRUNE = 2, OCT_ALGO_STRUCT = 3 };
+enum oct_algo { OCT_ALGO_NEIGHBORS = 1, OCT_ALGO_PRUNE = 2, OCT_ALGO_STRUCT = 3, OCT_ALGO_APPROX = 4 /* oct_radius_approx */ };
 /* Arithmetic of the per-point / per-node tests. FP64 evaluates the reference's expressions as
  * written. FP32_RECHECK evaluates them in FP32 on leaf-recentred coordinates and re-evaluates in
  * FP64 every decision that falls inside a proven rounding band; results are identical. */
@@ -152,6 +152,17 @@ int oct_radius_fetch(oct_ctx *ctx, uint64_t *offsets, uint32_t *entries, size_t 
 /* Device-resident variant for throughput runs: counts stay on the device (counts_dev may be NULL). */
 int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint32_t *center_idx_dev, double radius,
                    size_t nq, uint32_t *counts_dev, oct_search_stats *stats);
+
+/* neighborsApprox (linear_octree.hpp:981-1035): neighborsStruct with a depth cutoff. A node deeper
+ * than `level` (the reference's uint32_t precisionLevel; getPrecisionLevel derives it from a
+ * tolerance, and the (Point, radius, double tolerancePercentage, bool) overload passes the tolerance
+ * itself as the level, linear_octree.hpp:1089-1093) is added whole when upper_bound != 0 and dropped
+ * otherwise, without a kernel test. Queries, radii, precision and lists (Struct ranges, fetched with
+ * oct_radius_fetch) as for oct_radius; radius3 non-NULL: cube / square with per-axis half extents
+ * (then the comparisons run in FP64). */
+int oct_radius_approx(oct_ctx *ctx, int kernel, int precision, const double *centers, const uint32_t *center_idx,
+                      const double *radii, double radius, const double radius3[3], size_t nq, uint32_t level, int upper_bound,
+                      int want_lists, uint32_t *counts_out, oct_search_stats *stats);
 
 /* ---- H6 kNN: LinearOctree::knn (linear_octree.hpp:899-942), benchmark label KNNV2. Exact k
  * nearest under the (squared FP64 distance, sorted index) order; returns min(n, k) per query,

@@ -384,7 +384,8 @@ static int read_stats(oct_ctx *ctx, oct_search_stats *stats, double ms) {
 
 static int radius_impl(oct_ctx *ctx, int kernel, int algo, int precision, const double *centers,
                        const uint32_t *center_idx, const double *radii, double radius, const double *radii3,
-                       const double *radius3, size_t nq, int want_lists, uint32_t *counts_out, oct_search_stats *stats) {
+                       const double *radius3, size_t nq, int want_lists, uint32_t *counts_out, oct_search_stats *stats,
+                       uint32_t approx_level = ~0u, bool approx_upper = false) {
     int s = check_search_args(ctx, kernel, algo, precision);
     if (s) return s;
     if (nq >= (1ull << 31)) return fail(ctx, OCT_ERR_ARG, "too many queries");
@@ -419,6 +420,7 @@ static int radius_impl(oct_ctx *ctx, int kernel, int algo, int precision, const 
     L.group = radius_group(ctx, kernel, algo, precision);
     L.mx = radius_mx(ctx, kernel, algo, precision);
     L.remap = ctx->remap_override != 0;
+    L.approx_level = approx_level, L.approx_upper = approx_upper;
     L.order = d_order;
     L.qidx = center_idx ? d_idx : nullptr;
     L.qxyz = centers ? ctx->qxyz.p : nullptr;
@@ -481,6 +483,16 @@ int oct_radius_box(oct_ctx *ctx, int kernel, int algo, const double *centers, co
     if (!radii3 && !radius3) return fail(ctx, OCT_ERR_ARG, "no radii");
     return radius_impl(ctx, kernel, algo, OCT_PREC_FP64, centers, center_idx, nullptr, 0.0, radii3, radius3, nq,
                        want_lists, counts_out, stats);
+}
+
+int oct_radius_approx(oct_ctx *ctx, int kernel, int precision, const double *centers, const uint32_t *center_idx,
+                      const double *radii, double radius, const double radius3[3], size_t nq, uint32_t level, int upper_bound,
+                      int want_lists, uint32_t *counts_out, oct_search_stats *stats) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (radius3 && kernel != OCT_KERNEL_CUBE && kernel != OCT_KERNEL_SQUARE)
+        return fail(ctx, OCT_ERR_UNSUPPORTED, "per-axis radii need the cube or square kernel (kernel_factory.hpp:33-41)");
+    return radius_impl(ctx, kernel, OCT_ALGO_STRUCT, radius3 ? OCT_PREC_FP64 : precision, centers, center_idx, radii, radius,
+                       nullptr, radius3, nq, want_lists, counts_out, stats, level, upper_bound != 0);
 }
 
 static int key_codec(oct_ctx *ctx, int encoder, bool decode, uint32_t *xyz, uint64_t *keys, size_t n) {

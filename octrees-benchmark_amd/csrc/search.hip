@@ -119,6 +119,9 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
     // ---- classification helper: returns the lane's judgement for node c at depth d
     auto judge = [&](uint64_t m, const NodeRec &nr, uint32_t d) -> int {
         if (!(m & lanebit)) return -1;
+        // neighborsApprox beyond its precision level: the node's range whole, or nothing
+        // (linear_octree.hpp:987-994)
+        if (ALGO == OCT_ALGO_STRUCT && d > a.approx_level) return a.approx_upper ? J_INSIDE : J_OUTSIDE;
         if (ALGO == OCT_ALGO_NEIGHBORS)
             return overlap64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]) ? J_OVERLAP : J_OUTSIDE;
         return classify64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]);
@@ -716,6 +719,13 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
     // judges all 8 children for its own query (the S lanes of a query agree); results are lane
     // masks, so the per-child decisions are scalar-unit work.
     auto judge8 = [&](ConstNodeExt &E, uint32_t d, bool act, bool (&in)[8], bool (&ov)[8]) {
+        if (ALGO == OCT_ALGO_STRUCT && d > a.approx_level) {
+            // neighborsApprox beyond its precision level: every child whole, or none
+            // (linear_octree.hpp:987-994)
+#pragma unroll
+            for (int c = 0; c < 8; c++) in[c] = act && a.approx_upper, ov[c] = false;
+            return;
+        }
         const double rx = radp[3 * d], ry = radp[3 * d + 1], rz = radp[3 * d + 2];
         constexpr bool is3d = (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CUBE);
         if (ALGO == OCT_ALGO_NEIGHBORS || KERNEL == OCT_KERNEL_CUBE || KERNEL == OCT_KERNEL_SQUARE) {
@@ -957,6 +967,8 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     for (int k = 0; k < 3; k++) a.r3[k] = L.radius3[k];
     a.use_r3 = L.use_r3 ? 1 : 0;
     a.remap = L.remap ? 1 : 0;
+    a.approx_level = L.approx_level;
+    a.approx_upper = L.approx_upper ? 1 : 0;
     a.radtab = L.radtab ? L.radtab : ctx->radtab.p;
     a.tile_info = L.tile_info;
     a.roots = L.roots;

@@ -67,6 +67,16 @@ int main(int argc, char **argv) {
     run_kernel<Kernel_t::circle>(oct, centers, r, "circle");
     run_kernel<Kernel_t::cube>(oct, centers, r, "cube");
     run_kernel<Kernel_t::square>(oct, centers, r, "square");
+    // neighborsApprox: the tolerance argument is the precision level itself (reference overload)
+    for (size_t q = 0; q < centers.size(); q++)
+        for (int lv = 0; lv < 4; lv++)
+            for (int up = 0; up < 2; up++) {
+                auto s = oct.neighborsApprox<Kernel_t::sphere>(centers[q], r, (double)lv, up != 0);
+                std::printf("approx %d:%d %zu %zu", lv, up, q, s.size());
+                for (auto &rg : s.ranges) std::printf(" %zu:%zu", rg.first, rg.second);
+                std::printf("\n");
+            }
+    std::printf("precision_level %u %u\n", oct.getPrecisionLevel(r), oct.getPrecisionLevel(Vector(r, r, r), 50.0));
     for (size_t q = 0; q < centers.size(); q++) {
         std::vector<size_t> idx(7);
         std::vector<double> d2(7);

@@ -90,6 +90,40 @@ public:
         return filter(neighbors<K>(p, radii), condition);
     }
 
+    // ---- approximate search (linear_octree.hpp:945-1035, 1089-1099): neighborsStruct with a depth
+    // cutoff; nodes deeper than the precision level are added whole (upperBound) or dropped. As in
+    // the reference, the tolerance argument of these overloads is passed on as the uint32_t precision
+    // level itself (the double converts to the level parameter of the kernel overload).
+    template <Kernel_t K = Kernel_t::square>
+    NeighborSet<Container> neighborsApprox(const Point &p, double radius, double tolerancePercentage, bool upperBound) const {
+        return as_set(approx((int)K, p, radius, nullptr, (uint32_t)tolerancePercentage, upperBound));
+    }
+    template <Kernel_t K = Kernel_t::cube>
+    NeighborSet<Container> neighborsApprox(const Point &p, const Vector &radii, double tolerancePercentage,
+                                           bool upperBound) const {
+        return as_set(approx((int)K, p, 0.0, &radii, (uint32_t)tolerancePercentage, upperBound));
+    }
+    // getPrecisionLevel (linear_octree.hpp:945-973): the first level whose node radii are below the
+    // tolerance; the z condition is the reference's `radii.getZ() && toleranceVector.getZ()`
+    uint32_t getPrecisionLevel(const Vector &toleranceVector) const {
+        fetch_tree();
+        uint32_t precisionLevel = info_.max_depth_seen;
+        for (uint32_t i = 0; i < info_.max_depth_seen; i++) {
+            const double *r = &radii_[3 * i];
+            if (r[0] < toleranceVector.getX() && r[1] < toleranceVector.getY() && r[2] && toleranceVector.getZ()) {
+                precisionLevel = i;
+                break;
+            }
+        }
+        return precisionLevel;
+    }
+    uint32_t getPrecisionLevel(double tolerance) const { return getPrecisionLevel(Vector{tolerance, tolerance, tolerance}); }
+    uint32_t getPrecisionLevel(const Vector &kernelRadii, double tolerancePercentage) const {
+        return getPrecisionLevel(Vector{kernelRadii.getX() * tolerancePercentage / 100.0,
+                                        kernelRadii.getY() * tolerancePercentage / 100.0,
+                                        kernelRadii.getZ() * tolerancePercentage / 100.0});
+    }
+
     // ---- kNN (linear_octree.hpp:899-942): min(n, k) nearest, ascending squared distance
     size_t knn(const Point &p, const size_t k, std::vector<size_t> &indexes, std::vector<double> &dists) const {
         const size_t m = std::min(points_.size(), k);
@@ -184,6 +218,7 @@ private:
     mutable std::vector<Point> centers_;
     mutable std::vector<std::pair<size_t, size_t>> ranges_;
     mutable std::vector<uint32_t> offsets_;
+    mutable std::vector<double> radii_;  // precomputedRadii, 22 x 3
 
     oct_ctx *ctx() const { return ctx_->get(); }
 
@@ -209,6 +244,21 @@ private:
             octamd::check(ctx(), oct_radius_fetch(ctx(), offs, l.entries.data(), l.entries.size()), "oct_radius_fetch");
         return l;
     }
+    Lists approx(int kernel, const Point &p, double radius, const Vector *radii, uint32_t level, bool upper) const {
+        const double c[3] = {p.getX(), p.getY(), p.getZ()};
+        std::lock_guard<std::mutex> g(ctx_->mutex());
+        double r3[3] = {0, 0, 0};
+        if (radii) r3[0] = radii->getX(), r3[1] = radii->getY(), r3[2] = radii->getZ();
+        octamd::check(ctx(), oct_radius_approx(ctx(), kernel, kPrecision, c, nullptr, nullptr, radius, radii ? r3 : nullptr, 1,
+                                               level, upper ? 1 : 0, 1, nullptr, nullptr),
+                      "oct_radius_approx");
+        uint64_t offs[2];
+        octamd::check(ctx(), oct_radius_fetch(ctx(), offs, nullptr, 0), "oct_radius_fetch");
+        Lists l{OCT_ALGO_STRUCT, std::vector<uint32_t>((size_t)offs[1] * 2)};
+        if (!l.entries.empty())
+            octamd::check(ctx(), oct_radius_fetch(ctx(), offs, l.entries.data(), l.entries.size()), "oct_radius_fetch");
+        return l;
+    }
     NeighborSet<Container> as_set(const Lists &l) const {
         NeighborSet<Container> s(&points_);
         for (size_t i = 0; i + 1 < l.entries.size(); i += 2) s.addRange(l.entries[i], l.entries[i + 1]);
@@ -230,7 +280,8 @@ private:
         std::vector<double> ce(3 * nt);
         offsets_.resize(nt);
         std::lock_guard<std::mutex> g(ctx_->mutex());
-        octamd::check(ctx(), oct_get_tree(ctx(), nullptr, offsets_.data(), rg.data(), ce.data(), nullptr), "oct_get_tree");
+        radii_.resize(66);
+        octamd::check(ctx(), oct_get_tree(ctx(), nullptr, offsets_.data(), rg.data(), ce.data(), radii_.data()), "oct_get_tree");
         centers_.resize(nt);
         ranges_.resize(nt);
         for (size_t i = 0; i < nt; i++) {

@@ -23,7 +23,7 @@ STAGES = ("bbox", "encode", "sort", "gather", "build", "search", "knn")
 EXPORTS = (
     "oct_create", "oct_destroy", "oct_last_error", "oct_synchronize", "oct_version", "oct_set_points",
     "oct_set_points_dev", "oct_bbox", "oct_encode", "oct_sort", "oct_sort_points", "oct_set_sorted", "oct_build",
-    "oct_get_tree", "oct_radius", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_knn_dev", "oct_knn_fetch", "oct_autotune_knn", "oct_tuned_knn", "oct_stage_times",
+    "oct_get_tree", "oct_radius", "oct_radius_approx", "oct_radius_fetch", "oct_radius_dev", "oct_knn", "oct_knn_dev", "oct_knn_fetch", "oct_autotune_knn", "oct_tuned_knn", "oct_stage_times",
     "oct_set_tuning", "oct_autotune_radius", "oct_tuned_radius", "oct_radius_box", "oct_encode_coords",
     "oct_decode_keys", "oct_batch_create", "oct_batch_destroy", "oct_batch_last_error", "oct_batch_add",
     "oct_batch_build", "oct_batch_info", "oct_batch_stage_times", "oct_batch_radius", "oct_batch_knn",
@@ -83,6 +83,8 @@ def lib():
         L.oct_knn.argtypes = [_vp, C.c_int, _vp, _vp, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_knn_dev.argtypes = [_vp, C.c_int, _vp, C.c_size_t, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_knn_fetch.argtypes = [_vp, C.c_size_t, C.c_size_t, _vp, _vp]
+        L.oct_radius_approx.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_double, _vp, C.c_size_t, C.c_uint32,
+                                        C.c_int, C.c_int, _vp, C.POINTER(SearchStats)]
         L.oct_batch_create.argtypes = [C.POINTER(_vp), C.c_int]
         L.oct_batch_destroy.argtypes = [_vp]
         L.oct_batch_last_error.argtypes = [_vp]
@@ -390,6 +392,27 @@ class Octree:
             else:
                 out.append(ent[a:b].astype(np.int64))
         return counts, out, st.as_dict()
+
+    def radius_approx(self, kernel, precision, level, upper, centers=None, idx=None, radius=0.0, radius3=None, nq=None,
+                      lists=False):
+        """neighborsApprox: Struct with a depth cutoff at `level`; upper-bound or lower-bound mode."""
+        if centers is not None:
+            centers = np.ascontiguousarray(centers, np.float64).reshape(-1, 3)
+            nq = centers.shape[0]
+        elif idx is not None:
+            idx = np.ascontiguousarray(idx, np.uint32)
+            nq = idx.shape[0]
+        elif nq is None:
+            nq = self.n
+        r3 = None if radius3 is None else np.ascontiguousarray(radius3, np.float64).reshape(3)
+        counts = np.empty(nq, np.uint32)
+        st = SearchStats()
+        self._chk(self.L.oct_radius_approx(self.h, kernel, precision, _ptr(centers), _ptr(idx), None, float(radius), _ptr(r3),
+                                           nq, int(level), 1 if upper else 0, 1 if lists else 0, _ptr(counts), C.byref(st)),
+                  "oct_radius_approx")
+        if not lists:
+            return counts, st.as_dict()
+        return self._fetch_lists(A_STRUCT, nq, counts, st)
 
     def radius_csr(self, kernel, algo, precision, idx, radius):
         """Result lists of a search over sorted-cloud indices as CSR: counts, offsets (nq + 1) and the flat

@@ -677,6 +677,8 @@ typedef struct {
     const double *xyz;
     kern_t k;
     int algo;
+    uint32_t level;  /* neighborsApprox: precision level and bound mode */
+    int upper;
     int count_only;
     vec_t out;
     uint64_t size;
@@ -690,15 +692,20 @@ static void add_range(search_t *s, uint64_t first, uint64_t last) { /* NeighborS
     }
 }
 
-/* continuationCriterion of the three variants (linear_octree.hpp:698-714, 749-769, 800-802) */
+/* continuationCriterion of the variants (linear_octree.hpp:698-714, 749-769, 800-802; neighborsApprox
+ * 984-1005: beyond the precision level a node is added whole in upper-bound mode and pruned either way) */
 static int criterion(search_t *s, uint32_t node, uint32_t depth) {
     const orc_tree *t = s->t;
     s->classified++;
     if (s->algo == ORC_A_NEIGHBORS) return kern_box_overlap(&s->k, &t->centers[3 * (size_t)node], &t->radii[3 * depth]);
+    if (s->algo == ORC_A_APPROX && depth > s->level) {
+        if (s->upper) add_range(s, t->ranges[2 * (size_t)node], t->ranges[2 * (size_t)node + 1]);
+        return 0;
+    }
     int j = kern_box_intersect(&s->k, &t->centers[3 * (size_t)node], &t->radii[3 * depth]);
     if (j == INSIDE) {
         uint64_t a = t->ranges[2 * (size_t)node], b = t->ranges[2 * (size_t)node + 1];
-        if (s->algo == ORC_A_STRUCT) {
+        if (s->algo == ORC_A_STRUCT || s->algo == ORC_A_APPROX) {
             add_range(s, a, b);
         } else {
             for (uint64_t i = a; i < b; ++i) {
@@ -716,7 +723,7 @@ static void endpoint(search_t *s, uint32_t node) {
     const orc_tree *t = s->t;
     uint64_t a = t->ranges[2 * (size_t)node], b = t->ranges[2 * (size_t)node + 1];
     s->tested += b - a;
-    if (s->algo == ORC_A_STRUCT) {
+    if (s->algo == ORC_A_STRUCT || s->algo == ORC_A_APPROX) {
         uint64_t rs = a;
         for (uint64_t i = a; i < b; ++i) {
             if (!kern_inside(&s->k, &s->xyz[3 * i])) {
@@ -777,20 +784,27 @@ static void kern_init3(kern_t *k, int type, const double c[3], const double r3[3
 }
 
 static orc_result *radius_any(const orc_tree *t, const double *xyz, int kernel, int algo, const double *centers,
-                              const double *radii, const double *radii3, size_t nq, uint64_t *stats);
+                              const double *radii, const double *radii3, size_t nq, uint64_t *stats, uint32_t level,
+                              int upper);
 
 orc_result *orc_radius(const orc_tree *t, const double *xyz, int kernel, int algo, const double *centers,
                        const double *radii, size_t nq, uint64_t *stats) {
-    return radius_any(t, xyz, kernel, algo, centers, radii, NULL, nq, stats);
+    return radius_any(t, xyz, kernel, algo, centers, radii, NULL, nq, stats, 0, 0);
 }
 
 orc_result *orc_radius_box(const orc_tree *t, const double *xyz, int kernel, int algo, const double *centers,
                            const double *radii3, size_t nq) {
-    return radius_any(t, xyz, kernel, algo, centers, NULL, radii3, nq, NULL);
+    return radius_any(t, xyz, kernel, algo, centers, NULL, radii3, nq, NULL, 0, 0);
+}
+
+orc_result *orc_radius_approx(const orc_tree *t, const double *xyz, int kernel, const double *centers,
+                              const double *radii, const double *radii3, size_t nq, uint32_t level, int upper) {
+    return radius_any(t, xyz, kernel, ORC_A_APPROX, centers, radii, radii3, nq, NULL, level, upper);
 }
 
 static orc_result *radius_any(const orc_tree *t, const double *xyz, int kernel, int algo, const double *centers,
-                              const double *radii, const double *radii3, size_t nq, uint64_t *stats) {
+                              const double *radii, const double *radii3, size_t nq, uint64_t *stats, uint32_t level,
+                              int upper) {
     vec_t *outs = (vec_t *)calloc(nq ? nq : 1, sizeof(vec_t));
     orc_result *r = (orc_result *)calloc(1, sizeof(orc_result));
     r->nq = nq;
@@ -799,7 +813,7 @@ static orc_result *radius_any(const orc_tree *t, const double *xyz, int kernel, 
     for (size_t q = 0; q < nq; q++) {
         search_t s;
         memset(&s, 0, sizeof(s));
-        s.t = t, s.xyz = xyz, s.algo = algo;
+        s.t = t, s.xyz = xyz, s.algo = algo, s.level = level, s.upper = upper;
         if (radii3)
             kern_init3(&s.k, kernel, &centers[3 * q], &radii3[3 * q]);
         else

@@ -52,6 +52,7 @@ extern "C" {
 #define ORC_A_NEIGHBORS 1
 #define ORC_A_PRUNE 2
 #define ORC_A_STRUCT 3
+#define ORC_A_APPROX 4 /* neighborsApprox: Struct with a depth cutoff */
 
 /* geometry (inc/geometry/box.hpp:94-137, util.hpp:139-143) */
 void orc_mbb(const double *xyz, size_t n, double center[3], double radii[3]);
@@ -97,6 +98,10 @@ orc_result *orc_radius(const orc_tree *t, const double *sorted_xyz, int kernel, 
  * linear_octree.hpp:1044-1060, 1069-1075. */
 orc_result *orc_radius_box(const orc_tree *t, const double *sorted_xyz, int kernel, int algo, const double *centers,
                            const double *radii3, size_t nq);
+/* neighborsApprox (linear_octree.hpp:981-1035): Struct with a depth cutoff; a node deeper than
+ * `level` is added whole (upper != 0) or dropped, without a kernel test. radii3 NULL: scalar radii. */
+orc_result *orc_radius_approx(const orc_tree *t, const double *xyz, int kernel, const double *centers,
+                              const double *radii, const double *radii3, size_t nq, uint32_t level, int upper);
 size_t orc_result_total(const orc_result *r);        /* total entries (indices, or ranges) */
 const uint64_t *orc_result_offsets(const orc_result *r); /* nq+1 offsets into entries */
 const uint64_t *orc_result_entries(const orc_result *r); /* indices, or 2*ranges interleaved */

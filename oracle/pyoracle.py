@@ -13,7 +13,7 @@ _LIB = None
 
 ENC_MORTON, ENC_HILBERT = 1, 5
 K_CIRCLE, K_SPHERE, K_SQUARE, K_CUBE = 0, 1, 2, 3
-A_NEIGHBORS, A_PRUNE, A_STRUCT = 1, 2, 3
+A_NEIGHBORS, A_PRUNE, A_STRUCT, A_APPROX = 1, 2, 3, 4
 
 _P = C.c_void_p
 _U64P = np.ctypeslib.ndpointer(np.uint64, flags="C")
@@ -61,6 +61,8 @@ def lib():
         L.orc_radius_harness.restype = C.c_uint64
         L.orc_radius_box.argtypes = [_P, _F64P, C.c_int, C.c_int, _F64P, _F64P, C.c_size_t]
         L.orc_radius_box.restype = _P
+        L.orc_radius_approx.argtypes = [_P, _F64P, C.c_int, _F64P, _F64P, C.c_void_p, C.c_size_t, C.c_uint32, C.c_int]
+        L.orc_radius_approx.restype = _P
         L.orc_result_total.argtypes = [_P]
         L.orc_result_total.restype = C.c_size_t
         for f in ("orc_result_offsets", "orc_result_entries", "orc_result_sizes"):
@@ -218,6 +220,17 @@ def radius_box(tree, sorted_xyz, kernel, algo, centers, radii3):
     r3 = _f64(np.broadcast_to(np.asarray(radii3, np.float64), (nq, 3)))
     res = lib().orc_radius_box(tree.h, _f64(sorted_xyz), kernel, algo, centers, r3, nq)
     return _unpack(res, nq, algo == A_STRUCT)
+
+
+def radius_approx(tree, sorted_xyz, kernel, centers, level, upper, radii=None, radii3=None):
+    """neighborsApprox: Struct ranges with a depth cutoff at `level` (upper / lower bound mode)."""
+    centers = _f64(centers).reshape(-1, 3)
+    nq = centers.shape[0]
+    r = _f64(np.broadcast_to(np.asarray(0.0 if radii is None else radii, np.float64), (nq,)))
+    r3 = None if radii3 is None else _f64(np.broadcast_to(np.asarray(radii3, np.float64), (nq, 3)))
+    res = lib().orc_radius_approx(tree.h, _f64(sorted_xyz), kernel, centers, r, None if r3 is None else r3.ctypes.data,
+                                  nq, int(level), 1 if upper else 0)
+    return _unpack(res, nq, True)
 
 
 def radius_count(tree, sorted_xyz, kernel, algo, centers, radii):

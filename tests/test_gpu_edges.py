@@ -216,3 +216,31 @@ def test_radius_box_vector_radii(ctx):
                     np.testing.assert_array_equal(lists[q], ref[q])
     with pytest.raises(A.OctError):
         ctx.radius_box(A.K_SPHERE, A.A_PRUNE, [1.0, 1.0, 1.0], centers=centers)
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3, 5, 21])
+def test_radius_approx_levels(ctx, level):
+    """neighborsApprox counts and Struct range lists against the oracle, both bound modes."""
+    print(f"[gpu] approx level={level}", flush=True)
+    ctx.current = None
+    xyz = A.lidar_tile(60_000, seed=8, extent=50.0)
+    sx, keys, perm, c, r = O.sort_points(O.ENC_HILBERT, xyz)
+    tree = O.Tree(keys, 32, O.ENC_HILBERT, c, r)
+    ctx.set_points(xyz)
+    ctx.sort_points(A.ENC_HILBERT)
+    ctx.build(32)
+    idx = A.random_indices(400, len(xyz), seed=level)
+    centers = sx[idx.astype(np.int64)]
+    for upper in (True, False):
+        for kernel in (A.K_SPHERE, A.K_CIRCLE, A.K_CUBE, A.K_SQUARE):
+            ref, ref_sizes = O.radius_approx(tree, sx, kernel, centers, level, upper, radii=1.7)
+            for prec in (A.PREC_FP64, A.PREC_FP32):
+                counts, lists, _ = ctx.radius_approx(kernel, prec, level, upper, idx=idx, radius=1.7, lists=True)
+                np.testing.assert_array_equal(counts, ref_sizes, err_msg=f"k={kernel} upper={upper} prec={prec}")
+                for q in range(len(idx)):
+                    np.testing.assert_array_equal(lists[q], ref[q])
+                cnt_only, _ = ctx.radius_approx(kernel, prec, level, upper, idx=idx, radius=1.7)
+                np.testing.assert_array_equal(cnt_only, ref_sizes)
+        ref, ref_sizes = O.radius_approx(tree, sx, A.K_CUBE, centers, level, upper, radii3=[1.0, 2.0, 0.5])
+        counts, lists, _ = ctx.radius_approx(A.K_CUBE, A.PREC_FP64, level, upper, idx=idx, radius3=[1.0, 2.0, 0.5], lists=True)
+        np.testing.assert_array_equal(counts, ref_sizes)

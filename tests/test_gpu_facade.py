@@ -19,7 +19,7 @@ EXE = os.path.join(ROOT, "octrees-benchmark_amd", "facade_check")
 
 
 def parse(text):
-    out = {"pt": [], "struct": {}, "prune": {}, "neighbors": {}, "knn": {}}
+    out = {"pt": [], "struct": {}, "prune": {}, "neighbors": {}, "knn": {}, "approx": {}}
     for line in text.splitlines():
         f = line.split()
         if not f:
@@ -37,6 +37,12 @@ def parse(text):
                 out["struct"][key] = np.array([[int(a) for a in v.split(":")] for v in vals], np.int64).reshape(-1, 2)
             else:
                 out[f[0]][key] = np.array([int(v) for v in vals], np.int64)
+        elif f[0] == "approx":
+            lv, up = (int(v) for v in f[1].split(":"))
+            out["approx"][(lv, up, int(f[2]))] = np.array([[int(a) for a in v.split(":")] for v in f[4:]],
+                                                          np.int64).reshape(-1, 2)
+        elif f[0] == "precision_level":
+            out["precision_level"] = (int(f[1]), int(f[2]))
         elif f[0] == "knn":
             out["knn"][int(f[1])] = [(int(v.split(":")[0]), float(v.split(":")[1])) for v in f[3:]]
         elif f[0] in ("countall", "traversal_leaves"):
@@ -78,6 +84,21 @@ def test_facade_matches_oracle(enc):
             ref, sizes = O.radius(tree, sx, kernel, algo, centers, radius)
             for q in range(len(centers)):
                 np.testing.assert_array_equal(got[tag][(name, q)], ref[q], err_msg=f"{tag} {name} q={q}")
+    # neighborsApprox (level = the tolerance argument, as the reference's overload) and getPrecisionLevel
+    for lv in range(4):
+        for up in (0, 1):
+            ref, _ = O.radius_approx(tree, sx, O.K_SPHERE, centers, lv, bool(up), radii=radius)
+            for q in range(len(centers)):
+                np.testing.assert_array_equal(got["approx"][(lv, up, q)], ref[q], err_msg=f"approx {lv}:{up} q={q}")
+
+    def precision_level(tol):  # linear_octree.hpp:945-956, its z condition included
+        rad = np.asarray(tree.radii).reshape(-1, 3)
+        for i in range(tree.maxDepthSeen):
+            if rad[i, 0] < tol[0] and rad[i, 1] < tol[1] and rad[i, 2] != 0 and tol[2] != 0:
+                return i
+        return tree.maxDepthSeen
+
+    assert got["precision_level"] == (precision_level([radius] * 3), precision_level([radius * 50.0 / 100.0] * 3))
     bi, bd = O.knn_brute(sx, centers, 7)
     for q in range(len(centers)):
         np.testing.assert_array_equal([i for i, _ in got["knn"][q]], bi[q])

@@ -454,3 +454,32 @@ def test_radius_box_vector_radii_equal_brute_force(radius_case, kernel, algo):
     np.testing.assert_array_equal(s1, s2)
     for a, b in zip(o1, o2):
         np.testing.assert_array_equal(a, b)
+
+
+def test_approx_bounds_and_cutoff():
+    """neighborsApprox (linear_octree.hpp:981-1035): reference KAT test_complex.cpp:116-142 (upper >=
+    exact >= lower at precision maxDepth / 2), the lower bound's points are a subset of the exact set
+    and the exact set of the upper bound's, and with the level at or beyond the tree's depth both
+    modes equal neighborsStruct range for range."""
+    rng = np.random.default_rng(42)
+    xyz = rng.uniform(0.0, 10.0, (1000, 3))
+    sx, keys, perm, c, r = O.sort_points(O.ENC_MORTON, xyz)
+    t = O.Tree(keys, 32, O.ENC_MORTON, c, r)
+    centers = np.vstack([[5.0, 5.0, 5.0], sx[::97]])
+    ex, es = O.radius(t, sx, O.K_SPHERE, O.A_STRUCT, centers, 2.0)
+    up, us = O.radius_approx(t, sx, O.K_SPHERE, centers, 21 // 2, True, radii=2.0)
+    lo, ls = O.radius_approx(t, sx, O.K_SPHERE, centers, 21 // 2, False, radii=2.0)
+    assert np.all(us >= es) and np.all(ls <= es)
+    expand = lambda rr: set(i for a, b in rr for i in range(int(a), int(b)))  # noqa: E731
+    for level in range(0, t.maxDepthSeen + 2):
+        up, us = O.radius_approx(t, sx, O.K_SPHERE, centers, level, True, radii=2.0)
+        lo, ls = O.radius_approx(t, sx, O.K_SPHERE, centers, level, False, radii=2.0)
+        for q in range(len(centers)):
+            e = expand(ex[q])
+            assert expand(lo[q]) <= e <= expand(up[q]), (level, q)
+            if level >= t.maxDepthSeen:
+                np.testing.assert_array_equal(up[q], ex[q])
+                np.testing.assert_array_equal(lo[q], ex[q])
+    # level 0, upper bound: an OVERLAP root adds all 8 children whole
+    up, us = O.radius_approx(t, sx, O.K_SPHERE, [[5.0, 5.0, 5.0]], 0, True, radii=2.0)
+    assert us[0] == len(sx)
