@@ -348,9 +348,10 @@ int oct_get_tree(oct_ctx *ctx, uint64_t *prefixes, uint32_t *offsets, uint64_t *
     return OCT_OK;
 }
 
-static int check_search_args(oct_ctx *ctx, int kernel, int algo, int precision) {
+static int check_search_args(oct_ctx *ctx, int kernel, int algo, int precision, bool approx = false) {
     if (kernel < OCT_KERNEL_CIRCLE || kernel > OCT_KERNEL_CUBE) return fail(ctx, OCT_ERR_ARG, "unknown kernel id");
-    if (algo < OCT_ALGO_NEIGHBORS || algo > OCT_ALGO_STRUCT) return fail(ctx, OCT_ERR_ARG, "unknown search algorithm id");
+    if (algo == OCT_ALGO_APPROX && !approx) return fail(ctx, OCT_ERR_ARG, "neighborsApprox runs through oct_radius_approx");
+    if (algo < OCT_ALGO_NEIGHBORS || algo > OCT_ALGO_APPROX) return fail(ctx, OCT_ERR_ARG, "unknown search algorithm id");
     if (precision != OCT_PREC_FP64 && precision != OCT_PREC_FP32_RECHECK) return fail(ctx, OCT_ERR_ARG, "unknown precision");
     if (!ctx->has_tree) return fail(ctx, OCT_ERR_STATE, "search before oct_build");
     return OCT_OK;
@@ -386,7 +387,7 @@ static int radius_impl(oct_ctx *ctx, int kernel, int algo, int precision, const 
                        const uint32_t *center_idx, const double *radii, double radius, const double *radii3,
                        const double *radius3, size_t nq, int want_lists, uint32_t *counts_out, oct_search_stats *stats,
                        uint32_t approx_level = ~0u, bool approx_upper = false) {
-    int s = check_search_args(ctx, kernel, algo, precision);
+    int s = check_search_args(ctx, kernel, algo, precision, algo == OCT_ALGO_APPROX);
     if (s) return s;
     if (nq >= (1ull << 31)) return fail(ctx, OCT_ERR_ARG, "too many queries");
     if (!centers && !center_idx && nq > ctx->n) return fail(ctx, OCT_ERR_ARG, "all-points search with nq > n");
@@ -447,7 +448,7 @@ static int radius_impl(oct_ctx *ctx, int kernel, int algo, int precision, const 
         uint64_t total = 0;
         HCHK(hipMemcpyAsync(&total, ctx->qoff.p + nq, 8, hipMemcpyDeviceToHost, ctx->stream), "copy total");
         HCHK(hipStreamSynchronize(ctx->stream), "scan");
-        const size_t words = (size_t)total * (algo == OCT_ALGO_STRUCT ? 2 : 1);
+        const size_t words = (size_t)total * (algo == OCT_ALGO_STRUCT || algo == OCT_ALGO_APPROX ? 2 : 1);
         HCHK(ctx->entries.reserve(words + 1), "alloc lists");
         L.mode = 2;
         L.offsets = ctx->qoff.p;
@@ -470,6 +471,7 @@ int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *
                const double *radii, double radius, size_t nq, int want_lists, uint32_t *counts_out,
                oct_search_stats *stats) {
     if (!ctx) return OCT_ERR_ARG;
+    if (algo == OCT_ALGO_APPROX) return fail(ctx, OCT_ERR_ARG, "neighborsApprox runs through oct_radius_approx");
     return radius_impl(ctx, kernel, algo, precision, centers, center_idx, radii, radius, nullptr, nullptr, nq, want_lists,
                        counts_out, stats);
 }
@@ -478,6 +480,7 @@ int oct_radius_box(oct_ctx *ctx, int kernel, int algo, const double *centers, co
                    const double *radii3, const double radius3[3], size_t nq, int want_lists, uint32_t *counts_out,
                    oct_search_stats *stats) {
     if (!ctx) return OCT_ERR_ARG;
+    if (algo == OCT_ALGO_APPROX) return fail(ctx, OCT_ERR_ARG, "neighborsApprox runs through oct_radius_approx");
     if (kernel != OCT_KERNEL_CUBE && kernel != OCT_KERNEL_SQUARE)
         return fail(ctx, OCT_ERR_UNSUPPORTED, "per-axis radii need the cube or square kernel (kernel_factory.hpp:33-41)");
     if (!radii3 && !radius3) return fail(ctx, OCT_ERR_ARG, "no radii");
@@ -491,7 +494,7 @@ int oct_radius_approx(oct_ctx *ctx, int kernel, int precision, const double *cen
     if (!ctx) return OCT_ERR_ARG;
     if (radius3 && kernel != OCT_KERNEL_CUBE && kernel != OCT_KERNEL_SQUARE)
         return fail(ctx, OCT_ERR_UNSUPPORTED, "per-axis radii need the cube or square kernel (kernel_factory.hpp:33-41)");
-    return radius_impl(ctx, kernel, OCT_ALGO_STRUCT, radius3 ? OCT_PREC_FP64 : precision, centers, center_idx, radii, radius,
+    return radius_impl(ctx, kernel, OCT_ALGO_APPROX, radius3 ? OCT_PREC_FP64 : precision, centers, center_idx, radii, radius,
                        nullptr, radius3, nq, want_lists, counts_out, stats, level, upper_bound != 0);
 }
 
@@ -531,7 +534,7 @@ int oct_radius_fetch(oct_ctx *ctx, uint64_t *offsets, uint32_t *entries, size_t 
     const size_t nq = ctx->last_nq;
     HCHK(hipMemcpyAsync(offsets, ctx->qoff.p, (nq + 1) * 8, hipMemcpyDeviceToHost, ctx->stream), "copy offsets");
     HCHK(hipStreamSynchronize(ctx->stream), "fetch");
-    const size_t words = (size_t)offsets[nq] * (ctx->last_algo == OCT_ALGO_STRUCT ? 2 : 1);
+    const size_t words = (size_t)offsets[nq] * (ctx->last_algo == OCT_ALGO_STRUCT || ctx->last_algo == OCT_ALGO_APPROX ? 2 : 1);
     if (words && entries) {  // entries == NULL: size query (offsets only)
         if (entries_capacity < words) return fail(ctx, OCT_ERR_ARG, "entries buffer too small");
         HCHK(hipMemcpyAsync(entries, ctx->entries.p, words * 4, hipMemcpyDeviceToHost, ctx->stream), "copy lists");

@@ -157,7 +157,7 @@ struct RadiusLaunch {
     int group = 16;  // count mode: queries per wave (16, 32, 64), or 0 for the one-lane-per-query kernel
     bool mx = false;  // count mode, FP32 sphere: point tests on the matrix cores
     bool remap = true;  // count mode, vector point tests: compact the queries of sparsely wanted leaves
-    uint32_t approx_level = ~0u;  // Struct only: neighborsApprox depth cutoff (~0u: none) and bound mode
+    uint32_t approx_level = ~0u;  // OCT_ALGO_APPROX: neighborsApprox depth cutoff and bound mode
     bool approx_upper = false;
     // count mode over a forest of clouds (batches): wave tiles of 64 slots, tile_info[tile] = cloud |
     // valid slots << 20, roots[cloud], radtab[cloud * 66 + 3 * level + axis]; null: the context's cloud

@@ -44,7 +44,7 @@ struct Sink {
     uint32_t *entries = nullptr;
 
     __device__ inline void range(uint32_t first, uint32_t last) {  // INSIDE node
-        if (ALGO == OCT_ALGO_STRUCT) {                             // NeighborSet::addRange keeps empty ranges
+        if (ALGO == OCT_ALGO_STRUCT || ALGO == OCT_ALGO_APPROX) {  // NeighborSet::addRange keeps empty ranges
             if (MODE == 2) {
                 entries[2 * (base + ecnt)] = first;
                 entries[2 * (base + ecnt) + 1] = last;
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
         if (!(m & lanebit)) return -1;
         // neighborsApprox beyond its precision level: the node's range whole, or nothing
         // (linear_octree.hpp:987-994)
-        if (ALGO == OCT_ALGO_STRUCT && d > a.approx_level) return a.approx_upper ? J_INSIDE : J_OUTSIDE;
+        if (ALGO == OCT_ALGO_APPROX && d > a.approx_level) return a.approx_upper ? J_INSIDE : J_OUTSIDE;
         if (ALGO == OCT_ALGO_NEIGHBORS)
             return overlap64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]) ? J_OVERLAP : J_OUTSIDE;
         return classify64<KERNEL>(q, nr.cx, nr.cy, nr.cz, a.rad[d][0], a.rad[d][1], a.rad[d][2]);
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
                 in = inside64<KERNEL>(q, p.x, p.y, p.z);
             }
             in = in && act;
-            if (ALGO == OCT_ALGO_STRUCT) {
+            if (ALGO == OCT_ALGO_STRUCT || ALGO == OCT_ALGO_APPROX) {
                 if (act && !in) {
                     if (rs < i) sink.run(rs, i);
                     rs = i + 1;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
                 sink.point(i);
             }
         }
-        if (ALGO == OCT_ALGO_STRUCT && act && rs < last) sink.run(rs, last);
+        if ((ALGO == OCT_ALGO_STRUCT || ALGO == OCT_ALGO_APPROX) && act && rs < last) sink.run(rs, last);
     };
 
     // ---- root (singleTraversal, linear_octree.hpp:655-662)
@@ -268,7 +268,8 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
             const NodeRec nr = a.nodes[c];
             // An empty child cannot change counts or index lists whatever its class; only Struct
             // range lists keep empty INSIDE ranges (neighbor_set.hpp:84-89).
-            if (!(ALGO == OCT_ALGO_STRUCT && MODE != 0) && uniform_u32(nr.first) == uniform_u32(nr.last)) continue;
+            if (!((ALGO == OCT_ALGO_STRUCT || ALGO == OCT_ALGO_APPROX) && MODE != 0) && uniform_u32(nr.first) == uniform_u32(nr.last))
+                continue;
             n_class += __popcll(m);
             n_cslots += 64;
             const int j = judge(m, nr, d);
@@ -719,7 +720,7 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
     // judges all 8 children for its own query (the S lanes of a query agree); results are lane
     // masks, so the per-child decisions are scalar-unit work.
     auto judge8 = [&](ConstNodeExt &E, uint32_t d, bool act, bool (&in)[8], bool (&ov)[8]) {
-        if (ALGO == OCT_ALGO_STRUCT && d > a.approx_level) {
+        if (ALGO == OCT_ALGO_APPROX && d > a.approx_level) {
             // neighborsApprox beyond its precision level: every child whole, or none
             // (linear_octree.hpp:987-994)
 #pragma unroll
@@ -904,6 +905,14 @@ static void launch_count_s(const SearchArgs &a, int G, hipStream_t s) {
 
 template <int KERNEL, int ALGO, int PREC>
 static void launch_count(const SearchArgs &a, int G, bool mx, hipStream_t s) {
+    if constexpr (ALGO == OCT_ALGO_APPROX) {  // one shape: 64 queries per wave, vector ALUs
+        const unsigned nb = (unsigned)(((size_t)a.nq + 64 * kWaves - 1) / (64 * kWaves));
+        if (a.stats)
+            k_count<KERNEL, ALGO, PREC, 64, true, false><<<nb, kSearchBlock, 0, s>>>(a);
+        else
+            k_count<KERNEL, ALGO, PREC, 64, false, false><<<nb, kSearchBlock, 0, s>>>(a);
+        return;
+    }
     if constexpr (KERNEL == OCT_KERNEL_SPHERE && PREC == OCT_PREC_FP32_RECHECK) {
         if (mx) {
             if (a.stats)
@@ -945,8 +954,10 @@ static void launch_a(const SearchArgs &a, int algo, int prec, int mode, int G, b
         launch_p<KERNEL, OCT_ALGO_NEIGHBORS>(a, prec, mode, G, mx, nb, s);
     else if (algo == OCT_ALGO_PRUNE)
         launch_p<KERNEL, OCT_ALGO_PRUNE>(a, prec, mode, G, mx, nb, s);
-    else
+    else if (algo == OCT_ALGO_STRUCT)
         launch_p<KERNEL, OCT_ALGO_STRUCT>(a, prec, mode, G, mx, nb, s);
+    else
+        launch_p<KERNEL, OCT_ALGO_APPROX>(a, prec, mode, G, mx, nb, s);
 }
 
 hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {

@@ -28,8 +28,8 @@ struct SearchArgs {
     double r3[3];
     int use_r3;
     int remap;  // count kernel: compact the queries of a leaf wanted by at most half the wave
-    // neighborsApprox (Struct with a depth cutoff): nodes deeper than approx_level are added whole
-    // (approx_upper) or dropped without a kernel test; ~0u: no cutoff
+    // neighborsApprox (algorithm OCT_ALGO_APPROX: Struct with a depth cutoff): nodes deeper than
+    // approx_level are added whole (approx_upper) or dropped without a kernel test
     uint32_t approx_level;
     int approx_upper;
     // Count kernel's tree geometry and roots: level radii radtab[cloud * 66 + 3 * level + axis]; for a
