@@ -8,7 +8,7 @@ namespace oct {
 
 constexpr int kSearchBlock = 256;
 constexpr int kWaves = kSearchBlock / 64;
-constexpr int kStack = 8 * 22 + 8;
+constexpr int kStack = 7 * 21 + 13;  // depth first over <= 21 internal levels: <= 7 pending siblings per level + 8
 
 enum { J_OUTSIDE = 0, J_OVERLAP = 1, J_INSIDE = 2 };
 
