@@ -153,7 +153,8 @@ def batch_workload(args):
     sizes = np.exp(rng.uniform(np.log(1e4), np.log(4e6), C))
     sizes = np.maximum(1e4, sizes * (200e6 / sizes.sum())).astype(np.int64)
     t0 = time.time()
-    b = A.Batch(0)
+    b = A.Batch(0)   # built on the device in one HIP graph; the searches run on this forest
+    bp = A.Batch(0)  # the same clouds built one by one (per-cloud contexts), for comparison
     clouds = []
     for i, n in enumerate(sizes):
         n = int(n)
@@ -164,13 +165,26 @@ def batch_workload(args):
         else:
             xyz = A.lidar_tile(n, seed=1000 + i, extent=1000.0 * (n / 50e6) ** 0.5, offset=(off[0], off[1]))
         b.add(xyz, A.ENC_HILBERT, args.max_leaf)
+        bp.add(xyz, A.ENC_HILBERT, args.max_leaf)
         clouds.append(xyz if i < 3 else None)
     gen_s = time.time() - t0
-    log(f"[batch] {C} clouds, {int(sizes.sum())} points generated and uploaded in {gen_s:.1f} s")
+    npts = int(sizes.sum())
+    log(f"[batch] {C} clouds, {npts} points generated and uploaded in {gen_s:.1f} s")
     t0 = time.time()
-    b.build()
+    first_ms = b.build_device(graph=True)
+    first_s = time.time() - t0
+    replay_ms, replay_wall = [], []
+    for _ in range(max(1, args.steps)):
+        t = time.perf_counter()
+        replay_ms.append(b.build_device(graph=True))
+        replay_wall.append(time.perf_counter() - t)
+    rep = float(np.median(replay_ms))
+    log(f"[batch] device build (bbox, keys, segmented sort, forest) in one graph: first call {first_s:.2f} s wall "
+        f"({first_ms:.1f} ms device), replays {rep:.2f} ms device, {float(np.median(replay_wall)) * 1e3:.1f} ms wall")
+    t0 = time.time()
+    bp.build()
     build_s = time.time() - t0
-    log(f"[batch] built every cloud in {build_s:.2f} s")
+    log(f"[batch] per-cloud build of every cloud in {build_s:.2f} s")
     per = 50_000
     cloud = np.repeat(np.arange(C, dtype=np.uint32), per)
     idx = (rng.uniform(0.0, 1.0, C * per) * np.repeat(sizes, per)).astype(np.uint32)
@@ -180,8 +194,11 @@ def batch_workload(args):
     rq, kq = np.nonzero(~is_knn)[0], np.nonzero(is_knn)[0]
     out = {"workload": f"{C} clouds, {int(sizes.sum())} points (log-uniform 1e4..4e6), {per} centres each, radius "
                        f"U[0.5, 5] m, 20 % kNN with k U{{8..256}}, Hilbert, maxLeaf {args.max_leaf}",
-           "clouds": C, "points": int(sizes.sum()), "radius_queries": int(len(rq)), "knn_queries": int(len(kq)),
-           "build_all_clouds_s": build_s}
+           "clouds": C, "points": npts, "radius_queries": int(len(rq)), "knn_queries": int(len(kq)),
+           "build_device_graph": {"replay_device_ms": rep, "replay_wall_ms": float(np.median(replay_wall)) * 1e3,
+                                  "points_per_s": npts / (rep * 1e-3), "first_call_wall_s": first_s,
+                                  "replays": len(replay_ms)},
+           "build_per_cloud_s": build_s}
     for name, prec in (("fp32", A.PREC_FP32), ("fp64", A.PREC_FP64)):
         b.radius(A.K_SPHERE, A.A_STRUCT, prec, cloud[rq], idx[rq], radii[rq])
         t = time.perf_counter()
@@ -197,6 +214,12 @@ def batch_workload(args):
                               "queries_per_s_device": len(kq) / max(st["kernel_ms"] * 1e-3, 1e-12)}
         log(f"[batch] {name}: radius {dev:.1f} ms device ({wall:.2f} s wall), knn {st['kernel_ms']:.1f} ms device")
         out[f"_{name}"] = (counts, ki, kd)
+    # the device-built forest against the per-cloud-built one: same counts, same kNN sets (a sample)
+    cp, _ = bp.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, cloud[rq], idx[rq], radii[rq])
+    ksub = kq[:100_000]
+    pki, pkd, _ = bp.knn(A.PREC_FP32, cloud[ksub], idx[ksub], kk[ksub], 256)
+    same_as_per_cloud = bool(np.array_equal(cp, out["_fp32"][0])) and bool(np.array_equal(pki, out["_fp32"][1][:len(ksub)]))
+    bp.close()
     # exact against the oracle on the first three clouds (each on its own)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -218,6 +241,7 @@ def batch_workload(args):
             m = int(kk[kq[q]])
             ok &= bool(np.array_equal(out["_fp32"][1][q, :m].astype(np.int64), bi[row, :m]))
     out["verify"] = {"fp32_eq_fp64": bool(np.array_equal(out["_fp32"][0], out["_fp64"][0])),
+                     "device_build_eq_per_cloud_build": same_as_per_cloud,
                      "oracle_sample_three_clouds": ok}
     for name in ("_fp32", "_fp64"):
         del out[name]

@@ -231,6 +231,22 @@ const char *oct_batch_last_error(const oct_batch *batch);
 int oct_batch_add(oct_batch *batch, const double *xyz, size_t n, size_t stride_bytes, int encoder, uint32_t max_points_leaf,
                   int *cloud_id);
 int oct_batch_build(oct_batch *batch);
+/* Device-driven build of every cloud at once, with the same keys, permutations and trees as
+ * oct_batch_build: per-cloud boxes and keys, one sort segmented by cloud, and a level-synchronous
+ * build of the whole forest (each level of every cloud in one launch), enqueued with no host
+ * synchronisation between the stages. With use_graph != 0 the sequence is captured in a HIP graph
+ * on the first call and the graph is replayed on later calls; call again after oct_batch_set_points
+ * to rebuild (the point counts stay). *ms (may be NULL) receives the device time of the sequence. */
+int oct_batch_build_device(oct_batch *batch, int use_graph, double *ms);
+/* Replaces the points of one cloud (same count; stride as oct_set_points) of a batch that is not
+ * built yet or was built by oct_batch_build_device; the batch must be built again before searching. */
+int oct_batch_set_points(oct_batch *batch, int cloud, const double *xyz, size_t stride_bytes);
+/* Parity dumps of one cloud of a built batch: its sorted keys and permutation (indices into the
+ * cloud's points as added), and (oct_batch_build_device only) its tree in the reference's node
+ * order, with the arrays and NULL rules of oct_get_tree. */
+int oct_batch_cloud_sorted(oct_batch *batch, int cloud, uint64_t *keys, uint32_t *perm);
+int oct_batch_cloud_tree(oct_batch *batch, int cloud, uint64_t *prefixes, uint32_t *offsets, uint64_t *ranges,
+                         double *centers, double *radii);
 int oct_batch_info(const oct_batch *batch, int cloud, size_t *n, oct_tree_info *info);
 /* device milliseconds of the last batch searches, as oct_stage_times ([5] radius, [6] kNN: the last class) */
 int oct_batch_stage_times(const oct_batch *batch, double ms[8]);

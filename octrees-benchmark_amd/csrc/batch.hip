@@ -1,20 +1,26 @@
 // Batches of independent clouds (no counterpart in the reference, whose LinearOctree holds one cloud;
 // the reference harness runs its clouds one after another).
 //
-// Each cloud is encoded, sorted and built by its own context, exactly as a single cloud (same keys,
-// permutation and tree arrays). oct_batch_build then lays the clouds out as one forest in a context
-// of its own: sorted points concatenated, every cloud's node arrays copied to a node offset that is
-// a multiple of 8 with point ranges and child indices rebased, so the count and kNN kernels walk any
-// cloud from its root with the unchanged child -> NodeExt rule ((first child - 1) / 8). A search
-// over many clouds is then one launch per search kind: queries are grouped per cloud into wave tiles
-// (64 slots for counts, 64 / lanes for kNN), and each tile names its cloud, which selects the root
-// and that cloud's level radii.
+// oct_batch_add appends each cloud's points to one device buffer. Two builds lay the clouds out as
+// one forest that the count and kNN kernels walk from any cloud's root:
+//  - oct_batch_build: each cloud is encoded, sorted and built by a context of its own, exactly as a
+//    single cloud, and its arrays are copied to point / node offsets (node offsets multiples of 8,
+//    point ranges and child indices rebased), cloud-major;
+//  - oct_batch_build_device: one device-driven sequence over all clouds at once (per-cloud boxes and
+//    keys, a sort segmented by cloud, a level-synchronous forest build) with no host
+//    synchronisation between its launches, optionally captured in a HIP graph on the first call and
+//    replayed on later calls; nodes are level-major over the whole forest (pipeline.hip).
+// Both give every cloud the same keys, permutation and tree as the single-cloud path, and both keep
+// the child -> NodeExt rule ((first child - 1) / 8). A search over many clouds is then one launch per
+// search kind: queries are grouped per cloud into wave tiles (64 slots for counts, 64 / lanes for
+// kNN), and each tile names its cloud, which selects the root and that cloud's level radii.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
 #include <vector>
 
 #include "device.hpp"
+#include "pipeline.hpp"
 
 struct oct_batch {
     int device = 0;
@@ -28,7 +34,10 @@ struct oct_batch {
         oct_tree_info info{};
     };
     std::vector<Cloud> clouds;
+    oct::DevBuf<double> xyz_all;  // every cloud's points, packed x, y, z, in the order added
+    size_t n_all = 0;
     bool built = false;
+    int mode = 0;  // 1: per-cloud contexts (oct_batch_build), 2: device-driven (oct_batch_build_device)
     std::vector<double> h_rad, h_mag;
     std::vector<uint32_t> h_roots;
     oct::DevBuf<double> radtab, magtab;
@@ -36,6 +45,24 @@ struct oct_batch {
     oct::DevBuf<double> qrad;
     oct::DevBuf<int32_t> kidx;
     oct::DevBuf<double> kd2;
+    // device-driven build
+    bool dev_ready = false;
+    uint32_t n_chunks = 0, n_tiles = 0, R = 0, cap = 0;
+    oct::DevBuf<oct::Chunk> chunks;
+    oct::DevBuf<oct::SortTile> stiles;
+    oct::DevBuf<uint32_t> cloud_chunk0, cloud_off, cloud_enc, cloud_leaf;
+    oct::DevBuf<oct::CloudGeom> geom;
+    oct::DevBuf<double> partial;
+    oct::DevBuf<uint32_t> hist, bases, ctrl, part, cl_internal, cl_depth, ncloud;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint32_t lv[oct::kLvCount] = {0};
+    double build_ms = 0.0;
+    // host copy of the forest for per-cloud tree dumps (filled on first use after a build)
+    bool dump_ready = false;
+    std::vector<oct::NodeRec> h_nodes;
+    std::vector<uint64_t> h_nkey;
+    std::vector<uint32_t> h_child, h_ncloud;
 };
 
 namespace oct {
@@ -68,6 +95,25 @@ __global__ void k_rebase_ext(const NodeExt *__restrict__ src, uint32_t n, uint32
     dst[i] = e;
 }
 
+// grow a device buffer to hold `need` elements, keeping its first `valid` (stream-ordered copy)
+template <class T>
+static hipError_t grow_keep(DevBuf<T> &b, size_t need, size_t valid, hipStream_t s) {
+    if (need <= b.cap) return hipSuccess;
+    size_t cap = b.cap ? b.cap : 1024;
+    while (cap < need) cap *= 2;
+    T *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, cap * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (b.p && valid) {
+        if ((e = hipMemcpyAsync(p, b.p, valid * sizeof(T), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    }
+    if (b.p) (void)hipFree(b.p);
+    b.p = p;
+    b.cap = cap;
+    return hipSuccess;
+}
+
 }  // namespace oct
 
 using namespace oct;
@@ -79,13 +125,19 @@ static int bfail(oct_batch *b, int code, const std::string &msg) {
 #define BCHK(call, what)                                                                      \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess) return bfail(b, OCT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return bfail(b, e_ == hipErrorOutOfMemory ? OCT_ERR_NOMEM : OCT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
     } while (0)
 #define BAPI(call, what)                                                                                   \
     do {                                                                                                   \
         int r_ = (call);                                                                                   \
         if (r_ != OCT_OK) return bfail(b, r_, std::string(what) + ": " + oct_last_error(ctx_for_error)); \
     } while (0)
+
+static void drop_graph(oct_batch *b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr, b->graph = nullptr;
+}
 
 extern "C" {
 
@@ -108,8 +160,13 @@ int oct_batch_destroy(oct_batch *b) {
         if (c.ctx) oct_destroy(c.ctx);
     hipSetDevice(b->device);
     hipStreamSynchronize(b->forest->stream);
+    drop_graph(b);
+    b->xyz_all.release();
     b->radtab.release(), b->magtab.release(), b->roots.release(), b->tile_info.release(), b->order.release();
     b->qidx.release(), b->kq.release(), b->counts.release(), b->qrad.release(), b->kidx.release(), b->kd2.release();
+    b->chunks.release(), b->stiles.release(), b->cloud_chunk0.release(), b->cloud_off.release(), b->cloud_enc.release();
+    b->cloud_leaf.release(), b->geom.release(), b->partial.release(), b->hist.release(), b->bases.release();
+    b->ctrl.release(), b->part.release(), b->cl_internal.release(), b->cl_depth.release(), b->ncloud.release();
     oct_destroy(b->forest);
     delete b;
     return OCT_OK;
@@ -117,49 +174,90 @@ int oct_batch_destroy(oct_batch *b) {
 
 const char *oct_batch_last_error(const oct_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
+// packed copy of n points (stride bytes apart) into the batch buffer at point offset `at`
+static int put_points(oct_batch *b, const double *xyz, size_t n, size_t stride_bytes, size_t at) {
+    if (!n) return OCT_OK;
+    hipStream_t s = b->forest->stream;
+    double *dst = b->xyz_all.p + 3 * at;
+    if (stride_bytes == 24) {
+        BCHK(hipMemcpyAsync(dst, xyz, n * 24, hipMemcpyHostToDevice, s), "copy points");
+    } else {
+        std::vector<double> packed(3 * n);
+        const size_t sd = stride_bytes / 8;
+        for (size_t i = 0; i < n; i++)
+            for (int a = 0; a < 3; a++) packed[3 * i + a] = xyz[sd * i + a];
+        BCHK(hipMemcpyAsync(dst, packed.data(), n * 24, hipMemcpyHostToDevice, s), "copy points");
+        BCHK(hipStreamSynchronize(s), "copy points");  // the staging vector ends here
+    }
+    return OCT_OK;
+}
+
 int oct_batch_add(oct_batch *b, const double *xyz, size_t n, size_t stride_bytes, int encoder, uint32_t max_points_leaf,
                   int *cloud_id) {
     if (!b) return OCT_ERR_ARG;
     if (b->built) return bfail(b, OCT_ERR_STATE, "batch already built");
     if (encoder != OCT_ENC_MORTON3D && encoder != OCT_ENC_HILBERT3D) return bfail(b, OCT_ERR_UNSUPPORTED, "encoder");
     if (max_points_leaf == 0) return bfail(b, OCT_ERR_ARG, "max_points_leaf must be positive");
+    if (stride_bytes < 24 || stride_bytes % 8) return bfail(b, OCT_ERR_ARG, "stride_bytes must be >= 24 and a multiple of 8");
+    if (n && !xyz) return bfail(b, OCT_ERR_ARG, "null point buffer");
+    if (n >= (1ull << 30)) return bfail(b, OCT_ERR_ARG, "at most 2^30 - 1 points per cloud");
+    if (b->n_all + n >= 0xffffffffull) return bfail(b, OCT_ERR_ARG, "batch beyond 2^32 - 1 points");
     if (b->clouds.size() >= (1u << 20) - 1) return bfail(b, OCT_ERR_ARG, "too many clouds in one batch");
+    BCHK(hipSetDevice(b->device), "set device");
+    BCHK(grow_keep(b->xyz_all, 3 * (b->n_all + n) + 3, 3 * b->n_all, b->forest->stream), "alloc points");
+    int r = put_points(b, xyz, n, stride_bytes, b->n_all);
+    if (r) return r;
     oct_batch::Cloud c;
-    int r = oct_create(&c.ctx, b->device);
-    if (r != OCT_OK) return bfail(b, r, "context of a cloud");
-    oct_ctx *ctx_for_error = c.ctx;
-    if ((r = oct_set_points(c.ctx, xyz, n, stride_bytes)) != OCT_OK) {
-        std::string m = oct_last_error(ctx_for_error);
-        oct_destroy(c.ctx);
-        return bfail(b, r, "set points: " + m);
-    }
-    c.encoder = encoder, c.max_leaf = max_points_leaf, c.n = n;
+    c.encoder = encoder, c.max_leaf = max_points_leaf, c.n = n, c.pt_off = b->n_all;
+    b->n_all += n;
     b->clouds.push_back(c);
     if (cloud_id) *cloud_id = (int)b->clouds.size() - 1;
     return OCT_OK;
 }
 
+int oct_batch_set_points(oct_batch *b, int cloud, const double *xyz, size_t stride_bytes) {
+    if (!b) return OCT_ERR_ARG;
+    if (cloud < 0 || (size_t)cloud >= b->clouds.size()) return bfail(b, OCT_ERR_ARG, "cloud id out of range");
+    if (b->built && b->mode != 2) return bfail(b, OCT_ERR_STATE, "points of a batch built per cloud cannot change");
+    if (stride_bytes < 24 || stride_bytes % 8) return bfail(b, OCT_ERR_ARG, "stride_bytes must be >= 24 and a multiple of 8");
+    const auto &c = b->clouds[(size_t)cloud];
+    if (c.n && !xyz) return bfail(b, OCT_ERR_ARG, "null point buffer");
+    BCHK(hipSetDevice(b->device), "set device");
+    int r = put_points(b, xyz, c.n, stride_bytes, c.pt_off);
+    if (r) return r;
+    BCHK(hipStreamSynchronize(b->forest->stream), "copy points");
+    b->built = false;  // the forest no longer matches the points until the next oct_batch_build_device
+    b->dump_ready = false;
+    return OCT_OK;
+}
+
 int oct_batch_build(oct_batch *b) {
     if (!b) return OCT_ERR_ARG;
-    if (b->built) return bfail(b, OCT_ERR_STATE, "batch already built");
+    if (b->built || b->mode) return bfail(b, OCT_ERR_STATE, "batch already built");
     BCHK(hipSetDevice(b->device), "set device");
+    BCHK(hipStreamSynchronize(b->forest->stream), "points");
     // every cloud on its own: bbox, encode, sort, build (the single-cloud path, bit-identical)
     size_t pts = 0, nodes = 0;
     for (auto &c : b->clouds) {
+        int r = oct_create(&c.ctx, b->device);
+        if (r != OCT_OK) return bfail(b, r, "context of a cloud");
         oct_ctx *ctx_for_error = c.ctx;
         double cen[3], rad[3];
+        BAPI(oct_set_points_dev(c.ctx, b->xyz_all.p + 3 * c.pt_off, c.n, 24), "points of a cloud");
         BAPI(oct_sort_points(c.ctx, c.encoder, cen, rad, nullptr, nullptr), "sort a cloud");
         BAPI(oct_build(c.ctx, c.max_leaf, &c.info), "build a cloud");
         BCHK(hipStreamSynchronize(c.ctx->stream), "build a cloud");
-        c.pt_off = pts, c.node_off = nodes;
+        c.node_off = nodes;
         pts += c.n;
         nodes += ((size_t)c.info.n_total + 7) / 8 * 8;
     }
-    if (pts >= 0xffffffffull || nodes >= 0xffffffffull) return bfail(b, OCT_ERR_ARG, "batch beyond 2^32 points or nodes");
+    if (nodes >= 0xffffffffull) return bfail(b, OCT_ERR_ARG, "batch beyond 2^32 nodes");
     // the forest
     oct_ctx *f = b->forest;
     BCHK(f->p64.reserve(pts + 1), "alloc forest points");
     BCHK(f->p32.reserve(pts + 1), "alloc forest points");
+    BCHK(f->keys[0].reserve(pts + 1), "alloc forest keys");
+    BCHK(f->vals[0].reserve(pts + 1), "alloc forest keys");
     BCHK(f->nodes.reserve(nodes + 8), "alloc forest nodes");
     BCHK(f->child.reserve(nodes + 8), "alloc forest nodes");
     BCHK(f->amax.reserve(nodes + 8), "alloc forest nodes");
@@ -173,6 +271,8 @@ int oct_batch_build(oct_batch *b) {
         if (c.n) {
             BCHK(hipMemcpyAsync(f->p64.p + c.pt_off, x->p64.p, c.n * sizeof(double4), hipMemcpyDeviceToDevice, f->stream), "copy");
             BCHK(hipMemcpyAsync(f->p32.p + c.pt_off, x->p32.p, c.n * sizeof(float4), hipMemcpyDeviceToDevice, f->stream), "copy");
+            BCHK(hipMemcpyAsync(f->keys[0].p + c.pt_off, x->sorted_keys, c.n * 8, hipMemcpyDeviceToDevice, f->stream), "copy");
+            BCHK(hipMemcpyAsync(f->vals[0].p + c.pt_off, x->perm, c.n * 4, hipMemcpyDeviceToDevice, f->stream), "copy");
         }
         BCHK(hipMemcpyAsync(f->amax.p + c.node_off, x->amax.p, nt * sizeof(float), hipMemcpyDeviceToDevice, f->stream), "copy");
         const unsigned nb = (nt + 255) / 256;
@@ -186,6 +286,10 @@ int oct_batch_build(oct_batch *b) {
             for (int a = 0; a < 3; a++) b->h_rad[66 * i + 3 * l + a] = x->level_radii[l][a];
         for (int a = 0; a < 3; a++) b->h_mag[3 * i + a] = std::fabs(x->center[a]) + std::fabs(x->radii[a]);
         b->h_roots[i] = (uint32_t)c.node_off;
+        // the context's work must finish before the context (and its buffers) go
+        BCHK(hipStreamSynchronize(f->stream), "assemble the forest");
+        oct_destroy(c.ctx);
+        b->clouds[i].ctx = nullptr;
     }
     BCHK(b->radtab.reserve(b->h_rad.size()), "alloc");
     BCHK(b->magtab.reserve(b->h_mag.size()), "alloc");
@@ -196,9 +300,238 @@ int oct_batch_build(oct_batch *b) {
     BCHK(hipStreamSynchronize(f->stream), "assemble the forest");
     f->n = pts;
     f->has_tree = true;
-    // the clouds' own contexts are no longer needed: the forest holds every array the searches read
-    for (auto &c : b->clouds) oct_destroy(c.ctx), c.ctx = nullptr;
     b->built = true;
+    b->mode = 1;
+    return OCT_OK;
+}
+
+// ------------------------------------------------------------------ device-driven build
+// Work tables (chunks, sort tiles, cloud offsets) and every buffer the sequence touches; node
+// storage at b->cap. Called before the first build and again when the node capacity grows.
+static int device_setup(oct_batch *b) {
+    oct_ctx *f = b->forest;
+    hipStream_t s = f->stream;
+    const uint32_t C = (uint32_t)b->clouds.size();
+    const size_t n = b->n_all;
+    if (!b->dev_ready) {
+        std::vector<Chunk> ch;
+        std::vector<SortTile> st;
+        std::vector<uint32_t> c0(C + 1), off(C + 1), enc(C ? C : 1), leaf(C ? C : 1);
+        size_t est = 0;
+        for (uint32_t c = 0; c < C; c++) {
+            const auto &cl = b->clouds[c];
+            c0[c] = (uint32_t)ch.size();
+            off[c] = (uint32_t)cl.pt_off;
+            enc[c] = (uint32_t)cl.encoder, leaf[c] = cl.max_leaf;
+            for (size_t p = 0; p < cl.n; p += kChunkPoints)
+                ch.push_back(Chunk{(uint32_t)(cl.pt_off + p), (uint32_t)std::min<size_t>(kChunkPoints, cl.n - p), c, 0u});
+            const uint32_t first = (uint32_t)st.size();
+            for (size_t p = 0; p < cl.n; p += kSortTile)
+                st.push_back(SortTile{(uint32_t)(cl.pt_off + p), (uint32_t)std::min<size_t>(kSortTile, cl.n - p), c, first});
+            est += 64 + 8 * (cl.n / cl.max_leaf);
+        }
+        c0[C] = (uint32_t)ch.size();
+        off[C] = (uint32_t)n;
+        b->n_chunks = (uint32_t)ch.size();
+        b->n_tiles = (uint32_t)st.size();
+        b->R = C ? (C + 6) / 8 * 8 + 1 : 1;
+        if (b->cap == 0) b->cap = (uint32_t)std::min<size_t>(b->R + est, 0xfffffff0ull);
+        BCHK(b->chunks.reserve(ch.size() + 1), "alloc");
+        BCHK(b->stiles.reserve(st.size() + 1), "alloc");
+        BCHK(b->cloud_chunk0.reserve(C + 1), "alloc");
+        BCHK(b->cloud_off.reserve(C + 1), "alloc");
+        BCHK(b->cloud_enc.reserve(C + 1), "alloc");
+        BCHK(b->cloud_leaf.reserve(C + 1), "alloc");
+        if (!ch.empty()) BCHK(hipMemcpyAsync(b->chunks.p, ch.data(), ch.size() * sizeof(Chunk), hipMemcpyHostToDevice, s), "copy");
+        if (!st.empty())
+            BCHK(hipMemcpyAsync(b->stiles.p, st.data(), st.size() * sizeof(SortTile), hipMemcpyHostToDevice, s), "copy");
+        BCHK(hipMemcpyAsync(b->cloud_chunk0.p, c0.data(), (C + 1) * 4, hipMemcpyHostToDevice, s), "copy");
+        BCHK(hipMemcpyAsync(b->cloud_off.p, off.data(), (C + 1) * 4, hipMemcpyHostToDevice, s), "copy");
+        BCHK(hipMemcpyAsync(b->cloud_enc.p, enc.data(), C * 4, hipMemcpyHostToDevice, s), "copy");
+        BCHK(hipMemcpyAsync(b->cloud_leaf.p, leaf.data(), C * 4, hipMemcpyHostToDevice, s), "copy");
+        BCHK(hipStreamSynchronize(s), "copy tables");  // the host vectors end here
+        BCHK(b->geom.reserve(C + 1), "alloc");
+        BCHK(b->radtab.reserve(66 * (size_t)(C + 1)), "alloc");
+        BCHK(b->magtab.reserve(3 * (size_t)(C + 1)), "alloc");
+        BCHK(b->roots.reserve(C + 1), "alloc");
+        BCHK(b->partial.reserve(6 * (size_t)b->n_chunks + 6), "alloc");
+        BCHK(b->hist.reserve((size_t)C * kSegBaseStride + 1), "alloc");
+        BCHK(b->bases.reserve((size_t)C * kSegBaseStride + 1), "alloc");
+        BCHK(f->lookback.reserve((size_t)kRadixPassCount * b->n_tiles * 256 + 1), "alloc");
+        BCHK(b->ctrl.reserve(kCtrlWords), "alloc");
+        BCHK(b->part.reserve(1024), "alloc");
+        BCHK(b->cl_internal.reserve(C + 1), "alloc");
+        BCHK(b->cl_depth.reserve(C + 1), "alloc");
+        BCHK(f->keys[0].reserve(n + 1), "alloc keys");
+        BCHK(f->keys[1].reserve(n + 1), "alloc keys");
+        BCHK(f->vals[0].reserve(n + 1), "alloc keys");
+        BCHK(f->vals[1].reserve(n + 1), "alloc keys");
+        BCHK(f->p64.reserve(n + 1), "alloc points");
+        BCHK(f->p32.reserve(n + 1), "alloc points");
+        b->dev_ready = true;
+    }
+    const size_t cap = b->cap;
+    BCHK(f->nodes.reserve(cap), "alloc nodes");
+    BCHK(f->nkey.reserve(cap), "alloc nodes");
+    BCHK(f->child.reserve(cap + 1), "alloc nodes");
+    BCHK(f->flags.reserve(cap), "alloc nodes");
+    BCHK(f->amax.reserve(cap), "alloc nodes");
+    BCHK(b->ncloud.reserve(cap), "alloc nodes");
+    BCHK(f->next.reserve(cap / 8 + 1), "alloc nodes");
+    return OCT_OK;
+}
+
+static ForestBuild forest_args(oct_batch *b) {
+    oct_ctx *f = b->forest;
+    ForestBuild F;
+    F.n = b->n_all, F.C = (uint32_t)b->clouds.size(), F.R = b->R, F.cap = b->cap;
+    F.xyz = b->xyz_all.p;
+    F.chunks = b->chunks.p, F.nchunks = b->n_chunks;
+    F.cloud_chunk0 = b->cloud_chunk0.p, F.cloud_off = b->cloud_off.p, F.cloud_enc = b->cloud_enc.p, F.cloud_leaf = b->cloud_leaf.p;
+    F.tiles = b->stiles.p, F.ntiles = b->n_tiles;
+    F.geom = b->geom.p, F.radtab = b->radtab.p, F.magtab = b->magtab.p, F.roots = b->roots.p;
+    F.partial = b->partial.p;
+    F.keys[0] = f->keys[0].p, F.keys[1] = f->keys[1].p, F.vals[0] = f->vals[0].p, F.vals[1] = f->vals[1].p;
+    F.hist = b->hist.p, F.bases = b->bases.p, F.lookback = f->lookback.p, F.ctrl = b->ctrl.p, F.part = b->part.p;
+    F.cl_internal = b->cl_internal.p, F.cl_depth = b->cl_depth.p;
+    F.p64 = f->p64.p, F.p32 = f->p32.p;
+    F.nodes = f->nodes.p, F.nkey = f->nkey.p, F.child = f->child.p, F.flags = f->flags.p, F.ncloud = b->ncloud.p;
+    F.amax = f->amax.p, F.ext = f->next.p;
+    return F;
+}
+
+int oct_batch_build_device(oct_batch *b, int use_graph, double *ms_out) {
+    if (!b) return OCT_ERR_ARG;
+    if (b->mode == 1) return bfail(b, OCT_ERR_STATE, "batch already built per cloud");
+    if (b->clouds.empty()) return bfail(b, OCT_ERR_STATE, "empty batch");
+    BCHK(hipSetDevice(b->device), "set device");
+    oct_ctx *f = b->forest;
+    hipStream_t s = f->stream;
+    BCHK(hipStreamSynchronize(s), "points");
+    const uint32_t C = (uint32_t)b->clouds.size();
+    std::vector<uint32_t> ctrl(kCtrlWords), cint(C), cdep(C);
+    for (int attempt = 0;; attempt++) {
+        int r = device_setup(b);
+        if (r) return r;
+        if (use_graph && !b->gexec) {
+            BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "begin capture");
+            hipError_t e = enqueue_forest_build(s, forest_args(b));
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(s, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return bfail(b, OCT_ERR_HIP, std::string("capture the build: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            b->graph = g;
+            BCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0), "instantiate the build graph");
+        }
+        BCHK(hipEventRecord(f->ev0[ST_BUILD], s), "event");
+        if (use_graph) {
+            BCHK(hipGraphLaunch(b->gexec, s), "launch the build graph");
+        } else {
+            BCHK(enqueue_forest_build(s, forest_args(b)), "build");
+        }
+        BCHK(hipEventRecord(f->ev1[ST_BUILD], s), "event");
+        BCHK(hipMemcpyAsync(ctrl.data(), b->ctrl.p, kCtrlWords * 4, hipMemcpyDeviceToHost, s), "copy status");
+        BCHK(hipMemcpyAsync(cint.data(), b->cl_internal.p, C * 4, hipMemcpyDeviceToHost, s), "copy status");
+        BCHK(hipMemcpyAsync(cdep.data(), b->cl_depth.p, C * 4, hipMemcpyDeviceToHost, s), "copy status");
+        BCHK(hipStreamSynchronize(s), "build");
+        if (ctrl[kCtrlSortErr]) return bfail(b, OCT_ERR_STATE, "sort: look-back wait exceeded its bound");
+        if (ctrl[kCtrlBuildErr] & kBuildErrDepth) return bfail(b, OCT_ERR_STATE, "build: split below the maximum depth");
+        if (ctrl[kCtrlBuildErr] & kBuildErrCapacity) {
+            if (attempt >= 8 || b->cap >= 0x7fffffffu) return bfail(b, OCT_ERR_NOMEM, "build: node capacity");
+            b->cap = (uint32_t)std::min<uint64_t>(2ull * b->cap, 0xfffffff0ull);
+            drop_graph(b);  // node buffers move
+            continue;
+        }
+        break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, f->ev0[ST_BUILD], f->ev1[ST_BUILD]);
+    f->ev_used[ST_BUILD] = true;
+    b->build_ms = ms;
+    if (ms_out) *ms_out = ms;
+    for (uint32_t l = 0; l < kLvCount; l++) b->lv[l] = ctrl[kCtrlLv + l];
+    b->h_rad.assign(66 * (size_t)C, 0.0);
+    b->h_mag.assign(3 * (size_t)C, 0.0);
+    BCHK(hipMemcpyAsync(b->h_rad.data(), b->radtab.p, 66 * (size_t)C * 8, hipMemcpyDeviceToHost, s), "copy radii");
+    BCHK(hipMemcpyAsync(b->h_mag.data(), b->magtab.p, 3 * (size_t)C * 8, hipMemcpyDeviceToHost, s), "copy radii");
+    BCHK(hipStreamSynchronize(s), "copy radii");
+    for (uint32_t c = 0; c < C; c++) {
+        auto &cl = b->clouds[c];
+        oct_tree_info &t = cl.info;
+        t.n_internal = cint[c];
+        t.n_total = 1 + 8 * cint[c];
+        t.n_leaf = t.n_total - t.n_internal;
+        t.max_depth_seen = cdep[c];
+        t.n_levels = cdep[c] + 1;
+        t.max_points_leaf = cl.max_leaf;
+        t.build_ms = ms;  // the whole batch's device time
+    }
+    f->n = b->n_all;
+    f->has_tree = true;
+    b->built = true;
+    b->mode = 2;
+    b->dump_ready = false;
+    return OCT_OK;
+}
+
+// sorted keys and permutation (indices into the cloud's points as added) of one cloud
+int oct_batch_cloud_sorted(oct_batch *b, int cloud, uint64_t *keys_out, uint32_t *perm_out) {
+    if (!b) return OCT_ERR_ARG;
+    if (!b->built) return bfail(b, OCT_ERR_STATE, "batch not built");
+    if (cloud < 0 || (size_t)cloud >= b->clouds.size()) return bfail(b, OCT_ERR_ARG, "cloud id out of range");
+    const auto &c = b->clouds[(size_t)cloud];
+    if (!c.n) return OCT_OK;
+    BCHK(hipSetDevice(b->device), "set device");
+    oct_ctx *f = b->forest;
+    if (keys_out) BCHK(hipMemcpyAsync(keys_out, f->keys[0].p + c.pt_off, c.n * 8, hipMemcpyDeviceToHost, f->stream), "copy keys");
+    if (perm_out) BCHK(hipMemcpyAsync(perm_out, f->vals[0].p + c.pt_off, c.n * 4, hipMemcpyDeviceToHost, f->stream), "copy perm");
+    BCHK(hipStreamSynchronize(f->stream), "copy");
+    if (perm_out && b->mode == 2)  // the device build sorts global positions
+        for (size_t i = 0; i < c.n; i++) perm_out[i] -= (uint32_t)c.pt_off;
+    return OCT_OK;
+}
+
+// One cloud's tree in the reference's numbering, as oct_get_tree (device-built batches: the cloud's
+// nodes taken in forest order are level-major, key-minor).
+int oct_batch_cloud_tree(oct_batch *b, int cloud, uint64_t *prefixes, uint32_t *offsets, uint64_t *ranges, double *centers,
+                         double *radii) {
+    if (!b) return OCT_ERR_ARG;
+    if (!b->built || b->mode != 2) return bfail(b, OCT_ERR_STATE, "tree dumps need oct_batch_build_device");
+    if (cloud < 0 || (size_t)cloud >= b->clouds.size()) return bfail(b, OCT_ERR_ARG, "cloud id out of range");
+    BCHK(hipSetDevice(b->device), "set device");
+    oct_ctx *f = b->forest;
+    const uint32_t nt = b->lv[kLvCount - 1];
+    if (!b->dump_ready) {
+        b->h_nodes.resize(nt), b->h_nkey.resize(nt), b->h_child.resize(nt), b->h_ncloud.resize(nt);
+        BCHK(hipMemcpyAsync(b->h_nodes.data(), f->nodes.p, (size_t)nt * sizeof(NodeRec), hipMemcpyDeviceToHost, f->stream), "copy");
+        BCHK(hipMemcpyAsync(b->h_nkey.data(), f->nkey.p, (size_t)nt * 8, hipMemcpyDeviceToHost, f->stream), "copy");
+        BCHK(hipMemcpyAsync(b->h_child.data(), f->child.p, (size_t)nt * 4, hipMemcpyDeviceToHost, f->stream), "copy");
+        BCHK(hipMemcpyAsync(b->h_ncloud.data(), b->ncloud.p, (size_t)nt * 4, hipMemcpyDeviceToHost, f->stream), "copy");
+        BCHK(hipStreamSynchronize(f->stream), "copy");
+        b->dump_ready = true;
+    }
+    const auto &c = b->clouds[(size_t)cloud];
+    std::vector<uint32_t> ids;
+    ids.reserve(c.info.n_total);
+    for (uint32_t u = 0; u < nt; u++)
+        if (b->h_ncloud[u] == (uint32_t)cloud) ids.push_back(u);
+    if (ids.size() != c.info.n_total) return bfail(b, OCT_ERR_STATE, "forest node count of the cloud is inconsistent");
+    std::vector<uint32_t> local(nt, 0u);
+    for (size_t i = 0; i < ids.size(); i++) local[ids[i]] = (uint32_t)i;
+    uint32_t level = 0;
+    for (size_t i = 0; i < ids.size(); i++) {
+        const uint32_t u = ids[i];
+        while (level + 1 < kLvCount && b->lv[level + 1] <= u) level++;
+        const NodeRec &r = b->h_nodes[u];
+        if (prefixes) prefixes[i] = encode_placeholder(b->h_nkey[u], level);
+        if (offsets) offsets[i] = b->h_child[u] ? local[b->h_child[u]] : 0u;
+        if (ranges) ranges[2 * i] = r.first - c.pt_off, ranges[2 * i + 1] = r.last - c.pt_off;
+        if (centers) centers[3 * i] = r.cx, centers[3 * i + 1] = r.cy, centers[3 * i + 2] = r.cz;
+    }
+    if (radii)
+        for (int k = 0; k < 66; k++) radii[k] = b->h_rad[66 * (size_t)cloud + k];
     return OCT_OK;
 }
 

@@ -13,7 +13,7 @@
 // range (the reference's lower_bound counting, :317-321); its FP64 centre is getCenter
 // (point_encoder.hpp:517-539) evaluated in the reference's operation order.
 // A last pass writes the leaf-recentred FP32 copy of the cloud used by the FP32 searches.
-#include "device.hpp"
+#include "tree_common.hpp"
 
 namespace oct {
 
@@ -24,26 +24,9 @@ struct BuildGeom {
     int encoder;
 };
 
-__device__ inline uint32_t lower_bound_keys(const uint64_t *__restrict__ keys, uint32_t lo, uint32_t hi, uint64_t v) {
-    while (lo < hi) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        if (keys[mid] < v)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 __device__ inline void write_node(NodeRec *__restrict__ nodes, uint64_t *__restrict__ nkey, uint32_t u, uint64_t key,
                                   uint32_t level, uint32_t first, uint32_t last, const BuildGeom &g) {
-    double c[3];
-    node_center(g.encoder, key, level, g.boxmin, g.hl, g.rad[level], c);
-    NodeRec r;
-    r.cx = c[0], r.cy = c[1], r.cz = c[2];
-    r.first = first, r.last = last;
-    nodes[u] = r;
-    nkey[u] = key;
+    put_node(nodes, nkey, u, key, level, first, last, g.encoder, g.boxmin, g.hl, g.rad[level]);
 }
 
 __global__ void k_root(NodeRec *nodes, uint64_t *nkey, uint32_t *flags, uint32_t n, uint32_t max_leaf, BuildGeom g) {
@@ -81,9 +64,7 @@ __global__ __launch_bounds__(256) void k_children(const uint64_t *__restrict__ k
     }
 }
 
-// Leaf-recentred FP32 coordinates: a = fl32(p - c_leaf) per axis (FP64 subtraction, one rounding to
-// FP32), w = fl32(|a|^2) computed in FP64 from the FP32 values. One wave per leaf; the wave also
-// records amax = max |a|^2 over the leaf (rounded up), which bounds the FP32 rounding band.
+// Leaf-recentred FP32 coordinates, one wave per leaf (recentre_leaf).
 __global__ __launch_bounds__(256) void k_recentre(const double4 *__restrict__ p64, const NodeRec *__restrict__ nodes,
                                                  const uint32_t *__restrict__ child, uint32_t n_total, float4 *__restrict__ p32,
                                                  float *__restrict__ amax) {
@@ -91,29 +72,11 @@ __global__ __launch_bounds__(256) void k_recentre(const double4 *__restrict__ p6
     const uint32_t lane = threadIdx.x & 63;
     if (wave >= n_total) return;
     if (child[wave] != 0) return;
-    const NodeRec r = nodes[wave];
-    double m = 0.0;
-    bool bad = false;  // a non-finite FP32 point (NaN / overflow): the leaf's FP32 tests are never trusted
-    for (uint32_t i = r.first + lane; i < r.last; i += 64) {
-        double4 p = p64[i];
-        float ax = (float)(p.x - r.cx), ay = (float)(p.y - r.cy), az = (float)(p.z - r.cz);
-        double w = (double)ax * (double)ax + (double)ay * (double)ay + (double)az * (double)az;
-        const float fw = (float)w;
-        p32[i] = make_float4(ax, ay, az, fw);
-        bad |= !(fw < __builtin_huge_valf());
-        m = w > m ? w : m;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        double o = __shfl_xor(m, off);
-        m = o > m ? o : m;
-    }
-    if (__ballot(bad)) m = __builtin_huge_val();
-    if (lane == 0) amax[wave] = __double2float_ru(m);
+    const float m = recentre_leaf(p64, nodes[wave], lane, p32);
+    if (lane == 0) amax[wave] = m;
 }
 
-// Per internal node u (first child fc, level L): its children regrouped by corner bits
-// (NodeExt). A child's bits come from its decoded key at level L + 1, so they are exact also on
-// degenerate axes where two children share a centre value.
+// Per internal node u: its children regrouped by corner bits (make_node_ext).
 struct LevelTable {
     uint32_t start[kMaxDepth + 3];
 };
@@ -126,24 +89,7 @@ __global__ __launch_bounds__(256) void k_node_ext(const NodeRec *__restrict__ no
     if (fc == 0) return;
     uint32_t level = 0;
     while (level + 1 < kMaxDepth + 2 && lt.start[level + 1] <= u) level++;
-    const uint32_t sh = kMaxDepth - (level + 1);  // corner bit of the children's level
-    NodeExt e;
-    for (int k = 0; k < 4; k++) e.pad[k] = 0;
-    for (uint32_t k = 0; k < 8; k++) {
-        const uint32_t v = fc + k;
-        uint32_t x, y, z;
-        if (encoder == OCT_ENC_HILBERT3D)
-            hilbert_decode(nkey[v], x, y, z);
-        else
-            morton_decode(nkey[v], x, y, z);
-        const uint32_t bx = (x >> sh) & 1u, by = (y >> sh) & 1u, bz = (z >> sh) & 1u;
-        const uint32_t c = bx | (by << 1) | (bz << 2);
-        const NodeRec r = nodes[v];
-        e.ax[bx] = r.cx, e.ax[2 + by] = r.cy, e.ax[4 + bz] = r.cz;
-        e.first[c] = r.first, e.last[c] = r.last;
-        e.gc[c] = child[v];
-        e.amax[c] = child[v] == 0 ? amax[v] : 0.0f;
-    }
+    const NodeExt e = make_node_ext(nodes, nkey, child, amax, fc, level, encoder);
     ext[(fc - 1) >> 3] = e;
 }
 

@@ -59,6 +59,16 @@ struct DevBuf {
     }
 };
 
+// Radix sort geometry: tiles of kSortTile keys, kRadixPassCount passes of 8 bits.
+constexpr int kSortTile = 4096;
+constexpr int kRadixPassCount = 8;
+// A sort tile of a batch: [start, start + len) inside cloud `cloud`, whose first tile is `first`.
+struct SortTile {
+    uint32_t start, len, cloud, first;
+};
+// Per-cloud bin bases of a segmented sort: bases[cloud * kSegBaseStride + pass * 256 + digit].
+constexpr uint32_t kSegBaseStride = kRadixPassCount * 256;
+
 enum Stage { ST_BBOX = 0, ST_ENCODE, ST_SORT, ST_GATHER, ST_BUILD, ST_SEARCH, ST_KNN, ST_N };
 
 }  // namespace oct
@@ -209,6 +219,14 @@ struct KnnLaunch {
 hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err);
 // lanes-per-query shapes offered for k (default first); returns how many
 int knn_lane_options(uint32_t k, int out[3]);
+// Batches: stable sort of every cloud's (key, value) range at once (8 passes; keys0 / vals0 in and
+// out, the values of the first pass are the global positions); lookback (8 x ntiles x 256) and
+// counters (8) must be zero.
+hipError_t launch_sort_segmented(hipStream_t s, uint64_t *keys0, uint64_t *keys1, uint32_t *vals0, uint32_t *vals1,
+                                 size_t n, const SortTile *tiles, size_t ntiles, const uint32_t *bases,
+                                 uint32_t *lookback, uint32_t *counters, uint32_t *err);
+// out[i] = (xyz[3 perm[i]], xyz[3 perm[i] + 1], xyz[3 perm[i] + 2], 0)
+hipError_t launch_gather_raw(hipStream_t s, const double *xyz, const uint32_t *perm, double4 *out, size_t n);
 hipError_t exclusive_scan_u32_to_u64(oct_ctx *ctx, const uint32_t *in, uint64_t *out, size_t n);
 hipError_t exclusive_scan_u32(oct_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev);
 

@@ -17,7 +17,7 @@ namespace oct {
 
 constexpr int kSortBlock = 256;
 constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortBlock * kSortItems;
+static_assert(kSortBlock * kSortItems == kSortTile, "sort tile size");
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagInc = 2u << 30, kValMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 24;
 
@@ -31,11 +31,16 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
+// SEG = false: one key range [0, n), tile t covers [t * kSortTile, ...). SEG = true (batches): tiles
+// come from a table, each inside one cloud's range; the look-back stops at the cloud's first tile and
+// bin offsets are the cloud's own (bin_base + cloud * kSegBaseStride), so every cloud is sorted on
+// its own, stably, in one launch.
+template <bool SEG>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                          uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, size_t n,
                                                          uint32_t shift, const uint32_t *__restrict__ bin_base,
                                                          uint32_t *__restrict__ lookback, uint32_t *__restrict__ tile_counter,
-                                                         uint32_t *__restrict__ err) {
+                                                         uint32_t *__restrict__ err, const SortTile *__restrict__ tiles) {
     __shared__ uint64_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_wave[kSortBlock / 64][256];
@@ -49,7 +54,14 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
     for (uint32_t i = tid; i < (kSortBlock / 64) * 256; i += kSortBlock) (&s_wave[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t tile_base = (size_t)tile * kSortTile;
+    size_t tile_base, limit;
+    uint32_t first_tile = 0, cloud = 0;
+    if constexpr (SEG) {
+        const SortTile T = tiles[tile];
+        tile_base = T.start, limit = (size_t)T.start + T.len, first_tile = T.first, cloud = T.cloud;
+    } else {
+        tile_base = (size_t)tile * kSortTile, limit = n;
+    }
     const size_t wave_base = tile_base + (size_t)w * 64 * kSortItems;
 
     uint64_t k[kSortItems];
@@ -57,7 +69,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         size_t p = wave_base + (size_t)r * 64 + lane;
-        bool valid = p < n;
+        bool valid = p < limit;
         k[r] = valid ? kin[p] : 0ull;
         v[r] = valid ? (vin ? vin[p] : (uint32_t)p) : 0u;
     }
@@ -65,7 +77,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         size_t p = wave_base + (size_t)r * 64 + lane;
-        bool valid = p < n;
+        bool valid = p < limit;
         uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
         uint64_t peers = ballot64(valid);
 #pragma unroll
@@ -94,7 +106,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
         tile_count += c;
     }
     uint32_t *lb = lookback + (size_t)tile * 256;
-    if (tile == 0)
+    if (tile == first_tile)
         store_flag(&lb[b], kFlagInc | tile_count);
     else
         store_flag(&lb[b], kFlagAgg | tile_count);
@@ -109,10 +121,10 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
 
     // decoupled look-back over preceding tiles for bin b
     uint32_t excl = 0;
-    if (tile > 0) {
+    if (tile > first_tile) {
         int64_t t = (int64_t)tile - 1;
         uint32_t spins = 0;
-        while (t >= 0) {
+        while (t >= (int64_t)first_tile) {
             uint32_t x = load_flag(&lookback[(size_t)t * 256 + b]);
             if ((x >> 30) == 0) {
                 if (++spins > kSpinLimit) {
@@ -128,14 +140,14 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
         }
         store_flag(&lb[b], kFlagInc | ((excl + tile_count) & kValMask));
     }
-    s_global[b] = bin_base[b] + excl;
+    s_global[b] = bin_base[(size_t)cloud * kSegBaseStride + b] + excl;
     __syncthreads();
 
     // stage the tile in LDS in (bin, original order) order
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         size_t p = wave_base + (size_t)r * 64 + lane;
-        if (p < n) {
+        if (p < limit) {
             uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
             uint32_t pos = s_tile_start[d] + s_wave[w][d] + rank[r];
             s_keys[pos] = k[r];
@@ -143,7 +155,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
         }
     }
     __syncthreads();
-    const uint32_t tile_n = (uint32_t)((n - tile_base) < (size_t)kSortTile ? (n - tile_base) : (size_t)kSortTile);
+    const uint32_t tile_n = (uint32_t)((limit - tile_base) < (size_t)kSortTile ? (limit - tile_base) : (size_t)kSortTile);
     for (uint32_t j = tid; j < tile_n; j += kSortBlock) {
         uint64_t key = s_keys[j];
         uint32_t d = (uint32_t)(key >> shift) & 255u;
@@ -201,9 +213,9 @@ hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     for (int i = 0; i < np; i++) {
         int p = passes[i];
         const uint32_t *vin = (i == 0) ? nullptr : ctx->vals[src].p;
-        k_onesweep<<<(unsigned)tiles, kSortBlock, 0, ctx->stream>>>(ctx->keys[src].p, vin, ctx->keys[src ^ 1].p,
-                                                                    ctx->vals[src ^ 1].p, n, 8u * p, bases + p * 256,
-                                                                    ctx->lookback.p + (size_t)i * tiles * 256, counters + i, errflag);
+        k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, ctx->stream>>>(
+            ctx->keys[src].p, vin, ctx->keys[src ^ 1].p, ctx->vals[src ^ 1].p, n, 8u * p, bases + p * 256,
+            ctx->lookback.p + (size_t)i * tiles * 256, counters + i, errflag, nullptr);
         src ^= 1;
     }
     if (np == 0) {
@@ -226,6 +238,21 @@ hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     return hipSuccess;
 }
 
+hipError_t launch_sort_segmented(hipStream_t s, uint64_t *keys0, uint64_t *keys1, uint32_t *vals0, uint32_t *vals1,
+                                 size_t n, const SortTile *tiles, size_t ntiles, const uint32_t *bases,
+                                 uint32_t *lookback, uint32_t *counters, uint32_t *err) {
+    if (n == 0 || ntiles == 0) return hipSuccess;
+    uint64_t *kb[2] = {keys0, keys1};
+    uint32_t *vb[2] = {vals0, vals1};
+    for (int p = 0; p < kRadixPassCount; p++) {
+        const int src = p & 1;
+        k_onesweep<true><<<(unsigned)ntiles, kSortBlock, 0, s>>>(kb[src], p == 0 ? nullptr : vb[src], kb[src ^ 1], vb[src ^ 1], n,
+                                                                8u * p, bases + p * 256, lookback + (size_t)p * ntiles * 256,
+                                                                counters + p, err, tiles);
+    }
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ gather of the sorted cloud
 __global__ void k_gather(const double *__restrict__ xyz, const uint32_t *__restrict__ perm, double4 *__restrict__ out,
                          size_t n) {
@@ -233,6 +260,14 @@ __global__ void k_gather(const double *__restrict__ xyz, const uint32_t *__restr
         size_t s = perm[i];
         out[i] = make_double4(xyz[3 * s], xyz[3 * s + 1], xyz[3 * s + 2], 0.0);
     }
+}
+
+hipError_t launch_gather_raw(hipStream_t s, const double *xyz, const uint32_t *perm, double4 *out, size_t n) {
+    if (!n) return hipSuccess;
+    int nb = (int)((n + 255) / 256);
+    nb = nb > 8192 ? 8192 : nb;
+    k_gather<<<nb, 256, 0, s>>>(xyz, perm, out, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_gather(oct_ctx *ctx) {

@@ -27,6 +27,7 @@ EXPORTS = (
     "oct_set_tuning", "oct_autotune_radius", "oct_tuned_radius", "oct_radius_box", "oct_encode_coords",
     "oct_decode_keys", "oct_batch_create", "oct_batch_destroy", "oct_batch_last_error", "oct_batch_add",
     "oct_batch_build", "oct_batch_info", "oct_batch_stage_times", "oct_batch_radius", "oct_batch_knn",
+    "oct_batch_build_device", "oct_batch_set_points", "oct_batch_cloud_sorted", "oct_batch_cloud_tree",
 )
 
 
@@ -91,6 +92,10 @@ def lib():
         L.oct_batch_last_error.restype = C.c_char_p
         L.oct_batch_add.argtypes = [_vp, _vp, C.c_size_t, C.c_size_t, C.c_int, C.c_uint32, C.POINTER(C.c_int)]
         L.oct_batch_build.argtypes = [_vp]
+        L.oct_batch_build_device.argtypes = [_vp, C.c_int, C.POINTER(C.c_double)]
+        L.oct_batch_set_points.argtypes = [_vp, C.c_int, _vp, C.c_size_t]
+        L.oct_batch_cloud_sorted.argtypes = [_vp, C.c_int, _vp, _vp]
+        L.oct_batch_cloud_tree.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
         L.oct_batch_stage_times.argtypes = [_vp, _vp]
         L.oct_batch_info.argtypes = [_vp, C.c_int, C.POINTER(C.c_size_t), C.POINTER(TreeInfo)]
         L.oct_batch_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_size_t, _vp, C.POINTER(SearchStats)]
@@ -202,6 +207,34 @@ class Batch:
 
     def build(self):
         self._chk(self.L.oct_batch_build(self.h), "oct_batch_build")
+
+    def build_device(self, graph=True):
+        """Every cloud built at once on the device (one HIP graph when `graph`); returns device ms."""
+        ms = C.c_double(0.0)
+        self._chk(self.L.oct_batch_build_device(self.h, 1 if graph else 0, C.byref(ms)), "oct_batch_build_device")
+        return ms.value
+
+    def set_points(self, cloud, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        self._chk(self.L.oct_batch_set_points(self.h, cloud, _ptr(xyz), 24), "oct_batch_set_points")
+
+    def cloud_sorted(self, cloud):
+        n, _ = self.info(cloud)
+        keys = np.empty(n, np.uint64)
+        perm = np.empty(n, np.uint32)
+        self._chk(self.L.oct_batch_cloud_sorted(self.h, cloud, _ptr(keys), _ptr(perm)), "oct_batch_cloud_sorted")
+        return keys, perm
+
+    def cloud_tree(self, cloud):
+        nt = self.info(cloud)[1]["n_total"]
+        pre = np.empty(nt, np.uint64)
+        off = np.empty(nt, np.uint32)
+        rng = np.empty((nt, 2), np.uint64)
+        cen = np.empty((nt, 3), np.float64)
+        rad = np.empty((22, 3), np.float64)
+        self._chk(self.L.oct_batch_cloud_tree(self.h, cloud, _ptr(pre), _ptr(off), _ptr(rng), _ptr(cen), _ptr(rad)),
+                  "oct_batch_cloud_tree")
+        return {"prefixes": pre, "offsets": off, "ranges": rng, "centers": cen, "radii": rad}
 
     def stage_times(self):
         ms = np.zeros(8)

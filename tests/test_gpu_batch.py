@@ -8,6 +8,7 @@ import pytest
 
 import octree_amd as A
 import pyoracle as O
+from batch_clouds import make_clouds
 
 pytestmark = pytest.mark.gpu
 
@@ -15,23 +16,6 @@ pytestmark = pytest.mark.gpu
 def say(*a):
     print(*a, flush=True)
     sys.stdout.flush()
-
-
-def make_clouds(seed=21, count=14):
-    rng = np.random.default_rng(seed)
-    sizes = [1, 7, 130] + [int(v) for v in np.exp(rng.uniform(np.log(1e3), np.log(2.5e5), count - 3))]
-    clouds = []
-    for i, n in enumerate(sizes):
-        off = rng.uniform(-1e5, 1e5, 3)
-        if i % 2:
-            xyz = A.uniform_box(n, seed=100 + i, size=(60.0, 60.0, 15.0), offset=tuple(off))
-        else:
-            ext = max(20.0, 200.0 * np.sqrt(n / 2.5e5))
-            xyz = A.lidar_tile(n, seed=100 + i, extent=ext, offset=(off[0], off[1]))
-        enc = A.ENC_HILBERT if i % 3 else A.ENC_MORTON
-        ml = (32, 128, 16)[i % 3]
-        clouds.append((xyz, enc, ml))
-    return clouds
 
 
 @pytest.fixture(scope="module")
