@@ -147,13 +147,16 @@ def batch_workload(args):
     """Batched workload: 256 independent clouds (sizes log-uniform in [1e4, 4e6], rescaled to ~200 M
     points; half uniform boxes at 5 pts/m^3, half terrain + building tiles at 50 pts/m^2; each at a
     random origin in [-1e5, 1e5]), 50 k random centres per cloud, radius per query uniform in
-    [0.5, 5] m (float), 20 % of the queries kNN with k uniform in {8..256}. One JSON line of its own."""
+    [0.5, 5] m (float), 20 % of the queries kNN with k uniform in {8..256}. The whole sequence
+    (per-cloud boxes, keys, segmented sort, forest build, query grouping, work-queue count and kNN
+    kernels) runs as one HIP graph; the same clouds built one by one on per-cloud contexts and
+    searched by the per-call batch API check it. One JSON line of its own."""
     rng = np.random.default_rng(4)
     C = 256
     sizes = np.exp(rng.uniform(np.log(1e4), np.log(4e6), C))
     sizes = np.maximum(1e4, sizes * (200e6 / sizes.sum())).astype(np.int64)
     t0 = time.time()
-    b = A.Batch(0)   # built on the device in one HIP graph; the searches run on this forest
+    b = A.Batch(0)   # built and searched on the device in one HIP graph
     bp = A.Batch(0)  # the same clouds built one by one (per-cloud contexts), for comparison
     clouds = []
     for i, n in enumerate(sizes):
@@ -170,21 +173,6 @@ def batch_workload(args):
     gen_s = time.time() - t0
     npts = int(sizes.sum())
     log(f"[batch] {C} clouds, {npts} points generated and uploaded in {gen_s:.1f} s")
-    t0 = time.time()
-    first_ms = b.build_device(graph=True)
-    first_s = time.time() - t0
-    replay_ms, replay_wall = [], []
-    for _ in range(max(1, args.steps)):
-        t = time.perf_counter()
-        replay_ms.append(b.build_device(graph=True))
-        replay_wall.append(time.perf_counter() - t)
-    rep = float(np.median(replay_ms))
-    log(f"[batch] device build (bbox, keys, segmented sort, forest) in one graph: first call {first_s:.2f} s wall "
-        f"({first_ms:.1f} ms device), replays {rep:.2f} ms device, {float(np.median(replay_wall)) * 1e3:.1f} ms wall")
-    t0 = time.time()
-    bp.build()
-    build_s = time.time() - t0
-    log(f"[batch] per-cloud build of every cloud in {build_s:.2f} s")
     per = 50_000
     cloud = np.repeat(np.arange(C, dtype=np.uint32), per)
     idx = (rng.uniform(0.0, 1.0, C * per) * np.repeat(sizes, per)).astype(np.uint32)
@@ -192,40 +180,67 @@ def batch_workload(args):
     radii = rng.uniform(0.5, 5.0, C * per).astype(np.float32).astype(np.float64)
     kk = rng.integers(8, 257, C * per).astype(np.uint32)
     rq, kq = np.nonzero(~is_knn)[0], np.nonzero(is_knn)[0]
-    out = {"workload": f"{C} clouds, {int(sizes.sum())} points (log-uniform 1e4..4e6), {per} centres each, radius "
+    steps = max(1, args.steps)
+
+    # build only (bbox, keys, segmented sort, forest), one graph
+    first_ms = b.build_device(graph=True)
+    bms = [b.build_device(graph=True) for _ in range(steps)]
+    build_ms = float(np.median(bms))
+    log(f"[batch] device build graph: first {first_ms:.1f} ms, replays {build_ms:.2f} ms device")
+    # the whole pass, one graph: build + query grouping + work-queue count and kNN kernels
+    b.queries(cloud[rq], idx[rq], radii[rq], cloud[kq], idx[kq], kk[kq], kmax=256)
+    lanes, tune_ms = b.autotune_knn(A.PREC_FP32)
+    log(f"[batch] autotuned kNN lanes per query by k class (16, 32, ... 1024): {lanes}")
+    out = {"workload": f"{C} clouds, {npts} points (log-uniform 1e4..4e6), {per} centres each, radius "
                        f"U[0.5, 5] m, 20 % kNN with k U{{8..256}}, Hilbert, maxLeaf {args.max_leaf}",
            "clouds": C, "points": npts, "radius_queries": int(len(rq)), "knn_queries": int(len(kq)),
-           "build_device_graph": {"replay_device_ms": rep, "replay_wall_ms": float(np.median(replay_wall)) * 1e3,
-                                  "points_per_s": npts / (rep * 1e-3), "first_call_wall_s": first_s,
-                                  "replays": len(replay_ms)},
-           "build_per_cloud_s": build_s}
+           "build_graph": {"replay_device_ms": build_ms, "points_per_s": npts / (build_ms * 1e-3)},
+           "knn_lanes_by_class": lanes}
+    res = {}
     for name, prec in (("fp32", A.PREC_FP32), ("fp64", A.PREC_FP64)):
-        b.radius(A.K_SPHERE, A.A_STRUCT, prec, cloud[rq], idx[rq], radii[rq])
+        b.pipeline(A.K_SPHERE, A.A_STRUCT, prec, graph=True)
+        pm, pw = [], []
+        for _ in range(steps):
+            t = time.perf_counter()
+            pm.append(b.pipeline(A.K_SPHERE, A.A_STRUCT, prec, graph=True))
+            pw.append(time.perf_counter() - t)
+        rep = float(np.median(pm))
+        res[name] = b.pipeline_results()
+        out[f"pipeline_graph_{name}"] = {"replay_device_ms": rep, "replay_wall_ms": float(np.median(pw)) * 1e3,
+                                         "search_ms": rep - build_ms,
+                                         "queries_per_s": (len(rq) + len(kq)) / (rep * 1e-3),
+                                         "neighbors": int(res[name][0].astype(np.int64).sum())}
+        log(f"[batch] {name}: whole pass in one graph {rep:.1f} ms device ({float(np.median(pw)) * 1e3:.1f} ms wall): "
+            f"build {build_ms:.1f} + grouping and searches {rep - build_ms:.1f}")
+    # the per-call batch API on the same forest: one launch per kind (kNN: per k class), host grouping
+    for name, prec in (("fp32", A.PREC_FP32),):
         t = time.perf_counter()
         counts, _ = b.radius(A.K_SPHERE, A.A_STRUCT, prec, cloud[rq], idx[rq], radii[rq])
         wall = time.perf_counter() - t
         dev = b.stage_times()["search"]
-        out[f"radius_{name}"] = {"device_ms": dev, "wall_s": wall, "queries_per_s_device": len(rq) / (dev * 1e-3),
-                                 "neighbors": int(counts.astype(np.int64).sum())}
         t = time.perf_counter()
         ki, kd, st = b.knn(prec, cloud[kq], idx[kq], kk[kq], 256)
         kwall = time.perf_counter() - t
-        out[f"knn_{name}"] = {"device_ms": st["kernel_ms"], "wall_s": kwall,
-                              "queries_per_s_device": len(kq) / max(st["kernel_ms"] * 1e-3, 1e-12)}
-        log(f"[batch] {name}: radius {dev:.1f} ms device ({wall:.2f} s wall), knn {st['kernel_ms']:.1f} ms device")
-        out[f"_{name}"] = (counts, ki, kd)
-    # the device-built forest against the per-cloud-built one: same counts, same kNN sets (a sample)
+        out[f"per_call_{name}"] = {"radius_device_ms": dev, "radius_wall_s": wall, "knn_device_ms": st["kernel_ms"],
+                                   "knn_wall_s": kwall}
+        log(f"[batch] per-call {name}: radius {dev:.1f} ms device ({wall:.2f} s wall), knn {st['kernel_ms']:.1f} ms "
+            f"device ({kwall:.2f} s wall)")
+        same_per_call = bool(np.array_equal(counts, res["fp32"][0])) and bool(np.array_equal(ki, res["fp32"][1]))
+    # the same clouds built one by one
+    t0 = time.time()
+    bp.build()
+    out["build_per_cloud_s"] = time.time() - t0
     cp, _ = bp.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, cloud[rq], idx[rq], radii[rq])
     ksub = kq[:100_000]
     pki, pkd, _ = bp.knn(A.PREC_FP32, cloud[ksub], idx[ksub], kk[ksub], 256)
-    same_as_per_cloud = bool(np.array_equal(cp, out["_fp32"][0])) and bool(np.array_equal(pki, out["_fp32"][1][:len(ksub)]))
+    same_per_cloud = bool(np.array_equal(cp, res["fp32"][0])) and bool(np.array_equal(pki, res["fp32"][1][:len(ksub)]))
     bp.close()
+    log(f"[batch] per-cloud build {out['build_per_cloud_s']:.2f} s")
     # exact against the oracle on the first three clouds (each on its own)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
-    ok = bool(np.array_equal(out["_fp32"][0], out["_fp64"][0]))
-    ok &= bool(np.array_equal(out["_fp32"][1], out["_fp64"][1]))
+    ok = True
     for c in range(3):
         xyz = clouds[c]
         sx, keys, perm, cc, rr = O.sort_points(A.ENC_HILBERT, xyz)
@@ -234,17 +249,16 @@ def batch_workload(args):
         for q in rsel:
             qq = rq[q]
             ref = O.radius_count(tree, sx, A.K_SPHERE, A.A_STRUCT, sx[[int(idx[qq])]], float(radii[qq]))[0]
-            ok &= int(out["_fp32"][0][q]) == int(ref)
+            ok &= int(res["fp32"][0][q]) == int(ref)
         ksel = np.nonzero(cloud[kq] == c)[0][:100]
         bi, bd = O.knn_brute(sx, sx[idx[kq[ksel]].astype(np.int64)], 256)
         for row, q in enumerate(ksel):
             m = int(kk[kq[q]])
-            ok &= bool(np.array_equal(out["_fp32"][1][q, :m].astype(np.int64), bi[row, :m]))
-    out["verify"] = {"fp32_eq_fp64": bool(np.array_equal(out["_fp32"][0], out["_fp64"][0])),
-                     "device_build_eq_per_cloud_build": same_as_per_cloud,
-                     "oracle_sample_three_clouds": ok}
-    for name in ("_fp32", "_fp64"):
-        del out[name]
+            ok &= bool(np.array_equal(res["fp32"][1][q, :m].astype(np.int64), bi[row, :m]))
+    out["verify"] = {"fp32_eq_fp64": bool(np.array_equal(res["fp32"][0], res["fp64"][0]))
+                     and bool(np.array_equal(res["fp32"][1], res["fp64"][1])),
+                     "pipeline_eq_per_call": same_per_call, "pipeline_eq_per_cloud_build": same_per_cloud,
+                     "oracle_sample_three_clouds": bool(ok)}
     print(json.dumps(out), flush=True)
     b.close()
 

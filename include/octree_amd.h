@@ -247,6 +247,25 @@ int oct_batch_set_points(oct_batch *batch, int cloud, const double *xyz, size_t 
 int oct_batch_cloud_sorted(oct_batch *batch, int cloud, uint64_t *keys, uint32_t *perm);
 int oct_batch_cloud_tree(oct_batch *batch, int cloud, uint64_t *prefixes, uint32_t *offsets, uint64_t *ranges,
                          double *centers, double *radii);
+/* The search pass of a batch built on the device. oct_batch_queries copies a query set to the device
+ * (radius queries: cloud, centre index in that cloud's sorted order, radius; kNN queries: cloud,
+ * centre index, k in 1 .. kmax). oct_batch_pipeline then runs the whole sequence: the device build
+ * of oct_batch_build_device, the grouping of the queries on the device (sorted by cloud and centre,
+ * cut into wave tiles per cloud: 64 radius queries, 64 / lanes kNN queries of one k size class), the
+ * counts of a count-mode radius search (kernel, algo NEIGHBORS / PRUNE / STRUCT, precision) and the
+ * exact kNN search, each by one work-queue kernel whose resident waves take tiles from a device
+ * counter. With use_graph != 0 the sequence is captured in a HIP graph once per shape (kernel,
+ * algorithm, precision, lanes) and replayed; *ms (may be NULL) receives its device time.
+ * oct_batch_pipeline_results copies the last pass's counts (nr) and kNN rows (nk x kmax, indices in
+ * the query's cloud, -1 / NaN past min(n, k)); any pointer may be NULL.
+ * oct_batch_autotune_knn times every lanes-per-query shape of each k size class present on the
+ * batch's own kNN queries and keeps the fastest (lanes[c] for class c: k <= 16 << c; ms[3c + o]: the
+ * time of option o, -1 when not run); the next pipeline run uses them. */
+int oct_batch_queries(oct_batch *batch, const uint32_t *r_cloud, const uint32_t *r_idx, const double *r_radius, size_t nr,
+                      const uint32_t *k_cloud, const uint32_t *k_idx, const uint32_t *k_k, size_t nk, uint32_t kmax);
+int oct_batch_pipeline(oct_batch *batch, int kernel, int algo, int precision, int use_graph, double *ms);
+int oct_batch_pipeline_results(oct_batch *batch, uint32_t *counts, int32_t *idx, double *d2);
+int oct_batch_autotune_knn(oct_batch *batch, int precision, int lanes[7], double ms[21]);
 int oct_batch_info(const oct_batch *batch, int cloud, size_t *n, oct_tree_info *info);
 /* device milliseconds of the last batch searches, as oct_stage_times ([5] radius, [6] kNN: the last class) */
 int oct_batch_stage_times(const oct_batch *batch, double ms[8]);

@@ -58,6 +58,25 @@ struct oct_batch {
     hipGraphExec_t gexec = nullptr;
     uint32_t lv[oct::kLvCount] = {0};
     double build_ms = 0.0;
+    // search pass of the device pipeline: device-resident queries, grouped on the device each run
+    struct Kind {
+        uint32_t n = 0, nclass = 1;
+        oct::DevBuf<uint32_t> cloud, idx, kq, gidx, vals0, vals1, hist, bases, lookback, ctrl, gstart, toff;
+        oct::DevBuf<uint64_t> keys0, keys1;
+        oct::DevBuf<uint32_t> order[oct::kKnnClasses], tile_info[oct::kKnnClasses];
+        bool present[oct::kKnnClasses] = {};
+        oct::QueryGroups Q;
+    } rpass, kpass;
+    oct::DevBuf<double> p_rad;
+    oct::DevBuf<uint32_t> p_counts;
+    oct::DevBuf<int32_t> p_kidx;
+    oct::DevBuf<double> p_kd2;
+    uint32_t p_kmax = 0;
+    bool queries_set = false, results_ready = false;
+    int knn_lanes[oct::kKnnClasses] = {0};  // lanes per kNN query per size class (0: the class's default)
+    hipGraph_t pgraph = nullptr;
+    hipGraphExec_t pexec = nullptr;
+    int pkey[3 + oct::kKnnClasses] = {0};  // kernel, algo, precision, lanes: the captured pass's shape
     // host copy of the forest for per-cloud tree dumps (filled on first use after a build)
     bool dump_ready = false;
     std::vector<oct::NodeRec> h_nodes;
@@ -136,7 +155,16 @@ static int bfail(oct_batch *b, int code, const std::string &msg) {
 static void drop_graph(oct_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr, b->graph = nullptr;
+    if (b->pexec) (void)hipGraphExecDestroy(b->pexec);
+    if (b->pgraph) (void)hipGraphDestroy(b->pgraph);
+    b->gexec = nullptr, b->graph = nullptr, b->pexec = nullptr, b->pgraph = nullptr;
+}
+
+static void release_kind(oct_batch::Kind &k) {
+    k.cloud.release(), k.idx.release(), k.kq.release(), k.gidx.release(), k.vals0.release(), k.vals1.release();
+    k.hist.release(), k.bases.release(), k.lookback.release(), k.ctrl.release(), k.gstart.release(), k.toff.release();
+    k.keys0.release(), k.keys1.release();
+    for (int c = 0; c < kKnnClasses; c++) k.order[c].release(), k.tile_info[c].release();
 }
 
 extern "C" {
@@ -158,8 +186,8 @@ int oct_batch_destroy(oct_batch *b) {
     if (!b) return OCT_ERR_ARG;
     for (auto &c : b->clouds)
         if (c.ctx) oct_destroy(c.ctx);
-    hipSetDevice(b->device);
-    hipStreamSynchronize(b->forest->stream);
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->forest->stream);
     drop_graph(b);
     b->xyz_all.release();
     b->radtab.release(), b->magtab.release(), b->roots.release(), b->tile_info.release(), b->order.release();
@@ -167,6 +195,8 @@ int oct_batch_destroy(oct_batch *b) {
     b->chunks.release(), b->stiles.release(), b->cloud_chunk0.release(), b->cloud_off.release(), b->cloud_enc.release();
     b->cloud_leaf.release(), b->geom.release(), b->partial.release(), b->hist.release(), b->bases.release();
     b->ctrl.release(), b->part.release(), b->cl_internal.release(), b->cl_depth.release(), b->ncloud.release();
+    release_kind(b->rpass), release_kind(b->kpass);
+    b->p_rad.release(), b->p_counts.release(), b->p_kidx.release(), b->p_kd2.release();
     oct_destroy(b->forest);
     delete b;
     return OCT_OK;
@@ -400,43 +430,120 @@ static ForestBuild forest_args(oct_batch *b) {
     return F;
 }
 
-int oct_batch_build_device(oct_batch *b, int use_graph, double *ms_out) {
-    if (!b) return OCT_ERR_ARG;
-    if (b->mode == 1) return bfail(b, OCT_ERR_STATE, "batch already built per cloud");
-    if (b->clouds.empty()) return bfail(b, OCT_ERR_STATE, "empty batch");
-    BCHK(hipSetDevice(b->device), "set device");
+// lanes per query of kNN class c (the autotuned or set value, else the class's default)
+static int class_lanes(const oct_batch *b, int c) {
+    int opts[3];
+    knn_lane_options(16u << c, opts);
+    return b->knn_lanes[c] > 0 ? b->knn_lanes[c] : opts[0];
+}
+
+// grouping shape of the kNN queries for the current lanes per class
+static void knn_shape(oct_batch *b) {
+    const uint32_t C = (uint32_t)b->clouds.size();
+    for (int c = 0; c < kKnnClasses; c++) {
+        b->kpass.Q.G[c] = 64u / (uint32_t)class_lanes(b, c);
+        b->kpass.Q.slots[c] = group_slots(b->kpass.n, C, b->kpass.Q.G[c]);
+    }
+}
+
+// the search pass: radius queries grouped and counted (work-queue count kernel), kNN queries grouped
+// and searched per size class (work-queue kNN kernel), kNN indices made cloud-local. only_class >= 0:
+// just that kNN class (autotuning), timed with ev0 / ev1 when given.
+static hipError_t enqueue_search(oct_batch *b, int kernel, int algo, int precision, std::string &err, int only_class = -1,
+                                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+    oct_ctx *f = b->forest;
+    hipStream_t s = f->stream;
+    const uint32_t C = (uint32_t)b->clouds.size();
+    hipError_t e;
+    if (b->rpass.n && only_class < 0) {
+        if ((e = enqueue_query_groups(s, b->rpass.Q, C, b->cloud_off.p)) != hipSuccess) return e;
+        RadiusLaunch L;
+        L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = 64, L.mx = false, L.remap = true;
+        L.order = b->rpass.Q.order[0], L.qidx = b->rpass.Q.gidx, L.qrad = b->p_rad.p, L.nq = b->rpass.Q.slots[0];
+        L.counts = b->p_counts.p, L.tile_info = b->rpass.Q.tile_info[0], L.roots = b->roots.p, L.radtab = b->radtab.p;
+        L.queue_tiles = b->rpass.Q.ctrl + kQCtrlTiles, L.queue_next = b->rpass.Q.ctrl + kQCtrlQueue;
+        if ((e = launch_radius(f, L)) != hipSuccess) return e;
+    }
+    if (b->kpass.n) {
+        if ((e = enqueue_query_groups(s, b->kpass.Q, C, b->cloud_off.p)) != hipSuccess) return e;
+        for (int c = 0; c < kKnnClasses; c++) {
+            if (!b->kpass.present[c] || (only_class >= 0 && c != only_class)) continue;
+            KnnLaunch L;
+            L.precision = precision;
+            L.order = b->kpass.Q.order[c], L.qidx = b->kpass.Q.gidx, L.nq = b->kpass.Q.slots[c], L.k = b->p_kmax, L.kq = b->kpass.Q.kq;
+            L.kclass = 16u << c, L.lanes = class_lanes(b, c);
+            L.idx = b->p_kidx.p, L.d2 = b->p_kd2.p;
+            L.tile_info = b->kpass.Q.tile_info[c], L.roots = b->roots.p, L.radtab = b->radtab.p, L.magtab = b->magtab.p;
+            L.queue_tiles = b->kpass.Q.ctrl + kQCtrlTiles + c, L.queue_next = b->kpass.Q.ctrl + kQCtrlQueue + c;
+            if (ev0) (void)hipEventRecord(ev0, s);
+            if ((e = launch_knn(f, L, err)) != hipSuccess) return e;
+            if (ev1) (void)hipEventRecord(ev1, s);
+        }
+        if (only_class < 0 && (e = enqueue_knn_local(s, b->p_kidx.p, b->kpass.n, b->p_kmax, b->kpass.Q.cloud, b->cloud_off.p)) != hipSuccess)
+            return e;
+    }
+    return hipGetLastError();
+}
+
+// Device build (and with `search`, the search pass) of every cloud: captured in a graph on first use
+// when use_graph, else launched directly; grows the node storage and runs again on overflow; fills
+// the per-cloud tree info.
+static int run_device(oct_batch *b, int use_graph, bool search, int kernel, int algo, int precision, double *ms_out) {
     oct_ctx *f = b->forest;
     hipStream_t s = f->stream;
     BCHK(hipStreamSynchronize(s), "points");
     const uint32_t C = (uint32_t)b->clouds.size();
-    std::vector<uint32_t> ctrl(kCtrlWords), cint(C), cdep(C);
+    std::vector<uint32_t> ctrl(kCtrlWords), rctrl(kCtrlWords), kctrl(kCtrlWords), cint(C), cdep(C);
+    int key[3 + kKnnClasses] = {kernel, algo, precision};
+    for (int c = 0; c < kKnnClasses; c++) key[3 + c] = class_lanes(b, c);
+    if (search && b->pexec && !std::equal(key, key + 3 + kKnnClasses, b->pkey)) {
+        (void)hipGraphExecDestroy(b->pexec), (void)hipGraphDestroy(b->pgraph);
+        b->pexec = nullptr, b->pgraph = nullptr;
+    }
+    f->stage_events_off = true;
+    struct Restore {
+        oct_ctx *f;
+        ~Restore() { f->stage_events_off = false; }
+    } restore{f};
     for (int attempt = 0;; attempt++) {
         int r = device_setup(b);
         if (r) return r;
-        if (use_graph && !b->gexec) {
+        hipGraph_t *gp = search ? &b->pgraph : &b->graph;
+        hipGraphExec_t *xp = search ? &b->pexec : &b->gexec;
+        std::string err;
+        if (use_graph && !*xp) {
             BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "begin capture");
             hipError_t e = enqueue_forest_build(s, forest_args(b));
+            if (e == hipSuccess && search) e = enqueue_search(b, kernel, algo, precision, err);
             hipGraph_t g = nullptr;
             hipError_t e2 = hipStreamEndCapture(s, &g);
             if (e != hipSuccess || e2 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
-                return bfail(b, OCT_ERR_HIP, std::string("capture the build: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+                if (!err.empty()) return bfail(b, OCT_ERR_ARG, err);
+                return bfail(b, OCT_ERR_HIP, std::string("capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
             }
-            b->graph = g;
-            BCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0), "instantiate the build graph");
+            *gp = g;
+            BCHK(hipGraphInstantiate(xp, *gp, nullptr, nullptr, 0), "instantiate the graph");
+            if (search) std::copy(key, key + 3 + kKnnClasses, b->pkey);
         }
         BCHK(hipEventRecord(f->ev0[ST_BUILD], s), "event");
         if (use_graph) {
-            BCHK(hipGraphLaunch(b->gexec, s), "launch the build graph");
+            BCHK(hipGraphLaunch(*xp, s), "launch the graph");
         } else {
-            BCHK(enqueue_forest_build(s, forest_args(b)), "build");
+            hipError_t e = enqueue_forest_build(s, forest_args(b));
+            if (e == hipSuccess && search) e = enqueue_search(b, kernel, algo, precision, err);
+            if (!err.empty()) return bfail(b, OCT_ERR_ARG, err);
+            BCHK(e, "device build");
         }
         BCHK(hipEventRecord(f->ev1[ST_BUILD], s), "event");
         BCHK(hipMemcpyAsync(ctrl.data(), b->ctrl.p, kCtrlWords * 4, hipMemcpyDeviceToHost, s), "copy status");
+        if (search && b->rpass.n) BCHK(hipMemcpyAsync(rctrl.data(), b->rpass.ctrl.p, kCtrlWords * 4, hipMemcpyDeviceToHost, s), "copy status");
+        if (search && b->kpass.n) BCHK(hipMemcpyAsync(kctrl.data(), b->kpass.ctrl.p, kCtrlWords * 4, hipMemcpyDeviceToHost, s), "copy status");
         BCHK(hipMemcpyAsync(cint.data(), b->cl_internal.p, C * 4, hipMemcpyDeviceToHost, s), "copy status");
         BCHK(hipMemcpyAsync(cdep.data(), b->cl_depth.p, C * 4, hipMemcpyDeviceToHost, s), "copy status");
-        BCHK(hipStreamSynchronize(s), "build");
-        if (ctrl[kCtrlSortErr]) return bfail(b, OCT_ERR_STATE, "sort: look-back wait exceeded its bound");
+        BCHK(hipStreamSynchronize(s), "device build");
+        if (ctrl[kCtrlSortErr] || rctrl[kCtrlSortErr] || kctrl[kCtrlSortErr])
+            return bfail(b, OCT_ERR_STATE, "sort: look-back wait exceeded its bound");
         if (ctrl[kCtrlBuildErr] & kBuildErrDepth) return bfail(b, OCT_ERR_STATE, "build: split below the maximum depth");
         if (ctrl[kCtrlBuildErr] & kBuildErrCapacity) {
             if (attempt >= 8 || b->cap >= 0x7fffffffu) return bfail(b, OCT_ERR_NOMEM, "build: node capacity");
@@ -448,7 +555,6 @@ int oct_batch_build_device(oct_batch *b, int use_graph, double *ms_out) {
     }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, f->ev0[ST_BUILD], f->ev1[ST_BUILD]);
-    f->ev_used[ST_BUILD] = true;
     b->build_ms = ms;
     if (ms_out) *ms_out = ms;
     for (uint32_t l = 0; l < kLvCount; l++) b->lv[l] = ctrl[kCtrlLv + l];
@@ -473,7 +579,179 @@ int oct_batch_build_device(oct_batch *b, int use_graph, double *ms_out) {
     b->built = true;
     b->mode = 2;
     b->dump_ready = false;
+    b->results_ready = search;
     return OCT_OK;
+}
+
+int oct_batch_build_device(oct_batch *b, int use_graph, double *ms_out) {
+    if (!b) return OCT_ERR_ARG;
+    if (b->mode == 1) return bfail(b, OCT_ERR_STATE, "batch already built per cloud");
+    if (b->clouds.empty()) return bfail(b, OCT_ERR_STATE, "empty batch");
+    BCHK(hipSetDevice(b->device), "set device");
+    return run_device(b, use_graph, false, 0, 0, 0, ms_out);
+}
+
+// device copies of one query kind and its grouping buffers
+static int setup_kind(oct_batch *b, oct_batch::Kind &K, const uint32_t *cloud, const uint32_t *idx, const uint32_t *k, size_t n) {
+    hipStream_t s = b->forest->stream;
+    const uint32_t C = (uint32_t)b->clouds.size();
+    K.n = (uint32_t)n;
+    K.nclass = k ? kKnnClasses : 1;
+    const size_t n1 = n + 1, tiles = (n + kSortTile - 1) / kSortTile;
+    BCHK(K.cloud.reserve(n1), "alloc queries");
+    BCHK(K.idx.reserve(n1), "alloc queries");
+    BCHK(K.gidx.reserve(n1), "alloc queries");
+    BCHK(K.vals0.reserve(n1), "alloc queries");
+    BCHK(K.vals1.reserve(n1), "alloc queries");
+    BCHK(K.keys0.reserve(n1), "alloc queries");
+    BCHK(K.keys1.reserve(n1), "alloc queries");
+    BCHK(K.hist.reserve(kRadixPassCount * 256), "alloc queries");
+    BCHK(K.bases.reserve(kRadixPassCount * 256), "alloc queries");
+    BCHK(K.lookback.reserve(5 * tiles * 256 + 1), "alloc queries");
+    BCHK(K.ctrl.reserve(kCtrlWords), "alloc queries");
+    BCHK(K.gstart.reserve((size_t)K.nclass * C + 1), "alloc queries");
+    BCHK(K.toff.reserve((size_t)K.nclass * C + 1), "alloc queries");
+    if (k) BCHK(K.kq.reserve(n1), "alloc queries");
+    for (uint32_t c = 0; c < K.nclass; c++) {
+        BCHK(K.order[c].reserve((size_t)group_slots((uint32_t)n, C, 64) + 64), "alloc queries");
+        BCHK(K.tile_info[c].reserve(n + C + 1), "alloc queries");
+    }
+    if (n) {
+        BCHK(hipMemcpyAsync(K.cloud.p, cloud, n * 4, hipMemcpyHostToDevice, s), "copy queries");
+        BCHK(hipMemcpyAsync(K.idx.p, idx, n * 4, hipMemcpyHostToDevice, s), "copy queries");
+        if (k) BCHK(hipMemcpyAsync(K.kq.p, k, n * 4, hipMemcpyHostToDevice, s), "copy queries");
+    }
+    for (int c = 0; c < kKnnClasses; c++) K.present[c] = false;
+    if (k)
+        for (size_t i = 0; i < n; i++) K.present[knn_class_of(k[i])] = true;
+    QueryGroups &Q = K.Q;
+    Q.n = K.n, Q.nclass = K.nclass;
+    Q.cloud = K.cloud.p, Q.idx = K.idx.p, Q.kq = k ? K.kq.p : nullptr;
+    Q.keys[0] = K.keys0.p, Q.keys[1] = K.keys1.p, Q.vals[0] = K.vals0.p, Q.vals[1] = K.vals1.p;
+    Q.gidx = K.gidx.p, Q.hist = K.hist.p, Q.bases = K.bases.p, Q.lookback = K.lookback.p, Q.ctrl = K.ctrl.p;
+    Q.gstart = K.gstart.p, Q.toff = K.toff.p;
+    for (uint32_t c = 0; c < (uint32_t)kKnnClasses; c++) {
+        Q.order[c] = c < K.nclass ? K.order[c].p : nullptr;
+        Q.tile_info[c] = c < K.nclass ? K.tile_info[c].p : nullptr;
+        Q.G[c] = 64;
+        Q.slots[c] = c < K.nclass ? group_slots((uint32_t)n, C, 64) : 0u;
+    }
+    return OCT_OK;
+}
+
+int oct_batch_queries(oct_batch *b, const uint32_t *r_cloud, const uint32_t *r_idx, const double *r_radius, size_t nr,
+                      const uint32_t *k_cloud, const uint32_t *k_idx, const uint32_t *k_k, size_t nk, uint32_t kmax) {
+    if (!b) return OCT_ERR_ARG;
+    if (b->mode == 1) return bfail(b, OCT_ERR_STATE, "the search pass runs on batches built on the device");
+    if (b->clouds.empty()) return bfail(b, OCT_ERR_STATE, "empty batch");
+    if (nr >= (1ull << 30) || nk >= (1ull << 30)) return bfail(b, OCT_ERR_ARG, "at most 2^30 - 1 queries of each kind");
+    if (nr && (!r_cloud || !r_idx || !r_radius)) return bfail(b, OCT_ERR_ARG, "null radius query arrays");
+    if (nk && (!k_cloud || !k_idx || !k_k)) return bfail(b, OCT_ERR_ARG, "null kNN query arrays");
+    if (nk && (kmax == 0 || kmax > 1024)) return bfail(b, OCT_ERR_ARG, "kmax must be 1 .. 1024");
+    if (nk && nk * (size_t)kmax >= (1ull << 40)) return bfail(b, OCT_ERR_ARG, "kNN output too large");
+    const size_t C = b->clouds.size();
+    for (size_t i = 0; i < nr; i++) {
+        if (r_cloud[i] >= C) return bfail(b, OCT_ERR_ARG, "cloud id out of range");
+        if (r_idx[i] >= b->clouds[r_cloud[i]].n) return bfail(b, OCT_ERR_ARG, "center index out of range");
+    }
+    for (size_t i = 0; i < nk; i++) {
+        if (k_cloud[i] >= C) return bfail(b, OCT_ERR_ARG, "cloud id out of range");
+        if (k_idx[i] >= b->clouds[k_cloud[i]].n) return bfail(b, OCT_ERR_ARG, "center index out of range");
+        if (k_k[i] == 0 || k_k[i] > kmax) return bfail(b, OCT_ERR_ARG, "per-query k must be 1 .. kmax");
+    }
+    BCHK(hipSetDevice(b->device), "set device");
+    hipStream_t s = b->forest->stream;
+    BCHK(hipStreamSynchronize(s), "queries");
+    drop_graph(b);  // query buffers move
+    int r = setup_kind(b, b->rpass, r_cloud, r_idx, nullptr, nr);
+    if (r) return r;
+    r = setup_kind(b, b->kpass, k_cloud, k_idx, k_k, nk);
+    if (r) return r;
+    knn_shape(b);
+    BCHK(b->p_rad.reserve(nr + 1), "alloc queries");
+    BCHK(b->p_counts.reserve(nr + 1), "alloc results");
+    BCHK(b->p_kidx.reserve(nk * (size_t)kmax + 1), "alloc results");
+    BCHK(b->p_kd2.reserve(nk * (size_t)kmax + 1), "alloc results");
+    if (nr) BCHK(hipMemcpyAsync(b->p_rad.p, r_radius, nr * 8, hipMemcpyHostToDevice, s), "copy queries");
+    BCHK(hipStreamSynchronize(s), "copy queries");
+    b->p_kmax = nk ? kmax : 0;
+    b->queries_set = true;
+    b->results_ready = false;
+    return OCT_OK;
+}
+
+int oct_batch_pipeline(oct_batch *b, int kernel, int algo, int precision, int use_graph, double *ms) {
+    if (!b) return OCT_ERR_ARG;
+    if (!b->queries_set) return bfail(b, OCT_ERR_STATE, "oct_batch_pipeline before oct_batch_queries");
+    if (b->mode == 1) return bfail(b, OCT_ERR_STATE, "batch already built per cloud");
+    if (kernel < 0 || kernel > 3 || algo < 1 || algo > 3 || (precision != 0 && precision != 1))
+        return bfail(b, OCT_ERR_ARG, "unknown kernel, algorithm or precision");
+    BCHK(hipSetDevice(b->device), "set device");
+    return run_device(b, use_graph, true, kernel, algo, precision, ms);
+}
+
+int oct_batch_pipeline_results(oct_batch *b, uint32_t *counts, int32_t *idx, double *d2) {
+    if (!b) return OCT_ERR_ARG;
+    if (!b->results_ready) return bfail(b, OCT_ERR_STATE, "no search pass results");
+    BCHK(hipSetDevice(b->device), "set device");
+    hipStream_t s = b->forest->stream;
+    if (counts && b->rpass.n) BCHK(hipMemcpyAsync(counts, b->p_counts.p, (size_t)b->rpass.n * 4, hipMemcpyDeviceToHost, s), "copy counts");
+    const size_t ke = (size_t)b->kpass.n * b->p_kmax;
+    if (idx && ke) BCHK(hipMemcpyAsync(idx, b->p_kidx.p, ke * 4, hipMemcpyDeviceToHost, s), "copy kNN");
+    if (d2 && ke) BCHK(hipMemcpyAsync(d2, b->p_kd2.p, ke * 8, hipMemcpyDeviceToHost, s), "copy kNN");
+    BCHK(hipStreamSynchronize(s), "copy results");
+    return OCT_OK;
+}
+
+// Per kNN size class present in the queries: every lanes-per-query shape offered for it is timed on
+// the batch's own queries of that class (the forest as last built), the fastest kept.
+int oct_batch_autotune_knn(oct_batch *b, int precision, int lanes_out[7], double ms_out[21]) {
+    if (!b) return OCT_ERR_ARG;
+    if (!b->queries_set || !b->built || b->mode != 2) return bfail(b, OCT_ERR_STATE, "autotune needs queries and a device-built batch");
+    if (precision != 0 && precision != 1) return bfail(b, OCT_ERR_ARG, "unknown precision");
+    BCHK(hipSetDevice(b->device), "set device");
+    oct_ctx *f = b->forest;
+    hipStream_t s = f->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BCHK(hipEventCreate(&e0), "event");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return bfail(b, OCT_ERR_HIP, "event");
+    }
+    f->stage_events_off = true;
+    int rc = OCT_OK;
+    for (int c = 0; c < kKnnClasses && rc == OCT_OK; c++) {
+        if (ms_out)
+            for (int o = 0; o < 3; o++) ms_out[3 * c + o] = -1.0;
+        if (!b->kpass.present[c]) continue;
+        int opts[3];
+        const int no = knn_lane_options(16u << c, opts);
+        int best = opts[0];
+        float best_ms = 0.f;
+        for (int o = 0; o < no; o++) {
+            b->knn_lanes[c] = opts[o];
+            knn_shape(b);
+            std::string err;
+            hipError_t e = enqueue_search(b, 0, 1, precision, err, c, e0, e1);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                rc = bfail(b, err.empty() ? OCT_ERR_HIP : OCT_ERR_ARG, err.empty() ? std::string("autotune: ") + hipGetErrorString(e) : err);
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms_out) ms_out[3 * c + o] = ms;
+            if (o == 0 || ms < best_ms) best = opts[o], best_ms = ms;
+        }
+        b->knn_lanes[c] = best;
+    }
+    f->stage_events_off = false;
+    knn_shape(b);
+    (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+    if (lanes_out)
+        for (int c = 0; c < kKnnClasses; c++) lanes_out[c] = class_lanes(b, c);
+    b->results_ready = false;  // the timed runs wrote partial results
+    return rc;
 }
 
 // sorted keys and permutation (indices into the cloud's points as added) of one cloud
@@ -585,7 +863,7 @@ static int batch_stats(oct_batch *b, oct_search_stats *stats, Stage st) {
     stats->nodes_classified = h[0], stats->points_tested = h[1], stats->rechecked = h[2], stats->total_neighbors = h[3];
     stats->class_slots = h[4], stats->test_slots = h[5];
     float ms = 0.f;
-    hipEventElapsedTime(&ms, f->ev0[st], f->ev1[st]);
+    (void)hipEventElapsedTime(&ms, f->ev0[st], f->ev1[st]);
     stats->kernel_ms = ms;
     return OCT_OK;
 }
@@ -689,7 +967,7 @@ int oct_batch_knn(oct_batch *b, int precision, const uint32_t *cloud, const uint
         if (e != hipSuccess) return err.empty() ? bfail(b, OCT_ERR_HIP, "batch knn") : bfail(b, OCT_ERR_ARG, err);
         BCHK(hipStreamSynchronize(f->stream), "batch knn");  // order / tile staging is reused by the next class
         float ms = 0.f;
-        hipEventElapsedTime(&ms, f->ev0[ST_KNN], f->ev1[ST_KNN]);
+        (void)hipEventElapsedTime(&ms, f->ev0[ST_KNN], f->ev1[ST_KNN]);
         total_ms += ms;
     }
     BCHK(hipMemcpyAsync(idx_out, b->kidx.p, nq * kmax * 4, hipMemcpyDeviceToHost, f->stream), "copy idx");

@@ -11,8 +11,11 @@
 using namespace oct;
 
 namespace oct {
-void stage_begin(oct_ctx *ctx, Stage s) { (void)hipEventRecord(ctx->ev0[s], ctx->stream); }
+void stage_begin(oct_ctx *ctx, Stage s) {
+    if (!ctx->stage_events_off) (void)hipEventRecord(ctx->ev0[s], ctx->stream);
+}
 void stage_end(oct_ctx *ctx, Stage s) {
+    if (ctx->stage_events_off) return;
     (void)hipEventRecord(ctx->ev1[s], ctx->stream);
     ctx->ev_used[s] = true;
 }

@@ -148,6 +148,7 @@ struct oct_ctx {
     int knn_lanes_tuned[2][8];      // autotuned lanes per query per (precision, size class), -1 = not tuned
 
     hipEvent_t ev0[oct::ST_N] = {}, ev1[oct::ST_N] = {};
+    bool stage_events_off = false;  // launch sequences captured in graphs record no stage events
     bool ev_used[oct::ST_N] = {};
 };
 
@@ -174,6 +175,10 @@ struct RadiusLaunch {
     const uint32_t *tile_info = nullptr;
     const uint32_t *roots = nullptr;
     const double *radtab = nullptr;
+    // work-queue form (count mode over a forest, 64-slot tiles, vector ALUs, no stats): device-resident
+    // tile count and a zeroed tile counter; the grid is the resident waves, not the tiles
+    const uint32_t *queue_tiles = nullptr;
+    uint32_t *queue_next = nullptr;
     const uint32_t *order = nullptr;
     const uint32_t *qidx = nullptr;
     const double *qxyz = nullptr;
@@ -215,6 +220,9 @@ struct KnnLaunch {
     const double *magtab = nullptr;
     const uint32_t *kq = nullptr;
     uint32_t kclass = 0;  // the largest k of the launch when kq is set (0: k)
+    // work-queue form: device-resident wave-tile count and a zeroed tile counter
+    const uint32_t *queue_tiles = nullptr;
+    uint32_t *queue_next = nullptr;
 };
 hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err);
 // lanes-per-query shapes offered for k (default first); returns how many
@@ -225,6 +233,12 @@ int knn_lane_options(uint32_t k, int out[3]);
 hipError_t launch_sort_segmented(hipStream_t s, uint64_t *keys0, uint64_t *keys1, uint32_t *vals0, uint32_t *vals1,
                                  size_t n, const SortTile *tiles, size_t ntiles, const uint32_t *bases,
                                  uint32_t *lookback, uint32_t *counters, uint32_t *err);
+// Stable sort of n (key, value) pairs by the low 8 * npasses key bits; values of the first pass are the
+// positions; result in keys0 / vals0 for an even npasses, keys1 / vals1 for an odd one. bases: the
+// exclusive bin offsets of every pass (pass * 256 + digit); lookback (npasses x tiles x 256) and
+// counters (npasses) must be zero.
+hipError_t launch_sort_passes(hipStream_t s, uint64_t *keys0, uint64_t *keys1, uint32_t *vals0, uint32_t *vals1, size_t n,
+                              int npasses, const uint32_t *bases, uint32_t *lookback, uint32_t *counters, uint32_t *err);
 // out[i] = (xyz[3 perm[i]], xyz[3 perm[i] + 1], xyz[3 perm[i] + 2], 0)
 hipError_t launch_gather_raw(hipStream_t s, const double *xyz, const uint32_t *perm, double4 *out, size_t n);
 hipError_t exclusive_scan_u32_to_u64(oct_ctx *ctx, const uint32_t *in, uint64_t *out, size_t n);

@@ -51,6 +51,9 @@ struct KnnArgs {
     const uint32_t *__restrict__ tile_info;
     const uint32_t *__restrict__ roots;
     const uint32_t *__restrict__ kq;  // per-query k (null: k for every query); k is the row width
+    // work-queue form: device-resident wave-tile count and the zeroed counter waves take tiles from
+    const uint32_t *__restrict__ queue_tiles;
+    uint32_t *__restrict__ queue_next;
 };
 
 __device__ inline bool knn_less(double da, uint32_t ia, double db, uint32_t ib) { return da < db || (da == db && ia < ib); }
@@ -62,8 +65,9 @@ __device__ inline double gap2(double q, double c, double r, double dl) {
     return gap > 0.0 ? gap * gap : 0.0;
 }
 
+// One wave tile (64 / S queries) of a kNN search, run by one wave (a 64-thread block).
 template <int S, int KC, int PREC>
-__global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
+__device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a, const uint32_t tile) {
     constexpr int G = 64 / S;                           // queries per wave
     constexpr int CAP = 2 * KC;                         // buffer entries per query
     constexpr int U = S >= 16 ? 1 : 16 / S;             // points per lane per step
@@ -81,11 +85,11 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
 
     const uint32_t lane = threadIdx.x;
     const uint32_t qs = lane / S, g = lane % S;
-    const uint32_t slot0 = blockIdx.x * G;
+    const uint32_t slot0 = tile * G;
     const uint32_t slot = slot0 + qs;
     uint32_t cloud = 0, nvt = G, root = 0;
     if (a.tile_info) {
-        const uint32_t ti = uniform_u32(a.tile_info[blockIdx.x]);
+        const uint32_t ti = uniform_u32(a.tile_info[tile]);
         cloud = ti & 0xfffffu, nvt = ti >> 20;
         root = uniform_u32(a.roots[cloud]);
     }
@@ -338,8 +342,47 @@ __global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
     }
 }
 
+template <int S, int KC, int PREC>
+__global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
+    knn_tile<S, KC, PREC>(a, blockIdx.x);
+}
+
+// Work-queue form (batches): a grid of resident waves; each takes the next wave tile from a device
+// counter (queue_next) until the device-resident tile count (queue_tiles) is reached. Every wave
+// leaves once the counter has passed the count.
+template <int S, int KC, int PREC>
+__global__ __launch_bounds__(64) void k_knn_queue(const KnnArgs a) {
+    const uint32_t ntiles = uniform_u32(*a.queue_tiles);
+    for (;;) {
+        uint32_t t = 0;
+        if (lane_id() == 0) t = atomicAdd(a.queue_next, 1u);
+        t = readlane_u32(t, 0);
+        if (t >= ntiles) return;
+        knn_tile<S, KC, PREC>(a, t);
+    }
+}
+
+template <int S, int KC, int PREC>
+static unsigned knn_queue_grid() {
+    static unsigned cached = 0;
+    if (cached) return cached;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_knn_queue<S, KC, PREC>, 64, 0);
+    cached = (unsigned)((cus > 0 ? cus : 256) * (per > 0 ? per : 1));
+    return cached;
+}
+
 template <int S, int KC>
 static void launch_sk(const KnnArgs &a, int precision, hipStream_t s) {
+    if (a.queue_next) {
+        if (precision == OCT_PREC_FP32_RECHECK)
+            k_knn_queue<S, KC, OCT_PREC_FP32_RECHECK><<<knn_queue_grid<S, KC, OCT_PREC_FP32_RECHECK>(), 64, 0, s>>>(a);
+        else
+            k_knn_queue<S, KC, OCT_PREC_FP64><<<knn_queue_grid<S, KC, OCT_PREC_FP64>(), 64, 0, s>>>(a);
+        return;
+    }
     const unsigned nb = (unsigned)((a.nq + (64 / S) - 1) / (64 / S));
     if (precision == OCT_PREC_FP32_RECHECK)
         k_knn<S, KC, OCT_PREC_FP32_RECHECK><<<nb, 64, 0, s>>>(a);
@@ -417,6 +460,8 @@ hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err) {
     a.tile_info = L.tile_info;
     a.roots = L.roots;
     a.kq = L.kq;
+    a.queue_tiles = L.queue_tiles;
+    a.queue_next = L.queue_next;
     if (L.radtab) a.radtab = L.radtab, a.magtab = L.magtab;
     const uint32_t kcls = L.kq && L.kclass ? L.kclass : k;  // selection size class of the launch
     int opts[3];

@@ -368,6 +368,127 @@ __global__ __launch_bounds__(256) void k_bp_node_ext(const NodeRec *__restrict__
     }
 }
 
+// ------------------------------------------------------------------ query grouping
+// sort keys (class << 32 | global centre index), global centre indices, digit histograms
+__global__ __launch_bounds__(256) void k_q_keys(const uint32_t *__restrict__ cloud, const uint32_t *__restrict__ idx,
+                                                const uint32_t *__restrict__ kq, const uint32_t *__restrict__ cloud_off,
+                                                uint32_t n, uint64_t *__restrict__ keys, uint32_t *__restrict__ gidx,
+                                                uint32_t *__restrict__ hist) {
+    __shared__ uint32_t sh[5 * 256];
+    for (int i = threadIdx.x; i < 5 * 256; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    const int np = kq ? 5 : 4;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = cloud_off[cloud[i]] + idx[i];
+        const uint64_t key = (uint64_t)g | (kq ? (uint64_t)knn_class_of(kq[i]) << 32 : 0ull);
+        keys[i] = key;
+        gidx[i] = g;
+        for (int p = 0; p < np; p++) atomicAdd(&sh[p * 256 + ((key >> (8 * p)) & 255u)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < np * 256; i += blockDim.x)
+        if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
+// first sorted position of every (class, cloud) group; gstart[nclass * C] = n
+__global__ void k_q_group_starts(const uint64_t *__restrict__ skeys, uint32_t n, const uint32_t *__restrict__ cloud_off,
+                                 uint32_t C, uint32_t nclass, uint32_t *__restrict__ gstart) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g > nclass * C) return;
+    if (g == nclass * C) {
+        gstart[g] = n;
+        return;
+    }
+    const uint64_t v = ((uint64_t)(g / C) << 32) | cloud_off[g % C];
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (skeys[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    gstart[g] = lo;
+}
+
+struct ClassShape {
+    uint32_t G[kKnnClasses];
+    uint32_t slots[kKnnClasses];
+    uint32_t *order[kKnnClasses];
+    uint32_t *tile_info[kKnnClasses];
+};
+
+// one block per class: wave tiles per group, exclusive offsets over the class's groups, tile count
+__global__ __launch_bounds__(1024) void k_q_tile_offsets(const uint32_t *__restrict__ gstart, uint32_t C, ClassShape sh,
+                                                         uint32_t *__restrict__ toff, uint32_t *__restrict__ ntiles) {
+    __shared__ uint32_t s_w[16];
+    const uint32_t k = blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const uint32_t G = sh.G[k];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < C; base += 1024) {
+        const uint32_t c = base + t;
+        uint32_t nt = 0;
+        if (c < C) {
+            const uint32_t g = k * C + c;
+            nt = (gstart[g + 1] - gstart[g] + G - 1) / G;
+        }
+        uint32_t incl = nt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off);
+            if (lane >= (uint32_t)off) incl += o;
+        }
+        if (lane == 63) s_w[w] = incl;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (uint32_t j = 0; j < 16; j++) {
+            if (j < w) pre += s_w[j];
+            tot += s_w[j];
+        }
+        if (c < C) toff[k * C + c] = carry + pre + incl - nt;
+        carry += tot;
+        __syncthreads();
+    }
+    if (t == 0) ntiles[k] = carry;
+}
+
+// slot s of class blockIdx.y: its tile t = s / G, the group holding t (last group whose offset is
+// <= t), the query at that group's position; tile_info written by the tile's first slot
+__global__ __launch_bounds__(256) void k_q_tiles(const uint32_t *__restrict__ svals, const uint32_t *__restrict__ gstart,
+                                                 const uint32_t *__restrict__ toff, const uint32_t *__restrict__ ntiles,
+                                                 uint32_t C, ClassShape sh) {
+    const uint32_t k = blockIdx.y;
+    const uint32_t G = sh.G[k];
+    const uint32_t nt = ntiles[k];
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < sh.slots[k]; s += gridDim.x * blockDim.x) {
+        const uint32_t t = s / G, j = s % G;
+        if (t >= nt) break;
+        const uint32_t *to = toff + (size_t)k * C;
+        uint32_t lo = 0, hi = C;  // first group offset > t
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (to[mid] <= t)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const uint32_t c = lo - 1, g = k * C + c;
+        const uint32_t start = gstart[g] + (t - to[c]) * G;
+        const uint32_t left = gstart[g + 1] - start;
+        const uint32_t nvalid = left < G ? left : G;
+        sh.order[k][s] = j < nvalid ? svals[start + j] : 0u;
+        if (j == 0) sh.tile_info[k][t] = c | (nvalid << 20);
+    }
+}
+
+__global__ void k_knn_local(int32_t *__restrict__ idx, size_t rows, uint32_t kmax, const uint32_t *__restrict__ cloud,
+                            const uint32_t *__restrict__ cloud_off) {
+    const size_t total = rows * kmax;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int32_t v = idx[e];
+        if (v >= 0) idx[e] = v - (int32_t)cloud_off[cloud[e / kmax]];
+    }
+}
+
 static inline unsigned grid_for(size_t n, unsigned per, unsigned cap) {
     size_t b = (n + per - 1) / per;
     if (b < 1) b = 1;
@@ -403,6 +524,42 @@ hipError_t enqueue_forest_build(hipStream_t s, const ForestBuild &F) {
     k_bp_sentinel<<<1, 64, 0, s>>>(lv, F.child);
     k_bp_recentre<<<4096, 256, 0, s>>>(F.p64, F.nodes, F.child, lv, F.p32, F.amax);
     k_bp_node_ext<<<2048, 256, 0, s>>>(F.nodes, F.nkey, F.child, F.amax, lv, F.ncloud, F.geom, F.ext);
+    return hipGetLastError();
+}
+
+hipError_t enqueue_query_groups(hipStream_t s, const QueryGroups &Q, uint32_t C, const uint32_t *cloud_off) {
+    k_fill_u32<<<1, 256, 0, s>>>(Q.ctrl, kCtrlWords, 0u);
+    k_fill_u32<<<1, 256, 0, s>>>(Q.hist, (size_t)kRadixPassCount * 256, 0u);
+    if (Q.n == 0) return hipGetLastError();  // tile counts stay 0: the work queues end at once
+    const int np = Q.kq ? 5 : 4;
+    const size_t tiles = (Q.n + kSortTile - 1) / kSortTile;
+    const size_t lbw = (size_t)np * tiles * 256;
+    k_fill_u32<<<grid_for(lbw, 256, 8192), 256, 0, s>>>(Q.lookback, lbw, 0u);
+    k_q_keys<<<grid_for(Q.n, 256, 2048), 256, 0, s>>>(Q.cloud, Q.idx, Q.kq, cloud_off, Q.n, Q.keys[0], Q.gidx, Q.hist);
+    k_bp_bases<<<1, 512, 0, s>>>(Q.hist, Q.ctrl + kQCtrlZero, Q.bases);
+    hipError_t e = launch_sort_passes(s, Q.keys[0], Q.keys[1], Q.vals[0], Q.vals[1], Q.n, np, Q.bases, Q.lookback,
+                                      Q.ctrl + kCtrlSortCounters, Q.ctrl + kCtrlSortErr);
+    if (e != hipSuccess) return e;
+    const uint64_t *sk = Q.keys[np & 1];
+    const uint32_t *sv = Q.vals[np & 1];
+    const uint32_t ng = Q.nclass * C + 1;
+    k_q_group_starts<<<grid_for(ng, 256, 1u << 20), 256, 0, s>>>(sk, Q.n, cloud_off, C, Q.nclass, Q.gstart);
+    ClassShape sh;
+    uint32_t maxslots = 0;
+    for (int k = 0; k < kKnnClasses; k++) {
+        sh.G[k] = Q.G[k], sh.slots[k] = (uint32_t)k < Q.nclass ? Q.slots[k] : 0u;
+        sh.order[k] = Q.order[k], sh.tile_info[k] = Q.tile_info[k];
+        maxslots = sh.slots[k] > maxslots ? sh.slots[k] : maxslots;
+    }
+    k_q_tile_offsets<<<Q.nclass, 1024, 0, s>>>(Q.gstart, C, sh, Q.toff, Q.ctrl + kQCtrlTiles);
+    k_q_tiles<<<dim3(grid_for(maxslots, 256, 4096), Q.nclass), 256, 0, s>>>(sv, Q.gstart, Q.toff, Q.ctrl + kQCtrlTiles, C, sh);
+    return hipGetLastError();
+}
+
+hipError_t enqueue_knn_local(hipStream_t s, int32_t *idx, size_t rows, uint32_t kmax, const uint32_t *cloud,
+                             const uint32_t *cloud_off) {
+    if (rows == 0) return hipSuccess;
+    k_knn_local<<<grid_for(rows * kmax, 256, 8192), 256, 0, s>>>(idx, rows, kmax, cloud, cloud_off);
     return hipGetLastError();
 }
 

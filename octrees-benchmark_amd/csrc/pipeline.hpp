@@ -70,4 +70,45 @@ struct ForestBuild {
 
 hipError_t enqueue_forest_build(hipStream_t s, const ForestBuild &F);
 
+// ---- queries of a batch search pass, grouped on the device
+constexpr int kKnnClasses = 7;  // kNN size classes: k <= 16, 32, ..., 1024
+// control words of one query kind: sort tile counters [0, 8), sort error [8], per-class wave-tile
+// counts [16, 24), per-class work-queue counters [32, 40), a zero word [48]
+constexpr uint32_t kQCtrlTiles = 16, kQCtrlQueue = 32, kQCtrlZero = 48;
+
+// One query kind: radius queries (one class, kq null) or kNN queries (classes by k). Every run sorts
+// the queries by (class, cloud, centre index), finds each (class, cloud) group's range, cuts the
+// groups into wave tiles of G[class] slots and writes, per class, the slot -> query table (order)
+// and the tile -> (cloud | valid slots << 20) table that the search kernels' work queues read.
+struct QueryGroups {
+    uint32_t n = 0;       // queries
+    uint32_t nclass = 1;  // 1 (radius) or kKnnClasses
+    const uint32_t *cloud = nullptr, *idx = nullptr, *kq = nullptr;
+    uint64_t *keys[2] = {nullptr, nullptr};  // n each
+    uint32_t *vals[2] = {nullptr, nullptr};  // n each
+    uint32_t *gidx = nullptr;      // per query: its centre's index in the concatenated sorted clouds
+    uint32_t *hist = nullptr;      // kRadixPassCount x 256
+    uint32_t *bases = nullptr;     // kRadixPassCount x 256
+    uint32_t *lookback = nullptr;  // 5 x tiles x 256
+    uint32_t *ctrl = nullptr;      // kCtrlWords
+    uint32_t *gstart = nullptr;    // nclass x C + 1
+    uint32_t *toff = nullptr;      // nclass x C
+    uint32_t G[kKnnClasses] = {64, 64, 64, 64, 64, 64, 64};  // slots per wave tile
+    uint32_t slots[kKnnClasses] = {0};                      // slot capacity per class
+    uint32_t *order[kKnnClasses] = {nullptr};
+    uint32_t *tile_info[kKnnClasses] = {nullptr};
+};
+// the slot capacity a class needs whatever the queries: (ceil(n / G) + C) tiles of G slots
+inline uint32_t group_slots(uint32_t n, uint32_t C, uint32_t G) { return ((n + G - 1) / G + C) * G; }
+hipError_t enqueue_query_groups(hipStream_t s, const QueryGroups &Q, uint32_t C, const uint32_t *cloud_off);
+// kNN rows of forest indices -> indices in each query's own cloud (entries < 0 stay)
+hipError_t enqueue_knn_local(hipStream_t s, int32_t *idx, size_t rows, uint32_t kmax, const uint32_t *cloud,
+                             const uint32_t *cloud_off);
+// k -> size class (16 << class >= k)
+__host__ __device__ inline uint32_t knn_class_of(uint32_t k) {
+    uint32_t c = 0;
+    while (c + 1 < (uint32_t)kKnnClasses && (16u << c) < k) c++;
+    return c;
+}
+
 }  // namespace oct

@@ -334,8 +334,9 @@ constexpr int count_waves() {
     if (MX || ALGO == OCT_ALGO_NEIGHBORS || !(KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE)) return 2;
     return PREC == OCT_PREC_FP32_RECHECK ? 8 : 6;
 }
+// One wave tile (G queries) of a count-mode search, run by one wave.
 template <int KERNEL, int ALGO, int PREC, int G, bool STATS, bool MX>
-__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(count_waves<KERNEL, ALGO, PREC, MX>()))) void k_count(const SearchArgs a) {
+__device__ __attribute__((always_inline)) inline void count_tile(const SearchArgs &a, const uint32_t tile) {
     constexpr int S = 64 / G;
     constexpr bool kFp32 = PREC == OCT_PREC_FP32_RECHECK && (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE);
     static_assert(!MX || (kFp32 && KERNEL == OCT_KERNEL_SPHERE), "matrix-core leaf tests are FP32 sphere only");
@@ -357,7 +358,6 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t g = lane / G, j = lane % G;
     const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
-    const uint32_t tile = blockIdx.x * kWaves + w;
     if (uniform_u32(tile * G) >= a.nq) return;
     const uint32_t slot = tile * G + j;
     // one cloud (root 0), or the wave tile's cloud of a forest and its number of valid slots
@@ -892,6 +892,27 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
     }
 }
 
+template <int KERNEL, int ALGO, int PREC, int G, bool STATS, bool MX>
+__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(count_waves<KERNEL, ALGO, PREC, MX>()))) void k_count(const SearchArgs a) {
+    count_tile<KERNEL, ALGO, PREC, G, STATS, MX>(a, blockIdx.x * kWaves + (threadIdx.x >> 6));
+}
+
+// Work-queue form (batches): a grid of resident waves; each wave takes the next wave tile from a
+// device counter (queue_next) until the device-resident tile count (queue_tiles) is reached, so the
+// launch shape does not depend on how many tiles the queries made. Every wave leaves once the
+// counter has passed the count.
+template <int KERNEL, int ALGO, int PREC>
+__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(count_waves<KERNEL, ALGO, PREC, false>()))) void k_count_queue(const SearchArgs a) {
+    const uint32_t ntiles = uniform_u32(*a.queue_tiles);
+    for (;;) {
+        uint32_t t = 0;
+        if (lane_id() == 0) t = atomicAdd(a.queue_next, 1u);
+        t = readlane_u32(t, 0);
+        if (t >= ntiles) return;
+        count_tile<KERNEL, ALGO, PREC, 64, false, false>(a, t);
+    }
+}
+
 template <int KERNEL, int ALGO, int PREC, bool STATS, bool MX>
 static void launch_count_s(const SearchArgs &a, int G, hipStream_t s) {
     const size_t tiles = ((size_t)a.nq + G - 1) / G;
@@ -904,8 +925,27 @@ static void launch_count_s(const SearchArgs &a, int G, hipStream_t s) {
         k_count<KERNEL, ALGO, PREC, 64, STATS, MX><<<nb, kSearchBlock, 0, s>>>(a);
 }
 
+// resident waves of the work-queue kernel: blocks per CU at its occupancy, times the CUs
+template <int KERNEL, int ALGO, int PREC>
+static unsigned queue_grid() {
+    static unsigned cached = 0;
+    if (cached) return cached;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_count_queue<KERNEL, ALGO, PREC>, kSearchBlock, 0);
+    cached = (unsigned)((cus > 0 ? cus : 256) * (per > 0 ? per : 1));
+    return cached;
+}
+
 template <int KERNEL, int ALGO, int PREC>
 static void launch_count(const SearchArgs &a, int G, bool mx, hipStream_t s) {
+    if constexpr (ALGO != OCT_ALGO_APPROX) {
+        if (a.queue_next) {  // work-queue form: 64 queries per wave tile, vector ALUs, no counters
+            k_count_queue<KERNEL, ALGO, PREC><<<queue_grid<KERNEL, ALGO, PREC>(), kSearchBlock, 0, s>>>(a);
+            return;
+        }
+    }
     if constexpr (ALGO == OCT_ALGO_APPROX) {  // one shape: 64 queries per wave, vector ALUs
         const unsigned nb = (unsigned)(((size_t)a.nq + 64 * kWaves - 1) / (64 * kWaves));
         if (a.stats)
@@ -984,6 +1024,8 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     a.radtab = L.radtab ? L.radtab : ctx->radtab.p;
     a.tile_info = L.tile_info;
     a.roots = L.roots;
+    a.queue_tiles = L.queue_tiles;
+    a.queue_next = L.queue_next;
     a.nq = (uint32_t)L.nq;
     a.counts = L.counts;
     a.ecounts = L.ecounts;

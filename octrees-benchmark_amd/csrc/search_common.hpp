@@ -38,6 +38,10 @@ struct SearchArgs {
     const double *__restrict__ radtab;
     const uint32_t *__restrict__ tile_info;
     const uint32_t *__restrict__ roots;
+    // work-queue form of the count kernel: number of wave tiles (device-resident) and the counter the
+    // waves take tiles from (zero at launch); null queue_next: one wave tile per wave of the grid
+    const uint32_t *__restrict__ queue_tiles;
+    uint32_t *__restrict__ queue_next;
     uint32_t nq;
     uint32_t *__restrict__ counts;
     uint32_t *__restrict__ ecounts;

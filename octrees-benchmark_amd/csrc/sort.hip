@@ -253,6 +253,21 @@ hipError_t launch_sort_segmented(hipStream_t s, uint64_t *keys0, uint64_t *keys1
     return hipGetLastError();
 }
 
+hipError_t launch_sort_passes(hipStream_t s, uint64_t *keys0, uint64_t *keys1, uint32_t *vals0, uint32_t *vals1, size_t n,
+                              int npasses, const uint32_t *bases, uint32_t *lookback, uint32_t *counters, uint32_t *err) {
+    if (n == 0) return hipSuccess;
+    const size_t tiles = (n + kSortTile - 1) / kSortTile;
+    uint64_t *kb[2] = {keys0, keys1};
+    uint32_t *vb[2] = {vals0, vals1};
+    for (int p = 0; p < npasses; p++) {
+        const int src = p & 1;
+        k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, s>>>(kb[src], p == 0 ? nullptr : vb[src], kb[src ^ 1], vb[src ^ 1], n,
+                                                                 8u * p, bases + p * 256, lookback + (size_t)p * tiles * 256,
+                                                                 counters + p, err, nullptr);
+    }
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ gather of the sorted cloud
 __global__ void k_gather(const double *__restrict__ xyz, const uint32_t *__restrict__ perm, double4 *__restrict__ out,
                          size_t n) {

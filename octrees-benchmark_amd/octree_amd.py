@@ -28,6 +28,7 @@ EXPORTS = (
     "oct_decode_keys", "oct_batch_create", "oct_batch_destroy", "oct_batch_last_error", "oct_batch_add",
     "oct_batch_build", "oct_batch_info", "oct_batch_stage_times", "oct_batch_radius", "oct_batch_knn",
     "oct_batch_build_device", "oct_batch_set_points", "oct_batch_cloud_sorted", "oct_batch_cloud_tree",
+    "oct_batch_queries", "oct_batch_pipeline", "oct_batch_pipeline_results", "oct_batch_autotune_knn",
 )
 
 
@@ -96,6 +97,10 @@ def lib():
         L.oct_batch_set_points.argtypes = [_vp, C.c_int, _vp, C.c_size_t]
         L.oct_batch_cloud_sorted.argtypes = [_vp, C.c_int, _vp, _vp]
         L.oct_batch_cloud_tree.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
+        L.oct_batch_queries.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp, _vp, C.c_size_t, C.c_uint32]
+        L.oct_batch_pipeline.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.oct_batch_pipeline_results.argtypes = [_vp, _vp, _vp, _vp]
+        L.oct_batch_autotune_knn.argtypes = [_vp, C.c_int, _vp, _vp]
         L.oct_batch_stage_times.argtypes = [_vp, _vp]
         L.oct_batch_info.argtypes = [_vp, C.c_int, C.POINTER(C.c_size_t), C.POINTER(TreeInfo)]
         L.oct_batch_radius.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_size_t, _vp, C.POINTER(SearchStats)]
@@ -246,6 +251,41 @@ class Batch:
         ti = TreeInfo()
         self._chk(self.L.oct_batch_info(self.h, cloud, C.byref(n), C.byref(ti)), "oct_batch_info")
         return n.value, {f: getattr(ti, f) for f, _ in ti._fields_}
+
+    def queries(self, r_cloud=(), r_idx=(), r_radius=(), k_cloud=(), k_idx=(), k_k=(), kmax=None):
+        """Device-resident query set of the search pass (oct_batch_queries)."""
+        rc_ = np.ascontiguousarray(r_cloud, np.uint32)
+        ri = np.ascontiguousarray(r_idx, np.uint32)
+        rr = np.ascontiguousarray(r_radius, np.float64)
+        kc = np.ascontiguousarray(k_cloud, np.uint32)
+        ki = np.ascontiguousarray(k_idx, np.uint32)
+        kk = np.ascontiguousarray(k_k, np.uint32)
+        kmax = (int(kk.max()) if len(kk) else 0) if kmax is None else kmax
+        self._nq = (len(rc_), len(kc), kmax)
+        self._chk(self.L.oct_batch_queries(self.h, _ptr(rc_), _ptr(ri), _ptr(rr), len(rc_), _ptr(kc), _ptr(ki), _ptr(kk),
+                                           len(kc), kmax), "oct_batch_queries")
+
+    def pipeline(self, kernel=0, algo=3, precision=1, graph=True):
+        """Device build + grouped searches (one HIP graph when `graph`); returns device ms."""
+        ms = C.c_double(0.0)
+        self._chk(self.L.oct_batch_pipeline(self.h, kernel, algo, precision, 1 if graph else 0, C.byref(ms)),
+                  "oct_batch_pipeline")
+        return ms.value
+
+    def pipeline_results(self, knn_d2=True):
+        nr, nk, kmax = self._nq
+        counts = np.empty(nr, np.uint32)
+        idx = np.empty((nk, kmax), np.int32)
+        d2 = np.empty((nk, kmax), np.float64) if knn_d2 else None
+        self._chk(self.L.oct_batch_pipeline_results(self.h, _ptr(counts), _ptr(idx), _ptr(d2) if knn_d2 else None),
+                  "oct_batch_pipeline_results")
+        return counts, idx, d2
+
+    def autotune_knn(self, precision=1):
+        lanes = np.zeros(7, np.int32)
+        ms = np.zeros(21)
+        self._chk(self.L.oct_batch_autotune_knn(self.h, precision, _ptr(lanes), _ptr(ms)), "oct_batch_autotune_knn")
+        return lanes.tolist(), ms.reshape(7, 3)
 
     def radius(self, kernel, algo, precision, cloud, idx, radii):
         cloud = np.ascontiguousarray(cloud, np.uint32)
