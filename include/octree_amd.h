@@ -274,6 +274,39 @@ int oct_batch_radius(oct_batch *batch, int kernel, int algo, int precision, cons
 int oct_batch_knn(oct_batch *batch, int precision, const uint32_t *cloud, const uint32_t *center_idx, const uint32_t *k,
                   size_t nq, uint32_t kmax, int32_t *idx_out, double *d2_out, oct_search_stats *stats);
 
+/* ---- LAS ingest (LasFileReaderParallel::read / readMeta, readers/las_file_reader_parallel.hpp:23-145;
+ * the reader handler, readers/handlers.hpp:57-72). Uncompressed LAS 1.0-1.4, point data formats 0-10,
+ * extra bytes allowed; LAZ is rejected (OCT_ERR_UNSUPPORTED). oct_las_read_header parses the public
+ * header block (n_points: the legacy count, or the LAS 1.4 64-bit count when the legacy one is 0).
+ * oct_las_load makes the file's points the context's cloud (as oct_set_points, input order):
+ * x = X * scale[0] + offset[0] in FP64 (:88-92), and decodes each point's attributes (:124-137,
+ * PointMetadata, point_metadata.hpp:9-105) on the device. oct_las_fetch copies the loaded cloud
+ * (n x 3) and / or the attributes (n records) to the host; either pointer may be NULL. */
+typedef struct oct_las_header {
+    uint8_t version_major, version_minor;
+    uint8_t point_format; /* 0 .. 10 */
+    uint8_t pad0;
+    uint16_t record_length; /* bytes per point record (extra bytes included) */
+    uint16_t header_size;
+    uint32_t offset_to_points;
+    uint32_t n_vlr;
+    uint64_t n_points;
+    double scale[3], offset[3];
+    double min[3], max[3]; /* as stored in the header */
+} oct_las_header;
+typedef struct oct_point_meta {
+    double intensity;
+    uint8_t return_number, number_of_returns, scan_direction, edge_of_flight_line;
+    uint8_t classification;
+    int8_t scan_angle_rank;
+    uint8_t user_data, pad0;
+    uint16_t point_source_id, pad1;
+    uint32_t r, g, b;
+} oct_point_meta;
+int oct_las_read_header(const char *path, oct_las_header *header);
+int oct_las_load(oct_ctx *ctx, const char *path, oct_las_header *header);
+int oct_las_fetch(oct_ctx *ctx, double *xyz, oct_point_meta *meta);
+
 /* ---- timing: device milliseconds of the stages of the last calls (HIP events on the context's
  * stream): [0] bbox, [1] encode, [2] sort, [3] gather, [4] build, [5] search, [6] knn. */
 int oct_stage_times(const oct_ctx *ctx, double ms[8]);

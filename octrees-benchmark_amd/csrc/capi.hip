@@ -61,6 +61,7 @@ __global__ void k_unstride(const double *__restrict__ src, size_t stride_d, doub
 int upload_xyz(oct_ctx *ctx, const double *src, size_t n, size_t stride, hipMemcpyKind kind) {
     if (stride < 24 || stride % 8) return fail(ctx, OCT_ERR_ARG, "stride_bytes must be >= 24 and a multiple of 8");
     if (n >= (1ull << 30)) return fail(ctx, OCT_ERR_ARG, "at most 2^30 - 1 points");
+    ctx->las_loaded = false;  // the cloud no longer comes from a LAS file
     HCHK(ctx->xyz.reserve(3 * n), "alloc points");
     if (n) {
         if (!src) return fail(ctx, OCT_ERR_ARG, "null point buffer");
@@ -171,6 +172,7 @@ int oct_destroy(oct_ctx *ctx) {
     ctx->qidx.release(), ctx->qxyz.release(), ctx->qrad.release(), ctx->counts.release(), ctx->qoff.release();
     ctx->entries.release(), ctx->stats.release(), ctx->kidx.release(), ctx->kd2.release();
     ctx->next.release(), ctx->qrad3.release(), ctx->kc_xyz.release(), ctx->kc_keys.release();
+    ctx->kd2f.release(), ctx->radtab.release(), ctx->las_rec.release(), ctx->las_meta.release();
     for (int s = 0; s < ST_N; s++) {
         if (ctx->ev0[s]) (void)hipEventDestroy(ctx->ev0[s]);
         if (ctx->ev1[s]) (void)hipEventDestroy(ctx->ev1[s]);

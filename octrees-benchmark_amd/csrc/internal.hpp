@@ -82,6 +82,11 @@ struct oct_ctx {
     size_t n = 0;
     oct::DevBuf<double> xyz;  // n x 3, input order
     bool has_points = false;
+    // LAS ingest: raw point records and the decoded per-point attributes of the last file loaded
+    oct::DevBuf<unsigned char> las_rec;
+    oct::DevBuf<oct_point_meta> las_meta;
+    size_t las_n = 0;
+    bool las_loaded = false;
 
     // encode + sort
     int encoder = 0;

@@ -1,18 +1,21 @@
 // Drives the drop-in facade the way the reference's own example and tests do (examples/example.cpp,
 // tests/structures/test_octree.cpp) and prints every result as text for comparison with the oracle.
-// Usage: facade_check <encoder 1|5> <max_leaf> <points.bin> <centers.bin> <radius>
-//   points.bin / centers.bin: raw little-endian FP64 x, y, z triples.
+// Usage: facade_check <encoder 1|5> <max_leaf> <points.bin | points.las> <centers.bin> <radius>
+//   points.bin / centers.bin: raw little-endian FP64 x, y, z triples; a .las file is read with
+//   readPoints (points and their attributes), and the attributes are printed after sortPoints.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iterator>
 #include <optional>
+#include <string>
 #include <vector>
 
 #include "encoding/hilbert_encoder_3d.hpp"
 #include "encoding/morton_encoder_3d.hpp"
 #include "encoding/point_encoder_factory.hpp"
+#include "readers/handlers.hpp"
 #include "structures/linear_octree.hpp"
 
 static std::vector<double> read_f64(const char *path) {
@@ -48,16 +51,31 @@ int main(int argc, char **argv) {
     if (argc < 6) return 2;
     const int enc_id = std::atoi(argv[1]);
     mainOptions.maxPointsLeaf = (size_t)std::atoi(argv[2]);
-    const std::vector<double> xyz = read_f64(argv[3]), cq = read_f64(argv[4]);
+    const std::vector<double> cq = read_f64(argv[4]);
     const double r = std::atof(argv[5]);
     PointsAoS points;
-    for (size_t i = 0; i < xyz.size() / 3; i++) points.push_back(Point(i, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+    std::optional<std::vector<PointMetadata>> metadata = std::nullopt;
+    const std::string src = argv[3];
+    if (src.size() > 4 && src.compare(src.size() - 4, 4, ".las") == 0) {
+        auto read = readPoints<PointsAoS>(src);
+        points = std::move(read.first);
+        metadata = std::move(read.second);
+    } else {
+        const std::vector<double> xyz = read_f64(argv[3]);
+        for (size_t i = 0; i < xyz.size() / 3; i++) points.push_back(Point(i, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+    }
     std::vector<Point> centers;
     for (size_t i = 0; i < cq.size() / 3; i++) centers.push_back(Point(cq[3 * i], cq[3 * i + 1], cq[3 * i + 2]));
 
-    std::optional<std::vector<PointMetadata>> metadata = std::nullopt;
     PointEncoding::PointEncoder &enc = PointEncoding::getEncoder((EncoderType)enc_id);
     auto [codes, box] = enc.sortPoints(points, metadata);
+    if (metadata)
+        for (size_t i = 0; i < metadata->size(); i++) {
+            const PointMetadata &m = (*metadata)[i];
+            std::printf("meta %zu %.17g %u %u %u %u %u %d %u %u %u %u %u\n", i, m.intensity, m.returnNumber, m.numberOfReturns,
+                        m.scanDirection, m.edgeOfFlightLine, m.classification, (int)m.scanAngleRank, m.userData,
+                        m.pointSourceId, m.r, m.g, m.b);
+        }
     std::printf("box %.17g %.17g %.17g %.17g %.17g %.17g\n", box.minX(), box.minY(), box.minZ(), box.maxX(), box.maxY(), box.maxZ());
     for (size_t i = 0; i < points.size(); i++) std::printf("pt %zu %zu %llu\n", i, points[i].id(), (unsigned long long)codes[i]);
 

@@ -29,7 +29,29 @@ EXPORTS = (
     "oct_batch_build", "oct_batch_info", "oct_batch_stage_times", "oct_batch_radius", "oct_batch_knn",
     "oct_batch_build_device", "oct_batch_set_points", "oct_batch_cloud_sorted", "oct_batch_cloud_tree",
     "oct_batch_queries", "oct_batch_pipeline", "oct_batch_pipeline_results", "oct_batch_autotune_knn",
+    "oct_las_read_header", "oct_las_load", "oct_las_fetch",
 )
+
+# oct_point_meta (per-point LAS attributes, PointMetadata)
+POINT_META = np.dtype([("intensity", "<f8"), ("return_number", "u1"), ("number_of_returns", "u1"),
+                       ("scan_direction", "u1"), ("edge_of_flight_line", "u1"), ("classification", "u1"),
+                       ("scan_angle_rank", "i1"), ("user_data", "u1"), ("pad0", "u1"), ("point_source_id", "<u2"),
+                       ("pad1", "<u2"), ("r", "<u4"), ("g", "<u4"), ("b", "<u4")])
+assert POINT_META.itemsize == 32
+
+
+class LasHeader(C.Structure):
+    _fields_ = [("version_major", C.c_uint8), ("version_minor", C.c_uint8), ("point_format", C.c_uint8),
+                ("pad0", C.c_uint8), ("record_length", C.c_uint16), ("header_size", C.c_uint16),
+                ("offset_to_points", C.c_uint32), ("n_vlr", C.c_uint32), ("n_points", C.c_uint64),
+                ("scale", C.c_double * 3), ("offset", C.c_double * 3), ("min", C.c_double * 3), ("max", C.c_double * 3)]
+
+    def as_dict(self):
+        d = {}
+        for f, _ in self._fields_:
+            v = getattr(self, f)
+            d[f] = list(v) if isinstance(v, C.Array) else v
+        return d
 
 
 class OctError(RuntimeError):
@@ -97,6 +119,9 @@ def lib():
         L.oct_batch_set_points.argtypes = [_vp, C.c_int, _vp, C.c_size_t]
         L.oct_batch_cloud_sorted.argtypes = [_vp, C.c_int, _vp, _vp]
         L.oct_batch_cloud_tree.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
+        L.oct_las_read_header.argtypes = [C.c_char_p, C.POINTER(LasHeader)]
+        L.oct_las_load.argtypes = [_vp, C.c_char_p, C.POINTER(LasHeader)]
+        L.oct_las_fetch.argtypes = [_vp, _vp, _vp]
         L.oct_batch_queries.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp, _vp, C.c_size_t, C.c_uint32]
         L.oct_batch_pipeline.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.oct_batch_pipeline_results.argtypes = [_vp, _vp, _vp, _vp]
@@ -137,6 +162,15 @@ def synth():
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def las_header(path):
+    """oct_las_read_header: the public header block of a LAS file (no context, no GPU)."""
+    h = LasHeader()
+    rc = lib().oct_las_read_header(os.fsencode(path), C.byref(h))
+    if rc != 0:
+        raise OctError(f"oct_las_read_header({path}) failed with status {rc}")
+    return h.as_dict()
 
 
 # ------------------------------------------------------------------ workloads
@@ -341,6 +375,21 @@ class Octree:
             raise OctError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
     # ---- H1-H3
+    def las_load(self, path):
+        """The LAS file's points become the context's cloud (decoded on the device); returns its header."""
+        h = LasHeader()
+        self._chk(self.L.oct_las_load(self.h, os.fsencode(path), C.byref(h)), "oct_las_load")
+        self.current = None
+        self.n = self._las_n = int(h.n_points)
+        return h.as_dict()
+
+    def las_fetch(self, meta=True):
+        n = self._las_n
+        xyz = np.empty((n, 3), np.float64)
+        m = np.empty(n, POINT_META) if meta else None
+        self._chk(self.L.oct_las_fetch(self.h, _ptr(xyz), _ptr(m) if meta else None), "oct_las_fetch")
+        return xyz, m
+
     def set_points(self, xyz):
         """xyz: (n, 3) packed points, or (n, 4) rows laid out like the reference's 32-byte Point."""
         xyz = np.ascontiguousarray(xyz, np.float64)

@@ -1036,3 +1036,122 @@ uint64_t orc_knn_brute(const double *xyz, size_t n, const double *centers, size_
     }
     return total;
 }
+
+/* ------------------------------------------------------------------ LAS ingest */
+
+#include <stdio.h>
+
+static uint64_t las_u(const unsigned char *p, int bytes) { /* little-endian unsigned field */
+    uint64_t v = 0;
+    for (int i = bytes - 1; i >= 0; --i) v = (v << 8) | p[i];
+    return v;
+}
+
+static double las_f64(const unsigned char *p) {
+    uint64_t bits = las_u(p, 8);
+    double d;
+    memcpy(&d, &bits, 8);
+    return d;
+}
+
+/* Public header block, LAS 1.4 R15 table 3 (1.2 and 1.3 are its first 227 and 235 bytes). LASlib's
+ * header.number_of_point_records is the legacy 32-bit count (offset 107); for a LAS 1.4 file whose
+ * legacy count is 0 the 64-bit count (offset 247) is used. */
+int orc_las_header(const char *path, uint64_t info[5], double so[6]) {
+    unsigned char h[375];
+    memset(h, 0, sizeof(h));
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    size_t got = fread(h, 1, sizeof(h), f);
+    fclose(f);
+    if (got < 227 || memcmp(h, "LASF", 4) != 0) return -1;
+    unsigned fmt = h[104];
+    if (fmt > 10) return -2; /* LAZ sets bits 6-7 */
+    uint64_t n = las_u(h + 107, 4);
+    if (h[25] >= 4 && n == 0 && got >= 255) n = las_u(h + 247, 8);
+    info[0] = h[25];
+    info[1] = fmt;
+    info[2] = las_u(h + 105, 2);
+    info[3] = n;
+    info[4] = las_u(h + 96, 4);
+    for (int a = 0; a < 3; ++a) {
+        so[a] = las_f64(h + 131 + 8 * a);
+        so[3 + a] = las_f64(h + 155 + 8 * a);
+    }
+    return 0;
+}
+
+/* Point records, LAS 1.4 R15 tables 7-17. Formats 0-5: intensity @12, return bits @14 (return number
+ * 0-2, number of returns 3-5, scan direction 6, edge 7), classification @15 (LASlib's 5-bit class),
+ * scan angle rank @16 (int8), user data @17, point source id @18, RGB @20 (format 2) or @28 (3, 5).
+ * Formats 6-10: intensity @12, return number (bits 0-3) and number of returns (4-7) @14, scan
+ * direction (bit 6) and edge (bit 7) @15, classification @16, user data @17, scan angle @18 (int16,
+ * 0.006 degree units), point source id @20, RGB @30 (7, 8, 10); LASlib fills its legacy fields from
+ * them (returns above 7 folded: number of returns 7, return number 7 when it is the last else 6;
+ * classification 0 when above 31), and the reference casts get_scan_angle() = 0.006f * angle to char. */
+int orc_las_read(const char *path, double *xyz, double *intensity, uint8_t *fields, uint16_t *psid, uint32_t *rgb) {
+    uint64_t info[5];
+    double so[6];
+    int r = orc_las_header(path, info, so);
+    if (r) return r;
+    const unsigned fmt = (unsigned)info[1], rl = (unsigned)info[2];
+    const uint64_t n = info[3];
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    if (fseek(f, (long)info[4], SEEK_SET) != 0) {
+        fclose(f);
+        return -1;
+    }
+    unsigned char *rec = (unsigned char *)malloc(rl ? rl : 1);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (fread(rec, 1, rl, f) != rl) {
+            free(rec);
+            fclose(f);
+            return -1;
+        }
+        const int32_t X = (int32_t)(uint32_t)las_u(rec, 4), Y = (int32_t)(uint32_t)las_u(rec + 4, 4);
+        const int32_t Z = (int32_t)(uint32_t)las_u(rec + 8, 4);
+        xyz[3 * i + 0] = X * so[0] + so[3];
+        xyz[3 * i + 1] = Y * so[1] + so[4];
+        xyz[3 * i + 2] = Z * so[2] + so[5];
+        intensity[i] = (double)las_u(rec + 12, 2);
+        uint8_t *fl = fields + 8 * i;
+        unsigned rgb_at = 0;
+        if (fmt < 6) {
+            const unsigned b = rec[14];
+            fl[0] = b & 7u;
+            fl[1] = (b >> 3) & 7u;
+            fl[2] = (b >> 6) & 1u;
+            fl[3] = (b >> 7) & 1u;
+            fl[4] = rec[15] & 31u;
+            fl[5] = rec[16];
+            fl[6] = rec[17];
+            psid[i] = (uint16_t)las_u(rec + 18, 2);
+            if (fmt == 2) rgb_at = 20;
+            if (fmt == 3 || fmt == 5) rgb_at = 28;
+        } else {
+            const unsigned rn = rec[14] & 15u, nr = rec[14] >> 4;
+            if (nr <= 7) {
+                fl[0] = (uint8_t)(rn & 7u); /* LASlib's 3-bit field keeps the low bits */
+                fl[1] = (uint8_t)nr;
+            } else {
+                fl[0] = (uint8_t)(rn <= 6 ? rn : (rn >= nr ? 7 : 6));
+                fl[1] = 7;
+            }
+            fl[2] = (rec[15] >> 6) & 1u;
+            fl[3] = (rec[15] >> 7) & 1u;
+            fl[4] = rec[16] <= 31 ? rec[16] : 0;
+            const int16_t ang = (int16_t)(uint16_t)las_u(rec + 18, 2);
+            const float deg = 0.006f * (float)ang;
+            fl[5] = (uint8_t)(int8_t)(int)deg;
+            fl[6] = rec[17];
+            psid[i] = (uint16_t)las_u(rec + 20, 2);
+            if (fmt == 7 || fmt == 8 || fmt == 10) rgb_at = 30;
+        }
+        fl[7] = 0;
+        for (int c = 0; c < 3; ++c) rgb[3 * i + c] = rgb_at ? (uint32_t)las_u(rec + rgb_at + 2 * c, 2) : 0u;
+    }
+    free(rec);
+    fclose(f);
+    return 0;
+}

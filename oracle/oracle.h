@@ -29,6 +29,14 @@
  *                            test_complex.cpp:37-94
  *   orc_knn / orc_knn_brute: brute-force (d2, index) order; reference KATs test_octree.cpp:126-145,
  *                            test_octree_advanced.cpp:213-231, test_complex.cpp:144-172
+ *   orc_las_*              : LAS files written field by field from the ASPRS LAS 1.2 / 1.3 / 1.4
+ *                            layouts by an independent writer (tests/las_files.py) with known
+ *                            integers and attributes, and power-of-two scales whose coordinates have a
+ *                            closed form. The reference reads through LASlib (LAStools, cloned at its
+ *                            head by scripts/install_laslib.sh, so no version is pinned), absent here:
+ *                            the attributes of point formats 6-10 follow LASlib's legacy view as
+ *                            restated in orc_las_read and are "parity unpinned"; formats 0-5 and all
+ *                            coordinates are pinned by the files' own fields.
  */
 #ifndef OCTREE_ORACLE_H
 #define OCTREE_ORACLE_H
@@ -126,6 +134,17 @@ uint64_t orc_knn_brute(const double *sorted_xyz, size_t n, const double *centers
                        uint32_t k, int64_t *idx, double *d2);
 
 int orc_max_threads(void);
+
+/* LAS ingest as the reference's reader sees it (readers/las_file_reader_parallel.hpp:52-64, 74-95,
+ * 112-137 through LASlib). orc_las_header: info[0] version minor, [1] point format, [2] record length,
+ * [3] number of points (legacy count, or the LAS 1.4 64-bit count when the legacy one is 0),
+ * [4] offset to the point records; so[6] = x, y, z scale then x, y, z offset. Returns 0, or -1 for
+ * an unreadable file / bad signature, -2 for LAZ or an undefined format.
+ * orc_las_read: xyz (n x 3) = X * scale + offset (:88-92); attributes as PointMetadata
+ * (:124-137): intensity (n), fields (n x 8: return number, number of returns, scan direction, edge
+ * of flight line, classification, scan angle (as int8), user data, 0), point source id (n), rgb (n x 3). */
+int orc_las_header(const char *path, uint64_t info[5], double so[6]);
+int orc_las_read(const char *path, double *xyz, double *intensity, uint8_t *fields, uint16_t *psid, uint32_t *rgb);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,9 @@ def lib():
         L.orc_knn_brute.argtypes = [_F64P, C.c_size_t, _F64P, C.c_size_t, C.c_uint32, _I64P, _F64P]
         L.orc_knn_brute.restype = C.c_uint64
         L.orc_max_threads.restype = C.c_int
+        L.orc_las_header.argtypes = [C.c_char_p, _U64P, _F64P]
+        L.orc_las_read.argtypes = [C.c_char_p, _F64P, _F64P, np.ctypeslib.ndpointer(np.uint8, flags="C"),
+                                   np.ctypeslib.ndpointer(np.uint16, flags="C"), _U32P]
         _LIB = L
     return _LIB
 
@@ -277,3 +280,28 @@ def knn_brute(sorted_xyz, centers, k):
     d2 = np.zeros(nq * k, np.float64)
     lib().orc_knn_brute(xyz, xyz.shape[0], centers, nq, k, idx, d2)
     return idx.reshape(nq, k), d2.reshape(nq, k)
+
+
+def las_header(path):
+    """orc_las_header: (version minor, format, record length, n, offset to points), scale, offset."""
+    info = np.zeros(5, np.uint64)
+    so = np.zeros(6)
+    r = lib().orc_las_header(os.fsencode(path), info, so)
+    if r:
+        raise RuntimeError(f"oracle LAS header: status {r}")
+    return [int(v) for v in info], so[:3].copy(), so[3:].copy()
+
+
+def las_read(path):
+    """orc_las_read: xyz (n x 3) and the per-point attributes as the reference's reader hands them out."""
+    info, _, _ = las_header(path)
+    n = info[3]
+    xyz = np.zeros((max(n, 1), 3))
+    inten = np.zeros(max(n, 1))
+    fields = np.zeros((max(n, 1), 8), np.uint8)
+    psid = np.zeros(max(n, 1), np.uint16)
+    rgb = np.zeros((max(n, 1), 3), np.uint32)
+    r = lib().orc_las_read(os.fsencode(path), xyz, inten, fields, psid, rgb)
+    if r:
+        raise RuntimeError(f"oracle LAS read: status {r}")
+    return {"xyz": xyz[:n], "intensity": inten[:n], "fields": fields[:n], "psid": psid[:n], "rgb": rgb[:n]}

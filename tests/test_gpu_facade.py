@@ -49,6 +49,8 @@ def parse(text):
             out[f[0]] = int(f[1])
         elif f[0] == "codec":
             out["codec"] = tuple(int(v) for v in f[1:])
+        elif f[0] == "meta":
+            out.setdefault("meta", []).append([float(f[2])] + [int(v) for v in f[3:]])
         elif f[0] == "iterator":
             raise AssertionError(line)
     return out
@@ -108,3 +110,42 @@ def test_facade_matches_oracle(enc):
     assert got["traversal_leaves"] == tree.nLeaf
     k = (O.hilbert_encode if enc == A.ENC_HILBERT else O.morton_encode)(1000, 2000, 3000)
     assert got["codec"] == (k, 1000, 2000, 3000)
+
+
+def test_facade_reads_las(tmp_path):
+    """readPoints on a LAS file (device decode), sortPoints co-permuting the attributes, then the tree
+    and a search: the oracle on the file's exactly known coordinates (power-of-two scale)."""
+    from las_files import expected_attributes, write_las
+    if not os.path.exists(EXE):
+        pytest.fail("facade_check is not built (make -C octrees-benchmark_amd)")
+    rng = np.random.default_rng(3)
+    n = 4000
+    X, Y, Z = rng.integers(0, 20 * 1024, n), rng.integers(0, 20 * 1024, n), rng.integers(0, 6 * 1024, n)
+    path = str(tmp_path / "p.las")
+    w = write_las(path, X, Y, Z, 3, minor=2, scale=(1 / 1024, 1 / 1024, 1 / 1024), offset=(100.0, 200.0, 0.0), seed=4)
+    xyz = np.stack([X / 1024 + 100.0, Y / 1024 + 200.0, Z / 1024], axis=1)
+    centers = xyz[rng.integers(0, n, 10)]
+    cf = str(tmp_path / "c.bin")
+    centers.astype("<f8").tofile(cf)
+    print("[gpu] facade_check on a LAS file", flush=True)
+    r = subprocess.run([EXE, str(A.ENC_HILBERT), "16", path, cf, "1.0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = parse(r.stdout)
+    sx, keys, perm, c, rr = O.sort_points(A.ENC_HILBERT, xyz)
+    np.testing.assert_array_equal([i for i, _ in got["pt"]], perm)
+    np.testing.assert_array_equal(np.array([k for _, k in got["pt"]], np.uint64), keys)
+    f, inten, psid, rgb = expected_attributes(w)
+    meta = np.array(got["meta"])
+    p = perm.astype(np.int64)
+    np.testing.assert_array_equal(meta[:, 0], inten[p])
+    np.testing.assert_array_equal(meta[:, 1:5].astype(np.int64), f[p, 0:4])
+    np.testing.assert_array_equal(meta[:, 5].astype(np.int64), f[p, 4])
+    np.testing.assert_array_equal(meta[:, 6].astype(np.int64) & 255, f[p, 5])
+    np.testing.assert_array_equal(meta[:, 7].astype(np.int64), f[p, 6])
+    np.testing.assert_array_equal(meta[:, 8].astype(np.int64), psid[p])
+    np.testing.assert_array_equal(meta[:, 9:12].astype(np.int64), rgb[p])
+    tree = O.Tree(keys, 16, A.ENC_HILBERT, c, rr)
+    assert got["tree"] == (tree.nLeaf, tree.nInternal, tree.nTotal)
+    ref, _ = O.radius(tree, sx, O.K_SPHERE, O.A_STRUCT, centers, 1.0)
+    for q in range(len(centers)):
+        np.testing.assert_array_equal(got["struct"][("sphere", q)], ref[q])

@@ -483,3 +483,60 @@ def test_approx_bounds_and_cutoff():
     # level 0, upper bound: an OVERLAP root adds all 8 children whole
     up, us = O.radius_approx(t, sx, O.K_SPHERE, [[5.0, 5.0, 5.0]], 0, True, radii=2.0)
     assert us[0] == len(sx)
+
+
+# ------------------------------------------------------------------ LAS ingest (next: data formats)
+LAS_CASES = [(minor, fmt) for minor in (2, 3) for fmt in range(6)] + [(4, fmt) for fmt in range(11)]
+
+
+@pytest.mark.parametrize("minor,fmt", LAS_CASES)
+def test_las_oracle_reads_written_fields(tmp_path, minor, fmt):
+    """Files packed field by field from the LAS layouts: coordinates X * scale + offset, attributes as
+    written (formats 0-5) or through LASlib's legacy fields (6-10), extra bytes and VLRs skipped."""
+    from las_files import expected_attributes, write_las
+    rng = np.random.default_rng(100 * minor + fmt)
+    n = 1500
+    X = rng.integers(-2 ** 31, 2 ** 31, n)
+    Y = rng.integers(-2 ** 31, 2 ** 31, n)
+    Z = rng.integers(-10 ** 6, 10 ** 6, n)
+    scale, off = (0.001, 0.01, 0.0001), (652000.25, -4100000.5, 12.0)
+    path = str(tmp_path / f"c{minor}_{fmt}.las")
+    w = write_las(path, X, Y, Z, fmt, minor=minor, scale=scale, offset=off, extra=3 * (fmt % 2), vlrs=fmt % 3, seed=fmt)
+    info, s, o = O.las_header(path)
+    assert info[0] == minor and info[1] == fmt and info[3] == n
+    assert np.array_equal(s, scale) and np.array_equal(o, off)
+    got = O.las_read(path)
+    for a, v in enumerate((X, Y, Z)):
+        ref = v.astype(np.float64) * scale[a] + off[a]
+        assert np.array_equal(got["xyz"][:, a].view(np.uint64), ref.view(np.uint64)), f"axis {a}"
+    f, inten, psid, rgb = expected_attributes(w)
+    np.testing.assert_array_equal(got["fields"], f)
+    np.testing.assert_array_equal(got["intensity"], inten)
+    np.testing.assert_array_equal(got["psid"], psid)
+    np.testing.assert_array_equal(got["rgb"], rgb)
+
+
+def test_las_oracle_closed_form_coordinates(tmp_path):
+    """Power-of-two scales and integral offsets: every coordinate is exact, X / 4 + 1000 etc."""
+    from las_files import write_las
+    X = np.arange(-5000, 5000, 7)
+    Y = X[::-1] * 3
+    Z = X % 251
+    path = str(tmp_path / "exact.las")
+    write_las(path, X, Y, Z, 1, minor=2, scale=(0.25, 0.5, 0.125), offset=(1000.0, -20.0, 3.0))
+    got = O.las_read(path)["xyz"]
+    np.testing.assert_array_equal(got[:, 0], X / 4 + 1000.0)
+    np.testing.assert_array_equal(got[:, 1], Y / 2 - 20.0)
+    np.testing.assert_array_equal(got[:, 2], Z / 8 + 3.0)
+
+
+def test_las_oracle_rejects(tmp_path):
+    from las_files import write_las
+    path = str(tmp_path / "z.las")
+    write_las(path, [1, 2], [3, 4], [5, 6], 1, compressed=True)
+    with pytest.raises(RuntimeError):
+        O.las_header(path)
+    bad = tmp_path / "bad.las"
+    bad.write_bytes(b"LASX" + bytes(400))
+    with pytest.raises(RuntimeError):
+        O.las_header(str(bad))
