@@ -104,6 +104,12 @@ int oct_decode_keys(oct_ctx *ctx, int encoder, const uint64_t *keys, size_t n, u
  * key. perm_out[i] (nullable) = input index of the point now at sorted position i; keys_out
  * (nullable) = sorted keys. The device keeps the sorted cloud for the build and searches. */
 int oct_sort(oct_ctx *ctx, uint32_t *perm_out, uint64_t *keys_out);
+/* The reorder of sortPoints applied to caller records (point_encoder.hpp:321-323, 367-375: the points
+ * and the PointMetadata vector move with the keys): out[i] = in[perm[i]] for the ctx's n records of
+ * record_bytes bytes, perm the last oct_sort's. device_pointers != 0: in / out are device buffers
+ * (gathered in place on the device); else host buffers (copied in, gathered on the device, copied
+ * back). in and out must not overlap. */
+int oct_gather_records(oct_ctx *ctx, const void *in, void *out, size_t record_bytes, int device_pointers);
 
 /* One call for sortPoints(points, meta) (point_encoder.hpp:484-503): bbox + encode + sort. */
 int oct_sort_points(oct_ctx *ctx, int encoder, double center_out[3], double radii_out[3], uint32_t *perm_out,

@@ -173,6 +173,7 @@ int oct_destroy(oct_ctx *ctx) {
     ctx->entries.release(), ctx->stats.release(), ctx->kidx.release(), ctx->kd2.release();
     ctx->next.release(), ctx->qrad3.release(), ctx->kc_xyz.release(), ctx->kc_keys.release();
     ctx->kd2f.release(), ctx->radtab.release(), ctx->las_rec.release(), ctx->las_meta.release();
+    ctx->rec_in.release(), ctx->rec_out.release();
     for (int s = 0; s < ST_N; s++) {
         if (ctx->ev0[s]) (void)hipEventDestroy(ctx->ev0[s]);
         if (ctx->ev1[s]) (void)hipEventDestroy(ctx->ev1[s]);
@@ -259,6 +260,27 @@ int oct_sort(oct_ctx *ctx, uint32_t *perm_out, uint64_t *keys_out) {
     ctx->has_sorted = true;
     ctx->has_keys = false;  // the unsorted key buffer was consumed
     ctx->has_tree = ctx->has_lists = false;
+    return OCT_OK;
+}
+
+int oct_gather_records(oct_ctx *ctx, const void *in, void *out, size_t record_bytes, int device_pointers) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (!ctx->has_sorted || !ctx->perm) return fail(ctx, OCT_ERR_STATE, "oct_gather_records needs the permutation of oct_sort");
+    if (record_bytes == 0 || record_bytes > (1u << 16)) return fail(ctx, OCT_ERR_ARG, "record_bytes must be 1 .. 65536");
+    const size_t n = ctx->n;
+    if (!n) return OCT_OK;
+    if (!in || !out) return fail(ctx, OCT_ERR_ARG, "null record buffer");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    if (device_pointers) {
+        HCHK(launch_gather_records(ctx->stream, in, ctx->perm, out, n, record_bytes), "gather records");
+    } else {
+        HCHK(ctx->rec_in.reserve(n * record_bytes), "alloc records");
+        HCHK(ctx->rec_out.reserve(n * record_bytes), "alloc records");
+        HCHK(hipMemcpyAsync(ctx->rec_in.p, in, n * record_bytes, hipMemcpyHostToDevice, ctx->stream), "copy records");
+        HCHK(launch_gather_records(ctx->stream, ctx->rec_in.p, ctx->perm, ctx->rec_out.p, n, record_bytes), "gather records");
+        HCHK(hipMemcpyAsync(out, ctx->rec_out.p, n * record_bytes, hipMemcpyDeviceToHost, ctx->stream), "copy records");
+    }
+    HCHK(hipStreamSynchronize(ctx->stream), "gather records");
     return OCT_OK;
 }
 

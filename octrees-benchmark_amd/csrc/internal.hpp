@@ -85,6 +85,7 @@ struct oct_ctx {
     // LAS ingest: raw point records and the decoded per-point attributes of the last file loaded
     oct::DevBuf<unsigned char> las_rec;
     oct::DevBuf<oct_point_meta> las_meta;
+    oct::DevBuf<unsigned char> rec_in, rec_out;  // oct_gather_records staging of host records
     size_t las_n = 0;
     bool las_loaded = false;
 
@@ -244,6 +245,8 @@ hipError_t launch_sort_segmented(hipStream_t s, uint64_t *keys0, uint64_t *keys1
 // counters (npasses) must be zero.
 hipError_t launch_sort_passes(hipStream_t s, uint64_t *keys0, uint64_t *keys1, uint32_t *vals0, uint32_t *vals1, size_t n,
                               int npasses, const uint32_t *bases, uint32_t *lookback, uint32_t *counters, uint32_t *err);
+// out[i] = in[perm[i]] for records of rb bytes (device buffers)
+hipError_t launch_gather_records(hipStream_t s, const void *in, const uint32_t *perm, void *out, size_t n, size_t rb);
 // out[i] = (xyz[3 perm[i]], xyz[3 perm[i] + 1], xyz[3 perm[i] + 2], 0)
 hipError_t launch_gather_raw(hipStream_t s, const double *xyz, const uint32_t *perm, double4 *out, size_t n);
 hipError_t exclusive_scan_u32_to_u64(oct_ctx *ctx, const uint32_t *in, uint64_t *out, size_t n);

@@ -277,6 +277,32 @@ __global__ void k_gather(const double *__restrict__ xyz, const uint32_t *__restr
     }
 }
 
+// Records of any size moved by the permutation: out[i] = in[perm[i]], 16-byte words when the record
+// size and both buffers allow it, else 4-byte words, else bytes.
+template <class W>
+__global__ void k_gather_records(const W *__restrict__ in, const uint32_t *__restrict__ perm, W *__restrict__ out, size_t n,
+                                 uint32_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const W *src = in + (size_t)perm[i] * words;
+        W *dst = out + i * words;
+        for (uint32_t w = 0; w < words; w++) dst[w] = src[w];
+    }
+}
+
+hipError_t launch_gather_records(hipStream_t s, const void *in, const uint32_t *perm, void *out, size_t n, size_t rb) {
+    if (!n) return hipSuccess;
+    int nb = (int)((n + 255) / 256);
+    nb = nb > 8192 ? 8192 : nb;
+    const uintptr_t al = (uintptr_t)in | (uintptr_t)out;
+    if (rb % 16 == 0 && al % 16 == 0)
+        k_gather_records<uint4><<<nb, 256, 0, s>>>((const uint4 *)in, perm, (uint4 *)out, n, (uint32_t)(rb / 16));
+    else if (rb % 4 == 0 && al % 4 == 0)
+        k_gather_records<uint32_t><<<nb, 256, 0, s>>>((const uint32_t *)in, perm, (uint32_t *)out, n, (uint32_t)(rb / 4));
+    else
+        k_gather_records<uint8_t><<<nb, 256, 0, s>>>((const uint8_t *)in, perm, (uint8_t *)out, n, (uint32_t)rb);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather_raw(hipStream_t s, const double *xyz, const uint32_t *perm, double4 *out, size_t n) {
     if (!n) return hipSuccess;
     int nb = (int)((n + 255) / 256);

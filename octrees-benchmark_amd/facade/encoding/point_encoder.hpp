@@ -75,7 +75,7 @@ public:
         std::vector<uint32_t> perm(n);
         std::vector<key_t> keys(n);
         octamd::check(c.get(), oct_sort(c.get(), perm.data(), reinterpret_cast<uint64_t *>(keys.data())), "oct_sort");
-        apply(points, meta_opt, perm);
+        apply(c.get(), points, meta_opt, perm);
         if (log) {
             double ms[8];
             oct_stage_times(c.get(), ms);
@@ -98,7 +98,7 @@ public:
         octamd::check(c.get(),
                       oct_sort_points(c.get(), (int)getEncoder(), ctr, rad, perm.data(), reinterpret_cast<uint64_t *>(keys.data())),
                       "oct_sort_points");
-        apply(points, meta_opt, perm);
+        apply(c.get(), points, meta_opt, perm);
         if (log) {
             double ms[8];
             oct_stage_times(c.get(), ms);
@@ -135,14 +135,29 @@ private:
             octamd::check(c.get(), oct_encode_coords(c.get(), (int)getEncoder(), xyz, n, keys), "oct_encode_coords");
     }
 
+    // the points and (when present) their metadata reordered as the keys: records of trivially
+    // copyable types are gathered on the device by the permutation the context holds
     template <class Container>
-    static void apply(Container &points, std::optional<std::vector<PointMetadata>> &meta_opt, const std::vector<uint32_t> &perm) {
+    static void apply(oct_ctx *ctx, Container &points, std::optional<std::vector<PointMetadata>> &meta_opt,
+                      const std::vector<uint32_t> &perm) {
+        using T = typename Container::value_type;
         Container sorted(points.size());
-        for (size_t i = 0; i < perm.size(); i++) sorted[i] = points[perm[i]];
+        if constexpr (std::is_trivially_copyable<T>::value) {
+            if (!points.empty())
+                octamd::check(ctx, oct_gather_records(ctx, points.data(), sorted.data(), sizeof(T), 0), "oct_gather_records");
+        } else {
+            for (size_t i = 0; i < perm.size(); i++) sorted[i] = points[perm[i]];
+        }
         points.swap(sorted);
         if (meta_opt) {
             std::vector<PointMetadata> m(meta_opt->size());
-            for (size_t i = 0; i < perm.size() && i < m.size(); i++) m[i] = (*meta_opt)[perm[i]];
+            if (m.size() == perm.size()) {
+                if (!m.empty())
+                    octamd::check(ctx, oct_gather_records(ctx, meta_opt->data(), m.data(), sizeof(PointMetadata), 0),
+                                  "oct_gather_records");
+            } else {  // a metadata vector of another length: the reference's loop over the points' order
+                for (size_t i = 0; i < perm.size() && i < m.size(); i++) m[i] = (*meta_opt)[perm[i]];
+            }
             meta_opt->swap(m);
         }
     }

@@ -29,7 +29,7 @@ EXPORTS = (
     "oct_batch_build", "oct_batch_info", "oct_batch_stage_times", "oct_batch_radius", "oct_batch_knn",
     "oct_batch_build_device", "oct_batch_set_points", "oct_batch_cloud_sorted", "oct_batch_cloud_tree",
     "oct_batch_queries", "oct_batch_pipeline", "oct_batch_pipeline_results", "oct_batch_autotune_knn",
-    "oct_las_read_header", "oct_las_load", "oct_las_fetch",
+    "oct_las_read_header", "oct_las_load", "oct_las_fetch", "oct_gather_records",
 )
 
 # oct_point_meta (per-point LAS attributes, PointMetadata)
@@ -119,6 +119,7 @@ def lib():
         L.oct_batch_set_points.argtypes = [_vp, C.c_int, _vp, C.c_size_t]
         L.oct_batch_cloud_sorted.argtypes = [_vp, C.c_int, _vp, _vp]
         L.oct_batch_cloud_tree.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
+        L.oct_gather_records.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_int]
         L.oct_las_read_header.argtypes = [C.c_char_p, C.POINTER(LasHeader)]
         L.oct_las_load.argtypes = [_vp, C.c_char_p, C.POINTER(LasHeader)]
         L.oct_las_fetch.argtypes = [_vp, _vp, _vp]
@@ -375,6 +376,18 @@ class Octree:
             raise OctError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
     # ---- H1-H3
+    def gather_records(self, records):
+        """records (n rows of any fixed-size dtype, input order) reordered by the last sort's permutation."""
+        records = np.ascontiguousarray(records)
+        out = np.empty_like(records)
+        rb = records.dtype.itemsize * (int(np.prod(records.shape[1:])) if records.ndim > 1 else 1)
+        self._chk(self.L.oct_gather_records(self.h, _ptr(records), _ptr(out), rb, 0), "oct_gather_records")
+        return out
+
+    def gather_records_dev(self, in_ptr, out_ptr, record_bytes):
+        """Device-buffer form (e.g. torch tensors' data_ptr())."""
+        self._chk(self.L.oct_gather_records(self.h, in_ptr, out_ptr, record_bytes, 1), "oct_gather_records")
+
     def las_load(self, path):
         """The LAS file's points become the context's cloud (decoded on the device); returns its header."""
         h = LasHeader()
