@@ -70,6 +70,15 @@ def profile(args):
         ctx.close()
         return
     kind, _, spec = args.profile.partition(":")
+    if kind == "knn" and spec:  # 'knn:10' -> one all-points kNN per precision, default lanes per query
+        ctx.sort_points(A.ENC_HILBERT, fetch=False)
+        ctx.build(args.max_leaf)
+        for prec in (A.PREC_FP32, A.PREC_FP64):
+            ctx.knn_dev(int(spec), prec)
+            ctx.synchronize()
+            log(f"[profile] knn k={spec} precision {prec}: {ctx.stage_times()['knn']:.2f} ms")
+        ctx.close()
+        return
     if kind != "search" or not spec:
         sys.exit(f"unknown --profile {args.profile!r}")
     ctx.sort_points(A.ENC_HILBERT)
@@ -282,8 +291,8 @@ def main():
     ap.add_argument("--profile", default=None,
                     help="profiling driver instead of the benchmark: 'search:v64,d64' (one all-points search per "
                          "configuration: v = FP32 vector ALUs, m = FP32 matrix cores, d = FP64, then queries per "
-                         "wave), 'sort' (bbox + encode + sort + gather + build, 3 times) or 'copy' (a 1 GiB copy "
-                         "kernel, the counter calibration)")
+                         "wave), 'knn:10' (one all-points kNN per precision), 'sort' (bbox + encode + sort + gather "
+                         "+ build, 3 times) or 'copy' (a 1 GiB copy kernel, the counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
