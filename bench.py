@@ -288,6 +288,8 @@ def main():
                     help="tile50m: the benchmark line (default); mls400m: the mobile-laser-scan stress run; batch256: "
                          "256 independent clouds searched together (each prints one JSON line of its own)")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--las", default=None, help="run the benchmark line on the points of this (uncompressed) LAS file "
+                                                 "instead of the synthetic tile")
     ap.add_argument("--profile", default=None,
                     help="profiling driver instead of the benchmark: 'search:v64,d64' (one all-points search per "
                          "configuration: v = FP32 vector ALUs, m = FP32 matrix cores, d = FP64, then queries per "
@@ -308,14 +310,27 @@ def main():
         batch_workload(args)
         return
 
-    n = args.n
-    extent = args.extent if args.n == 50_000_000 else args.extent * (n / 50_000_000) ** 0.5
-    t0 = time.time()
-    xyz = A.lidar_tile(n, seed=2, extent=extent)
-    log(f"[bench] generated {n} points over {extent:.0f} m in {time.time() - t0:.1f} s")
-
     ctx = A.Octree(0)
-    ctx.set_points(xyz)
+    las = None
+    if args.las:
+        # a LAS file instead of the synthetic tile: header parsed on the host, records decoded on the device
+        t0 = time.perf_counter()
+        hdr = ctx.las_load(args.las)
+        ingest_s = time.perf_counter() - t0
+        xyz, _ = ctx.las_fetch(meta=False)
+        n = xyz.shape[0]
+        extent = float(max(hdr["max"][0] - hdr["min"][0], hdr["max"][1] - hdr["min"][1]))
+        las = {"file": os.path.basename(args.las), "points": n, "format": hdr["point_format"],
+               "version": f"{hdr['version_major']}.{hdr['version_minor']}", "ingest_wall_s": ingest_s,
+               "ingest_mb_per_s": n * hdr["record_length"] / ingest_s / 1e6}
+        log(f"[bench] LAS {args.las}: {n} points (format {hdr['point_format']}) read and decoded in {ingest_s:.2f} s")
+    else:
+        n = args.n
+        extent = args.extent if args.n == 50_000_000 else args.extent * (n / 50_000_000) ** 0.5
+        t0 = time.time()
+        xyz = A.lidar_tile(n, seed=2, extent=extent)
+        log(f"[bench] generated {n} points over {extent:.0f} m in {time.time() - t0:.1f} s")
+        ctx.set_points(xyz)
     # encode + sort + build (untimed warm-up, then timed with device events)
     sort_ms, enc_ms, bbox_ms, gather_ms, build_ms = [], [], [], [], []
     for rep in range(3):
@@ -493,8 +508,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64-recheck",
-        "data": "synthetic",
-        "config": {"workload": f"synthetic airborne-LiDAR tile, {n} FP64 points over {extent:.0f}x{extent:.0f} m, "
+        "data": "synthetic" if las is None else "LAS file",
+        "config": {"workload": (f"synthetic airborne-LiDAR tile, {n} FP64 points over {extent:.0f}x{extent:.0f} m, "
+                                if las is None else f"LAS file {las['file']}, {n} points, ") +
                                f"Hilbert 21 bits/axis, maxLeaf {args.max_leaf}, all-points sphere r={args.radius} m, "
                                f"neighborsStruct, per-query neighbour counts",
                    "points": n, "queries": n, "radius": args.radius, "max_points_leaf": args.max_leaf,
@@ -513,6 +529,7 @@ def main():
                       "build": med(build_ms), "search_fp32": fp32["kernel_ms"], "search_fp32_other_unit": fp32v["kernel_ms"],
                       "search_fp64": fp64["kernel_ms"]},
         "verify": verify,
+        "las": las,
         "autotune": tuned,
         "union_efficiency": {"fp32_test_slots_used": fp32["stats"]["points_tested"] / max(1, fp32["stats"]["test_slots"]),
                              "fp32_class_slots_used": fp32["stats"]["nodes_classified"] / max(1, fp32["stats"]["class_slots"])},
