@@ -225,7 +225,8 @@ int oct_tuned_knn(oct_ctx *ctx, int precision, uint32_t k, int *lanes);
  * oct_batch_build then lays all clouds out as one forest on the device, and each search call runs
  * every cloud's queries in one launch per search kind (kNN: one per k size class). Queries name a
  * cloud and a centre by its index in that cloud's sorted order; results use the same indices.
- * oct_batch_add copies the points (stride as oct_set_points); encoder OCT_ENC_MORTON3D or
+ * oct_batch_add copies the points (stride as oct_set_points), before any build and before
+ * oct_batch_queries (OCT_ERR_STATE after them); encoder OCT_ENC_MORTON3D or
  * OCT_ENC_HILBERT3D; *cloud_id receives the cloud's number (0, 1, ...). oct_batch_radius: counts of
  * a count-mode search with a radius per query. oct_batch_knn: exact k nearest with a k per query
  * (1 .. kmax, kmax <= 1024); rows of kmax entries, -1 / NaN past min(n, k). oct_batch_info: a cloud's

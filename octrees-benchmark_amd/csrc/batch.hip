@@ -225,7 +225,10 @@ static int put_points(oct_batch *b, const double *xyz, size_t n, size_t stride_b
 int oct_batch_add(oct_batch *b, const double *xyz, size_t n, size_t stride_bytes, int encoder, uint32_t max_points_leaf,
                   int *cloud_id) {
     if (!b) return OCT_ERR_ARG;
-    if (b->built) return bfail(b, OCT_ERR_STATE, "batch already built");
+    // the device build sizes its tables and buffers for the clouds present at its first call
+    if (b->built || b->mode || b->dev_ready) return bfail(b, OCT_ERR_STATE, "clouds cannot be added to a built batch");
+    // the query grouping buffers are sized for the clouds present at oct_batch_queries
+    if (b->queries_set) return bfail(b, OCT_ERR_STATE, "clouds cannot be added after oct_batch_queries");
     if (encoder != OCT_ENC_MORTON3D && encoder != OCT_ENC_HILBERT3D) return bfail(b, OCT_ERR_UNSUPPORTED, "encoder");
     if (max_points_leaf == 0) return bfail(b, OCT_ERR_ARG, "max_points_leaf must be positive");
     if (stride_bytes < 24 || stride_bytes % 8) return bfail(b, OCT_ERR_ARG, "stride_bytes must be >= 24 and a multiple of 8");

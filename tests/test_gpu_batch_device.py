@@ -165,3 +165,31 @@ def test_capacity_grows_and_rebuilds():
         check_cloud(b, 1, oracle_cloud(*clouds[1]))
     finally:
         b.close()
+
+
+def test_clouds_fixed_once_built_or_queried():
+    """The device build and the query grouping size their buffers for the clouds present: adding a
+    cloud afterwards is refused (also after oct_batch_set_points, which clears the built state)."""
+    a, b_ = A.uniform_box(5000, seed=1), A.uniform_box(3000, seed=2)
+    b = A.Batch(0)
+    try:
+        b.add(a, A.ENC_HILBERT, 32)
+        b.build_device(graph=False)
+        with pytest.raises(A.OctError):
+            b.add(b_, A.ENC_HILBERT, 32)
+        b.set_points(0, a[::-1])
+        with pytest.raises(A.OctError):
+            b.add(b_, A.ENC_HILBERT, 32)
+    finally:
+        b.close()
+    b = A.Batch(0)
+    try:
+        b.add(a, A.ENC_HILBERT, 32)
+        b.queries([0], [1], [1.0], [0], [2], [3], kmax=3)
+        with pytest.raises(A.OctError):
+            b.add(b_, A.ENC_HILBERT, 32)
+        b.pipeline(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, graph=True)
+        counts, idx, d2 = b.pipeline_results()
+        assert counts[0] >= 1 and idx[0, 0] == 2
+    finally:
+        b.close()
