@@ -288,6 +288,8 @@ def main():
                     help="tile50m: the benchmark line (default); mls400m: the mobile-laser-scan stress run; batch256: "
                          "256 independent clouds searched together (each prints one JSON line of its own)")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--competitor", action="store_true",
+                    help="also time the reference's competitor, its vendored nanoflann KD-tree, on the CPU")
     ap.add_argument("--las", default=None, help="run the benchmark line on the points of this (uncompressed) LAS file "
                                                  "instead of the synthetic tile")
     ap.add_argument("--profile", default=None,
@@ -417,6 +419,7 @@ def main():
         verify["avg_result_size"] = int(c64.astype(np.int64).sum() // max(n, 1))
 
     cpu = None
+    competitor = None
     if not args.no_cpu or not args.no_verify:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O
@@ -470,6 +473,23 @@ def main():
                              f"neighborsStruct sphere r={args.radius} with the reference harness loop "
                              f"(build each range list, keep its size), OpenMP dynamic schedule, {dt:.1f} s"}
             log(f"[bench] cpu baseline {cpu}")
+        if args.competitor and O.nanoflann_available():
+            # the reference's competitor: its vendored nanoflann KD-tree (leaf size maxLeaf) over the sorted
+            # cloud, radiusSearch per centre as its benchmark drives it, on a sample of the centres
+            t = time.time()
+            nf = O.NanoflannTree(sx, args.max_leaf)
+            nf_build = time.time() - t
+            nsamp = np.arange(0, n, 10, dtype=np.int64)
+            t = time.time()
+            nfc = nf.radius_counts(sx[nsamp], args.radius)
+            nf_dt = time.time() - t
+            gpu_c, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, idx=nsamp.astype(np.uint32), radius=args.radius)
+            competitor = {"name": "nanoflann KD-tree (reference's vendored 1.7.1), CPU", "build_s": nf_build,
+                          "radius_queries_per_s": len(nsamp) / nf_dt, "cores": int(O.lib().orc_max_threads()),
+                          "sample": f"{len(nsamp)} centres (every 10th SFC index), sphere r={args.radius}",
+                          "counts_eq_gpu": bool(np.array_equal(nfc, gpu_c.astype(np.uint64)))}
+            log(f"[bench] competitor {competitor}")
+            del nf
 
     # Roofline of the dominant kernel (the FP32 search): 9 flops (3 sub + 3 mul + 2 add + 1 compare) per
     # point test of the reference traversal (T, identical for both precisions), over the kernel's
@@ -537,6 +557,7 @@ def main():
             name: {kk: v for kk, v in knn[name].items() if kk != "_sample"} for name in ("fp32", "fp64") if name in knn},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "competitor_cpu": competitor,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -305,3 +305,50 @@ def las_read(path):
     if r:
         raise RuntimeError(f"oracle LAS read: status {r}")
     return {"xyz": xyz[:n], "intensity": inten[:n], "fields": fields[:n], "psid": psid[:n], "rgb": rgb[:n]}
+
+
+# ------------------------------------------------------------------ competitor baseline (reference's nanoflann)
+_NF = None
+
+
+def nanoflann_available():
+    return os.path.exists(os.path.join(_HERE, "_ref", "libref_nanoflann.so"))
+
+
+def _nf():
+    global _NF
+    if _NF is None:
+        L = C.CDLL(os.path.join(_HERE, "_ref", "libref_nanoflann.so"))
+        L.refnf_build.argtypes = [_F64P, C.c_size_t, C.c_uint32]
+        L.refnf_build.restype = _P
+        L.refnf_free.argtypes = [_P]
+        L.refnf_radius_counts.argtypes = [_P, _F64P, C.c_size_t, C.c_double, _U64P]
+        L.refnf_knn.argtypes = [_P, _F64P, C.c_size_t, C.c_uint32, _I64P, _F64P]
+        _NF = L
+    return _NF
+
+
+class NanoflannTree:
+    """The reference's vendored nanoflann KD-tree (competitor baseline), leaf size maxPointsLeaf."""
+
+    def __init__(self, xyz, leaf):
+        self.xyz = _f64(xyz).reshape(-1, 3)  # kept alive: the tree reads the caller's points
+        self.h = _nf().refnf_build(self.xyz, self.xyz.shape[0], leaf)
+
+    def radius_counts(self, centers, radius):
+        centers = _f64(centers).reshape(-1, 3)
+        out = np.zeros(centers.shape[0], np.uint64)
+        _nf().refnf_radius_counts(self.h, centers, centers.shape[0], float(radius), out)
+        return out
+
+    def knn(self, centers, k):
+        centers = _f64(centers).reshape(-1, 3)
+        idx = np.zeros((centers.shape[0], k), np.int64)
+        d2 = np.zeros((centers.shape[0], k))
+        _nf().refnf_knn(self.h, centers, centers.shape[0], k, idx, d2)
+        return idx, d2
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _nf().refnf_free(self.h)
+            self.h = None

@@ -540,3 +540,22 @@ def test_las_oracle_rejects(tmp_path):
     bad.write_bytes(b"LASX" + bytes(400))
     with pytest.raises(RuntimeError):
         O.las_header(str(bad))
+
+
+# ------------------------------------------------------------------ competitor baseline
+@pytest.mark.skipif(not O.nanoflann_available(), reason="oracle/_ref/libref_nanoflann.so not built")
+def test_nanoflann_baseline_agrees_with_oracle():
+    """The reference's nanoflann KD-tree (its competitor) gives the sphere-kernel counts of the oracle
+    (strict < on the same left-to-right sum of squares) and the same k smallest distances."""
+    rng = np.random.default_rng(31)
+    pts = np.vstack([rng.uniform(0, 30, (6000, 3)) * [1, 1, 0.2], rng.normal(size=(500, 3)) * 0.3 + 5])
+    sx, keys, perm, c, r = O.sort_points(O.ENC_HILBERT, pts)
+    tree = O.Tree(keys, 32, O.ENC_HILBERT, c, r)
+    nf = O.NanoflannTree(sx, 32)
+    centers = sx[rng.integers(0, len(sx), 300)]
+    for radius in (0.5, 1.5):
+        ref = O.radius_count(tree, sx, O.K_SPHERE, O.A_STRUCT, centers, radius)
+        np.testing.assert_array_equal(nf.radius_counts(centers, radius), ref)
+    bi, bd = O.knn_brute(sx, centers, 12)
+    ni, nd = nf.knn(centers, 12)
+    np.testing.assert_array_equal(nd, bd)
