@@ -122,3 +122,25 @@ def test_pipeline_new_points(pipe):
     same(b.pipeline_results(), per_call(b, q, A.K_SPHERE, A.A_STRUCT, A.PREC_FP32))
     b.set_points(i, xyz)
     b.pipeline(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, graph=True)
+
+
+def test_pipeline_one_query_kind():
+    """Only radius queries, then only kNN queries: the empty kind's work queues end at once."""
+    clouds = make_clouds(seed=45, count=5)
+    b = A.Batch(0)
+    try:
+        for xyz, enc, ml in clouds:
+            b.add(xyz, enc, ml)
+        rc, ri, rr, kc, ki, kk = query_set(clouds, seed=9, nr=3000, nk=800)
+        b.queries(rc, ri, rr, [], [], [], kmax=0)
+        b.pipeline(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, graph=True)
+        counts, _, _ = b.pipeline_results()
+        ref, _ = b.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, rc, ri, rr)
+        np.testing.assert_array_equal(counts, ref)
+        b.queries([], [], [], kc, ki, kk, kmax=1024)
+        b.pipeline(A.K_SPHERE, A.A_STRUCT, A.PREC_FP64, graph=True)
+        _, idx, d2 = b.pipeline_results()
+        gi, gd, _ = b.knn(A.PREC_FP64, kc, ki, kk, 1024)
+        np.testing.assert_array_equal(idx, gi)
+    finally:
+        b.close()
