@@ -60,7 +60,7 @@ struct DevBuf {
 };
 
 // Radix sort geometry: tiles of kSortTile keys, kRadixPassCount passes of 8 bits.
-constexpr int kSortTile = 4096;
+constexpr int kSortTile = 6144;
 constexpr int kRadixPassCount = 8;
 // A sort tile of a batch: [start, start + len) inside cloud `cloud`, whose first tile is `first`.
 struct SortTile {

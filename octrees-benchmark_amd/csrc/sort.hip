@@ -4,7 +4,7 @@
 // (point_encoder.hpp:293-376); its result is exactly "stable sort by key", which is what this sort
 // produces. Design:
 //  - digit histograms of every pass come from the encode kernel (no extra read of the keys);
-//  - one kernel per pass ("onesweep"): a 256-thread workgroup takes a 4096-key tile (dynamic tile
+//  - one kernel per pass ("onesweep"): a 256-thread workgroup takes a 6144-key tile (dynamic tile
 //    ids, so tiles start in order), ranks keys stably with wave64 ballots on the 8 digit bits,
 //    publishes per-bin tile counts and resolves its global bin offsets by decoupled look-back over
 //    the preceding tiles, stages the tile in LDS in bin order and writes it out in runs;
@@ -16,7 +16,7 @@
 namespace oct {
 
 constexpr int kSortBlock = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 24;  // 6144-key tiles: 2 blocks per CU (80 KB LDS each); measured best of 8..32 keys per thread
 static_assert(kSortBlock * kSortItems == kSortTile, "sort tile size");
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagInc = 2u << 30, kValMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 24;
