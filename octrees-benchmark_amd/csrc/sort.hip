@@ -96,51 +96,60 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
     }
     __syncthreads();
 
-    // per-bin tile count and per-wave exclusive offsets (thread b owns bin b)
+    // per-bin tile count and per-wave exclusive offsets (thread b < 256 owns bin b; with more than
+    // 256 threads the other waves wait at the barriers)
     const uint32_t b = tid;
+    const bool owner = kSortBlock == 256 || tid < 256;
     uint32_t tile_count = 0;
-#pragma unroll
-    for (int ww = 0; ww < kSortBlock / 64; ww++) {
-        uint32_t c = s_wave[ww][b];
-        s_wave[ww][b] = tile_count;
-        tile_count += c;
-    }
     uint32_t *lb = lookback + (size_t)tile * 256;
-    if (tile == first_tile)
-        store_flag(&lb[b], kFlagInc | tile_count);
-    else
-        store_flag(&lb[b], kFlagAgg | tile_count);
+    if (owner) {
+#pragma unroll
+        for (int ww = 0; ww < kSortBlock / 64; ww++) {
+            uint32_t c = s_wave[ww][b];
+            s_wave[ww][b] = tile_count;
+            tile_count += c;
+        }
+        if (tile == first_tile)
+            store_flag(&lb[b], kFlagInc | tile_count);
+        else
+            store_flag(&lb[b], kFlagAgg | tile_count);
+    }
 
     // local exclusive scan of the tile counts over bins
-    uint32_t incl = wave_incl_scan(tile_count);
-    if (lane == 63) s_wsum[w] = incl;
-    __syncthreads();
-    uint32_t wpre = 0;
-    for (uint32_t ww = 0; ww < w; ww++) wpre += s_wsum[ww];
-    s_tile_start[b] = wpre + incl - tile_count;
-
-    // decoupled look-back over preceding tiles for bin b
-    uint32_t excl = 0;
-    if (tile > first_tile) {
-        int64_t t = (int64_t)tile - 1;
-        uint32_t spins = 0;
-        while (t >= (int64_t)first_tile) {
-            uint32_t x = load_flag(&lookback[(size_t)t * 256 + b]);
-            if ((x >> 30) == 0) {
-                if (++spins > kSpinLimit) {
-                    atomicOr(err, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += x & kValMask;
-            if ((x >> 30) == 2) break;
-            t--;
-        }
-        store_flag(&lb[b], kFlagInc | ((excl + tile_count) & kValMask));
+    uint32_t incl = 0;
+    if (owner) {
+        incl = wave_incl_scan(tile_count);
+        if (lane == 63) s_wsum[w] = incl;
     }
-    s_global[b] = bin_base[(size_t)cloud * kSegBaseStride + b] + excl;
+    __syncthreads();
+    if (owner) {
+        uint32_t wpre = 0;
+        for (uint32_t ww = 0; ww < w; ww++) wpre += s_wsum[ww];
+        s_tile_start[b] = wpre + incl - tile_count;
+
+        // decoupled look-back over preceding tiles for bin b
+        uint32_t excl = 0;
+        if (tile > first_tile) {
+            int64_t t = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            while (t >= (int64_t)first_tile) {
+                uint32_t x = load_flag(&lookback[(size_t)t * 256 + b]);
+                if ((x >> 30) == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += x & kValMask;
+                if ((x >> 30) == 2) break;
+                t--;
+            }
+            store_flag(&lb[b], kFlagInc | ((excl + tile_count) & kValMask));
+        }
+        s_global[b] = bin_base[(size_t)cloud * kSegBaseStride + b] + excl;
+    }  // owner
     __syncthreads();
 
     // stage the tile in LDS in (bin, original order) order
