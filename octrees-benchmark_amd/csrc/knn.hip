@@ -342,8 +342,17 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
     }
 }
 
+// Occupancy target (waves per SIMD): 6 for the 8-lanes, k <= 16, FP32 shape (80 VGPRs and a 20-byte
+// spill instead of 92 VGPRs at 5 waves: 75.1 vs 79.9 ms for all 20 M points of the tile at k = 10);
+// every other shape, and the work-queue form (which would spill 56 bytes at that target), keep the
+// compiler's choice.
 template <int S, int KC, int PREC>
-__global__ __launch_bounds__(64) void k_knn(const KnnArgs a) {
+constexpr int knn_waves() {
+    return (S == 8 && KC == 16 && PREC == OCT_PREC_FP32_RECHECK) ? 6 : 1;
+}
+
+template <int S, int KC, int PREC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(knn_waves<S, KC, PREC>()))) void k_knn(const KnnArgs a) {
     knn_tile<S, KC, PREC>(a, blockIdx.x);
 }
 
