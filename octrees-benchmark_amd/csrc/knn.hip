@@ -14,12 +14,14 @@
 // Leaf points are staged in LDS once per wave. Each query keeps an LDS buffer of 2 KC entries that
 // starts with its best entries so far, sorted, and appends every point that beats its current k-th
 // entry. When a buffer could overflow, the wave selects (S lanes per query) and keeps each query's
-// k best, which sets the new k-th distance. For KC >= 32 the sorted entries own the first KC slots
-// and points are appended to the second KC: a selection sorts the second half descending (bitonic)
-// and merges it with the first (one bitonic merge of 2 KC). For KC = 16 points are appended right
-// after the kept entries and a selection sorts the whole buffer: as many network stages, but it
-// selects less often when k is well below KC (that class serves k = 1 .. 16; 18.3 vs 19.5 ms for
-// 4 M queries at k = 10, while k = 32 takes 23.5 ms by merge against 31.3 ms by full sorts).
+// k best, which sets the new k-th distance. By merge: the sorted entries own the first KC slots and
+// points are appended to the second KC; a selection sorts the second half descending (bitonic) and
+// merges it with the first (one bitonic merge of 2 KC). By full sort: points are appended right
+// after the kept entries and a selection sorts the whole buffer, as many network stages as a merge
+// for KC = 16 but less often when every k of the wave is well below KC. Merge serves every KC >= 32
+// and the KC = 16 waves whose largest k is above 12 (4 M queries, 8 lanes each, by full sorts vs by
+// merge: k = 10 18.3 vs 19.5 ms, k = 12 18.5 vs 18.5, k = 13 19.9 vs 19.1, k = 16 29.2 vs 20.9,
+// k = 32 31.3 vs 23.5).
 // FP32_RECHECK filters the points with the leaf-recentred FP32 form
 // e = w - 2 a.b + |b|^2 (the count kernel's representation), whose error is below
 // hw / 2 = 2^-19 (amax + |b|^2); only points with e <= k-th + hw get their exact FP64 distance.
@@ -32,7 +34,8 @@ namespace oct {
 
 constexpr int kKnnStack = 8 * 22 + 8;  // depth first: at most 7 entries per level + 8 live
 constexpr int kKnnChunk = 128;         // leaf points staged per round
-constexpr int kKnnMergeMinKC = 32;     // buffer size classes that select by sort-and-merge
+constexpr int kKnnMergeMinKC = 32;     // buffer size classes that always select by sort-and-merge
+constexpr uint32_t kKnnMergeMinK = 13; // smaller classes: by merge when the wave's largest k reaches this
 
 struct KnnArgs {
     const double4 *__restrict__ p64;
@@ -81,7 +84,6 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
     constexpr int STEP = S * U;                         // points per query per step
     constexpr bool F32 = PREC == OCT_PREC_FP32_RECHECK;
     constexpr int STG = kKnnChunk + STEP;               // staged slots incl. the last step's tail
-    constexpr bool MERGE = KC >= kKnnMergeMinKC;        // select by sort-and-merge (else a full sort)
     static_assert(STEP <= KC, "a step must fit after a selection");
     __shared__ double s_bd[G * CAP];
     __shared__ uint32_t s_bi[G * CAP];
@@ -131,6 +133,8 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
     const double dlx = 0x1p-40 * (magp[0] + fabs(qx)), dly = 0x1p-40 * (magp[1] + fabs(qy));
     const double dlz = 0x1p-40 * (magp[2] + fabs(qz));
 
+    // select by sort-and-merge (else by full sorts); the same in every lane (a ballot)
+    const bool merge = KC >= kKnnMergeMinKC || ballot64(valid && k >= kKnnMergeMinK) != 0;
     for (uint32_t t = g; t < (uint32_t)CAP; t += S) bd[t] = __builtin_huge_val(), bi[t] = 0xffffffffu;
     uint32_t cnt = 0;        // kept entries of this query, sorted, at [0, cnt) (same in its S lanes)
     uint32_t fresh = 0;      // appended entries, unsorted, at [KC or cnt, + fresh) (merge or full sort)
@@ -167,7 +171,7 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
     auto select = [&]() {
         n_select++;
         wsync();
-        if constexpr (MERGE) {
+        if (merge) {
             // appended half [KC, 2 KC) sorted descending (empty slots hold +inf and end up first)
             bitonic(bd + KC, bi + KC, (uint32_t)KC, true);
             // ascending kept half + descending appended half is bitonic: one merge sorts all 2 KC
@@ -229,7 +233,7 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
             wsync();
             for (uint32_t s = 0; s < nsteps; s++) {
                 // appended entries start at KC (merge) or right after the kept ones (full sort)
-                if (ballot64(need && fresh + STEP > (MERGE ? (uint32_t)KC : (uint32_t)CAP - cnt))) select();
+                if (ballot64(need && fresh + STEP > (merge ? (uint32_t)KC : (uint32_t)CAP - cnt))) select();
                 n_steps++;
                 float thr = __builtin_huge_valf();
                 if (F32 && full && !force) thr = ((tau32 + hw) * (1.0f + 0x1p-20f));
@@ -270,7 +274,7 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
                 for (int u = 0; u < U; u++) {
                     const uint64_t m = ballot64(c[u]) & segmask;
                     if (c[u]) {
-                        const uint32_t pos = (MERGE ? (uint32_t)KC : cnt) + fresh + (uint32_t)__popcll(m & lanemask_lt());
+                        const uint32_t pos = (merge ? (uint32_t)KC : cnt) + fresh + (uint32_t)__popcll(m & lanemask_lt());
                         bd[pos] = d[u];
                         bi[pos] = id[u];
                     }
