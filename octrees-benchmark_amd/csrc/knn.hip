@@ -20,8 +20,9 @@
 // after the kept entries and a selection sorts the whole buffer, as many network stages as a merge
 // for KC = 16 but less often when every k of the wave is well below KC. Merge serves every KC >= 32
 // and the KC = 16 waves whose largest k is above 12 (4 M queries, 8 lanes each, by full sorts vs by
-// merge: k = 10 18.3 vs 19.5 ms, k = 12 18.5 vs 18.5, k = 13 19.9 vs 19.1, k = 16 29.2 vs 20.9,
-// k = 32 31.3 vs 23.5).
+// merge, one network stage per LDS round trip: k = 10 18.3 vs 19.5 ms, k = 12 18.5 vs 18.5, k = 13
+// 19.9 vs 19.1, k = 16 29.2 vs 20.9, k = 32 31.3 vs 23.5). The networks run two stages per LDS round
+// trip (k = 10: 16.4 -> 15.4 ms FP32; k = 250, 64 lanes: 28.8 -> 24.8 ms for 0.5 M queries).
 // FP32_RECHECK filters the points with the leaf-recentred FP32 form
 // e = w - 2 a.b + |b|^2 (the count kernel's representation), whose error is below
 // hw / 2 = 2^-19 (amax + |b|^2); only points with e <= k-th + hw get their exact FP64 distance.
@@ -144,27 +145,56 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
     float tau32 = __builtin_huge_valf();  // tau rounded up to FP32
     uint64_t n_nodes = 0, n_test = 0, n_recheck = 0, n_select = 0, n_steps = 0;
 
+    // compare-exchange in registers: afterwards (da, ia) <= (db, ib) when up, >= when not
+    auto cx = [](double &da, uint32_t &ia, double &db, uint32_t &ib, const bool up) {
+        const bool sw = up ? knn_less(db, ib, da, ia) : knn_less(da, ia, db, ib);
+        const double lo = sw ? db : da, hi = sw ? da : db;
+        const uint32_t li = sw ? ib : ia, hj = sw ? ia : ib;
+        da = lo, db = hi, ia = li, ib = hj;
+    };
+    // bitonic stages with strides `stride`, stride / 2, ..., 1 over the n entries at (sd, si), in
+    // blocks of `size` entries alternately ascending and descending (descending first when desc).
+    // Two stages per LDS round trip: a lane loads the 4 entries that strides s and s / 2 connect
+    // (b, b + s / 2, b + s, b + 3 s / 2), applies both stages in registers and stores them back.
+    auto stages = [&](double *sd, uint32_t *si, const uint32_t n, const uint32_t size, uint32_t stride,
+                      const bool desc) {
+#pragma unroll 1
+        for (; stride >= 2; stride >>= 2) {
+            const uint32_t h = stride >> 1;
+#pragma unroll 1
+            for (uint32_t t = g; t < n / 4; t += S) {
+                const uint32_t e0 = ((t & ~(h - 1)) << 2) | (t & (h - 1));
+                const uint32_t e1 = e0 + h, e2 = e0 + stride, e3 = e2 + h;
+                const bool up = ((e0 & size) == 0) != desc;
+                double d0 = sd[e0], d1 = sd[e1], d2 = sd[e2], d3 = sd[e3];
+                uint32_t i0 = si[e0], i1 = si[e1], i2 = si[e2], i3 = si[e3];
+                cx(d0, i0, d2, i2, up);
+                cx(d1, i1, d3, i3, up);
+                cx(d0, i0, d1, i1, up);
+                cx(d2, i2, d3, i3, up);
+                sd[e0] = d0, sd[e1] = d1, sd[e2] = d2, sd[e3] = d3;
+                si[e0] = i0, si[e1] = i1, si[e2] = i2, si[e3] = i3;
+            }
+            wsync();
+        }
+        if (stride == 1) {  // an odd number of stages: the last one alone
+#pragma unroll 1
+            for (uint32_t t = g; t < n / 2; t += S) {
+                const uint32_t e0 = t << 1;
+                const bool up = ((e0 & size) == 0) != desc;
+                double d0 = sd[e0], d1 = sd[e0 + 1];
+                uint32_t i0 = si[e0], i1 = si[e0 + 1];
+                cx(d0, i0, d1, i1, up);
+                sd[e0] = d0, sd[e0 + 1] = d1;
+                si[e0] = i0, si[e0 + 1] = i1;
+            }
+            wsync();
+        }
+    };
     // bitonic sort of the n entries at (sd, si), ascending or descending by (distance, index)
     auto bitonic = [&](double *sd, uint32_t *si, const uint32_t n, const bool descending) {
 #pragma unroll 1
-        for (uint32_t size = 2; size <= n; size <<= 1) {
-#pragma unroll 1
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t t = g; t < n / 2; t += S) {
-                    const uint32_t i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
-                    const uint32_t j = i + stride;
-                    const bool up = ((i & size) == 0) != descending;
-                    const double di = sd[i], dj = sd[j];
-                    const uint32_t ii = si[i], ij = si[j];
-                    const bool sw = up ? knn_less(dj, ij, di, ii) : knn_less(di, ii, dj, ij);
-                    if (sw) {
-                        sd[i] = dj, sd[j] = di;
-                        si[i] = ij, si[j] = ii;
-                    }
-                }
-                wsync();
-            }
-        }
+        for (uint32_t size = 2; size <= n; size <<= 1) stages(sd, si, n, size, size >> 1, descending);
     };
 
     // merge every query's appended entries into its kept ones, keep its k best
@@ -175,20 +205,7 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
             // appended half [KC, 2 KC) sorted descending (empty slots hold +inf and end up first)
             bitonic(bd + KC, bi + KC, (uint32_t)KC, true);
             // ascending kept half + descending appended half is bitonic: one merge sorts all 2 KC
-#pragma unroll 1
-            for (uint32_t stride = KC; stride > 0; stride >>= 1) {
-                for (uint32_t t = g; t < (uint32_t)CAP / 2; t += S) {
-                    const uint32_t i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
-                    const uint32_t j = i + stride;
-                    const double di = bd[i], dj = bd[j];
-                    const uint32_t ii = bi[i], ij = bi[j];
-                    if (knn_less(dj, ij, di, ii)) {
-                        bd[i] = dj, bd[j] = di;
-                        bi[i] = ij, bi[j] = ii;
-                    }
-                }
-                wsync();
-            }
+            stages(bd, bi, (uint32_t)CAP, 2u * CAP, (uint32_t)KC, false);
         } else {
             bitonic(bd, bi, (uint32_t)CAP, false);  // kept and appended entries, [0, cnt + fresh)
         }
