@@ -22,7 +22,9 @@
 // and the KC = 16 waves whose largest k is above 12 (4 M queries, 8 lanes each, by full sorts vs by
 // merge, one network stage per LDS round trip: k = 10 18.3 vs 19.5 ms, k = 12 18.5 vs 18.5, k = 13
 // 19.9 vs 19.1, k = 16 29.2 vs 20.9, k = 32 31.3 vs 23.5). The networks run two stages per LDS round
-// trip (k = 10: 16.4 -> 15.4 ms FP32; k = 250, 64 lanes: 28.8 -> 24.8 ms for 0.5 M queries).
+// trip (k = 10: 16.4 -> 15.4 ms FP32; k = 250, 64 lanes: 28.8 -> 24.8 ms for 0.5 M queries), three
+// for KC >= 512, where LDS already bounds the occupancy (k = 1000: 37.6 -> 35.0 ms for 0.1 M queries;
+// for smaller KC three stages cost a wave per SIMD and run slower).
 // FP32_RECHECK filters the points with the leaf-recentred FP32 form
 // e = w - 2 a.b + |b|^2 (the count kernel's representation), whose error is below
 // hw / 2 = 2^-19 (amax + |b|^2); only points with e <= k-th + hw get their exact FP64 distance.
@@ -37,6 +39,7 @@ constexpr int kKnnStack = 8 * 22 + 8;  // depth first: at most 7 entries per lev
 constexpr int kKnnChunk = 128;         // leaf points staged per round
 constexpr int kKnnMergeMinKC = 32;     // buffer size classes that always select by sort-and-merge
 constexpr uint32_t kKnnMergeMinK = 13; // smaller classes: by merge when the wave's largest k reaches this
+constexpr int kKnnTripleMinKC = 512;   // buffer size classes whose networks run 3 stages per LDS round trip
 
 struct KnnArgs {
     const double4 *__restrict__ p64;
@@ -158,6 +161,35 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
     // (b, b + s / 2, b + s, b + 3 s / 2), applies both stages in registers and stores them back.
     auto stages = [&](double *sd, uint32_t *si, const uint32_t n, const uint32_t size, uint32_t stride,
                       const bool desc) {
+        if constexpr (KC >= kKnnTripleMinKC) {
+            // three stages per round trip: the 8 entries that strides s, s / 2 and s / 4 connect
+#pragma unroll 1
+            for (; stride >= 4; stride >>= 3) {
+                const uint32_t h = stride >> 1, q = stride >> 2;
+#pragma unroll 1
+                for (uint32_t t = g; t < n / 8; t += S) {
+                    const uint32_t e0 = ((t & ~(q - 1)) << 3) | (t & (q - 1));
+                    const bool up = ((e0 & size) == 0) != desc;
+                    const uint32_t o[8] = {0, q, h, h + q, stride, stride + q, stride + h, stride + h + q};
+                    double d[8];
+                    uint32_t ix[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) d[u] = sd[e0 + o[u]], ix[u] = si[e0 + o[u]];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) cx(d[u], ix[u], d[u + 4], ix[u + 4], up);
+#pragma unroll
+                    for (int u = 0; u < 8; u += 4) {
+                        cx(d[u], ix[u], d[u + 2], ix[u + 2], up);
+                        cx(d[u + 1], ix[u + 1], d[u + 3], ix[u + 3], up);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u += 2) cx(d[u], ix[u], d[u + 1], ix[u + 1], up);
+#pragma unroll
+                    for (int u = 0; u < 8; u++) sd[e0 + o[u]] = d[u], si[e0 + o[u]] = ix[u];
+                }
+                wsync();
+            }
+        }
 #pragma unroll 1
         for (; stride >= 2; stride >>= 2) {
             const uint32_t h = stride >> 1;

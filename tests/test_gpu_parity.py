@@ -183,7 +183,7 @@ def knn_case(ctx):
 
 
 @pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("k", [1, 10, 17, 50, 100, 250, 600, 1000])
+@pytest.mark.parametrize("k", [1, 10, 17, 50, 100, 250, 400, 600, 1000])
 def test_knn_exact(ctx, k, prec):
     say(f"[gpu] knn k={k} prec={prec}")
     sx, keys, perm, c, r, tree = knn_case(ctx)
