@@ -115,6 +115,37 @@ __host__ __device__ inline uint64_t hilbert_encode_tab(const uint16_t *tab, uint
     return key;
 }
 
+// Two levels per lookup: tab2[state * 64 + (upper level's octant << 3 | lower level's octant)] =
+// the two digits (upper << 3 | lower) | the state after both << 6, composed from the one-level table.
+constexpr int kHilbertTab2 = 48 * 64;
+__host__ __device__ inline void hilbert_tab2_fill(uint16_t *tab2, const uint16_t *tab, uint32_t first, uint32_t step) {
+    for (uint32_t i = first; i < (uint32_t)kHilbertTab2; i += step) {
+        const uint32_t s = i >> 6, oh = (i >> 3) & 7u, ol = i & 7u;
+        const uint32_t e1 = tab[s * 8 + oh], e2 = tab[(e1 >> 3) * 8 + ol];
+        tab2[i] = (uint16_t)(((e1 & 7u) << 3) | (e2 & 7u) | ((e2 >> 3) << 6));
+    }
+}
+// Every level's octant in one word: bits [3L, 3L + 3) = x bit L << 2 | y bit L << 1 | z bit L.
+__host__ __device__ inline uint64_t hilbert_octants(uint32_t x, uint32_t y, uint32_t z) {
+    return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+}
+// The key of hilbert_encode_tab from the octant word: the top level by the one-level table, the 20
+// levels below it two at a time (11 dependent lookups instead of 21, and no per-level bit gathering).
+__host__ __device__ inline uint64_t hilbert_encode_tab2(const uint16_t *tab, const uint16_t *tab2, uint32_t x, uint32_t y,
+                                                        uint32_t z) {
+    const uint64_t m = hilbert_octants(x, y, z);
+    const uint32_t e = tab[kHilbertStart * 8 + (uint32_t)(m >> (3 * (kMaxDepth - 1)))];
+    uint64_t key = e & 7u;
+    uint32_t s = e >> 3;
+#pragma unroll
+    for (int level = (int)kMaxDepth - 3; level >= 0; level -= 2) {  // levels (level + 1, level)
+        const uint32_t e2 = tab2[s * 64 + ((uint32_t)(m >> (3 * level)) & 63u)];
+        key = (key << 6) | (e2 & 63u);
+        s = e2 >> 6;
+    }
+    return key;
+}
+
 // HilbertEncoder3D::decode (hilbert_encoder_3d.hpp:256-289); used per node, not per point.
 __host__ __device__ inline void hilbert_decode(uint64_t code, uint32_t &ox, uint32_t &oy, uint32_t &oz) {
     uint32_t x = 0, y = 0, z = 0;

@@ -101,15 +101,20 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__
                                                       uint64_t *__restrict__ keys, uint32_t *__restrict__ hist) {
     __shared__ uint32_t sh[kRadixPasses * 256];
     __shared__ uint16_t stab[48 * 8];
+    __shared__ uint16_t stab2[kHilbertTab2];
     for (int i = threadIdx.x; i < kRadixPasses * 256; i += blockDim.x) sh[i] = 0;
     if (ENC == OCT_ENC_HILBERT3D)
         for (int i = threadIdx.x; i < 48 * 8; i += blockDim.x) stab[i] = kHilbert.t[i];
     __syncthreads();
+    if (ENC == OCT_ENC_HILBERT3D) {
+        hilbert_tab2_fill(stab2, stab, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         uint32_t x = quantize_axis(xyz[3 * i + 0], box.c[0], box.r[0]);
         uint32_t y = quantize_axis(xyz[3 * i + 1], box.c[1], box.r[1]);
         uint32_t z = quantize_axis(xyz[3 * i + 2], box.c[2], box.r[2]);
-        uint64_t k = (ENC == OCT_ENC_HILBERT3D) ? hilbert_encode_tab(stab, x, y, z) : morton_encode(x, y, z);
+        uint64_t k = (ENC == OCT_ENC_HILBERT3D) ? hilbert_encode_tab2(stab, stab2, x, y, z) : morton_encode(x, y, z);
         keys[i] = k;
 #pragma unroll
         for (int p = 0; p < kRadixPasses; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);

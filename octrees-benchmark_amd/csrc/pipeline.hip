@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) void k_bp_encode(const double *__restrict__ xy
                                                    uint32_t *__restrict__ hist) {
     __shared__ uint32_t sh[kRadixPassCount * 256];
     __shared__ uint16_t stab[48 * 8];
+    __shared__ uint16_t stab2[kHilbertTab2];
     const Chunk ch = chunks[blockIdx.x];
     const int enc = (int)geom[ch.cloud].encoder;
     double c[3], r[3];
@@ -131,12 +132,16 @@ __global__ __launch_bounds__(256) void k_bp_encode(const double *__restrict__ xy
     for (int i = threadIdx.x; i < kRadixPassCount * 256; i += blockDim.x) sh[i] = 0;
     for (int i = threadIdx.x; i < 48 * 8; i += blockDim.x) stab[i] = kHilbert.t[i];
     __syncthreads();
+    if (enc == OCT_ENC_HILBERT3D) {  // the same for the whole block
+        hilbert_tab2_fill(stab2, stab, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
     const size_t end = (size_t)ch.start + ch.len;
     for (size_t i = (size_t)ch.start + threadIdx.x; i < end; i += blockDim.x) {
         const uint32_t x = quantize_axis(xyz[3 * i + 0], c[0], r[0]);
         const uint32_t y = quantize_axis(xyz[3 * i + 1], c[1], r[1]);
         const uint32_t z = quantize_axis(xyz[3 * i + 2], c[2], r[2]);
-        const uint64_t k = (enc == OCT_ENC_HILBERT3D) ? hilbert_encode_tab(stab, x, y, z) : morton_encode(x, y, z);
+        const uint64_t k = (enc == OCT_ENC_HILBERT3D) ? hilbert_encode_tab2(stab, stab2, x, y, z) : morton_encode(x, y, z);
         keys[i] = k;
 #pragma unroll
         for (int p = 0; p < kRadixPassCount; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
