@@ -18,10 +18,10 @@
 // points are appended to the second KC; a selection sorts the second half descending (bitonic) and
 // merges it with the first (one bitonic merge of 2 KC). By full sort: points are appended right
 // after the kept entries and a selection sorts the whole buffer, as many network stages as a merge
-// for KC = 16 but less often when every k of the wave is well below KC. Merge serves every KC >= 32
-// and the KC = 16 waves whose largest k is above 12 (4 M queries, 8 lanes each, by full sorts vs by
-// merge, one network stage per LDS round trip: k = 10 18.3 vs 19.5 ms, k = 12 18.5 vs 18.5, k = 13
-// 19.9 vs 19.1, k = 16 29.2 vs 20.9, k = 32 31.3 vs 23.5). The networks run two stages per LDS round
+// for KC = 16 but less often when every k of the wave is below KC. Merge serves every KC >= 32 and
+// the KC = 16 waves whose largest k is 16 (4 M queries, 8 lanes each, by full sorts vs by merge:
+// k = 13 17.7 vs 20.1 ms, k = 14 19.1 vs 20.7, k = 15 20.9 vs 21.2, k = 16 24.0 vs 21.8; one stage per
+// round trip, k = 32: 31.3 vs 23.5). The networks run two stages per LDS round
 // trip (k = 10: 16.4 -> 15.4 ms FP32; k = 250, 64 lanes: 28.8 -> 24.8 ms for 0.5 M queries), three
 // for KC >= 512, where LDS already bounds the occupancy (k = 1000: 37.6 -> 35.0 ms for 0.1 M queries;
 // for smaller KC three stages cost a wave per SIMD and run slower).
@@ -38,7 +38,7 @@ namespace oct {
 constexpr int kKnnStack = 8 * 22 + 8;  // depth first: at most 7 entries per level + 8 live
 constexpr int kKnnChunk = 128;         // leaf points staged per round
 constexpr int kKnnMergeMinKC = 32;     // buffer size classes that always select by sort-and-merge
-constexpr uint32_t kKnnMergeMinK = 13; // smaller classes: by merge when the wave's largest k reaches this
+constexpr uint32_t kKnnMergeMinK = 16; // smaller classes: by merge when the wave's largest k reaches this
 constexpr int kKnnTripleMinKC = 512;   // buffer size classes whose networks run 3 stages per LDS round trip
 
 struct KnnArgs {
