@@ -64,16 +64,18 @@ __global__ __launch_bounds__(256) void k_children(const uint64_t *__restrict__ k
     }
 }
 
-// Leaf-recentred FP32 coordinates, one wave per leaf (recentre_leaf).
+// Leaf-recentred FP32 coordinates, kRecentreLanes lanes per node (recentre_leaf); internal nodes'
+// groups leave at once.
 __global__ __launch_bounds__(256) void k_recentre(const double4 *__restrict__ p64, const NodeRec *__restrict__ nodes,
                                                  const uint32_t *__restrict__ child, uint32_t n_total, float4 *__restrict__ p32,
                                                  float *__restrict__ amax) {
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t lane = threadIdx.x & 63;
-    if (wave >= n_total) return;
-    if (child[wave] != 0) return;
-    const float m = recentre_leaf(p64, nodes[wave], lane, p32);
-    if (lane == 0) amax[wave] = m;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t u = t / kRecentreLanes;
+    const uint32_t g = (uint32_t)(t % kRecentreLanes);
+    if (u >= n_total) return;
+    if (child[u] != 0) return;
+    const float m = recentre_leaf<kRecentreLanes>(p64, nodes[u], g, p32);
+    if (g == 0) amax[u] = m;
 }
 
 // Per internal node u: its children regrouped by corner bits (make_node_ext).
@@ -191,7 +193,7 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t M, std::string &err) {
     if ((e = ctx->amax.reserve(n_total)) != hipSuccess) return e;
 
     if (n_total) {
-        size_t threads = (size_t)n_total * 64;
+        size_t threads = (size_t)n_total * kRecentreLanes;
         unsigned nb = (unsigned)((threads + 255) / 256);
         k_recentre<<<nb, 256, 0, ctx->stream>>>(ctx->p64.p, ctx->nodes.p, ctx->child.p, n_total, ctx->p32.p, ctx->amax.p);
     }

@@ -345,17 +345,17 @@ __global__ void k_bp_sentinel(const uint32_t *__restrict__ lv, uint32_t *__restr
     if (blockIdx.x == 0 && threadIdx.x == 0) child[lv[kLvCount - 1]] = 0;
 }
 
-// one wave per leaf, grid-stride over the forest's nodes
+// kRecentreLanes lanes per leaf, grid-stride over the forest's nodes
 __global__ __launch_bounds__(256) void k_bp_recentre(const double4 *__restrict__ p64, const NodeRec *__restrict__ nodes,
                                                      const uint32_t *__restrict__ child, const uint32_t *__restrict__ lv,
                                                      float4 *__restrict__ p32, float *__restrict__ amax) {
     const uint32_t n_total = lv[kLvCount - 1];
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t u = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; u < n_total; u += waves) {
-        if (child[u] != 0) continue;
-        const float m = recentre_leaf(p64, nodes[u], lane, p32);
-        if (lane == 0) amax[u] = m;
+    const uint32_t g = threadIdx.x % kRecentreLanes;
+    const uint32_t groups = (gridDim.x * blockDim.x) / kRecentreLanes;
+    for (uint32_t u = (blockIdx.x * blockDim.x + threadIdx.x) / kRecentreLanes; u < n_total; u += groups) {
+        if (child[u] != 0) continue;  // the same for the whole group
+        const float m = recentre_leaf<kRecentreLanes>(p64, nodes[u], g, p32);
+        if (g == 0) amax[u] = m;
     }
 }
 

@@ -33,30 +33,35 @@ __device__ inline void put_node(NodeRec *__restrict__ nodes, uint64_t *__restric
     nkey[u] = key;
 }
 
-// Leaf-recentred FP32 coordinates of one leaf, by one wave: a = fl32(p - c_leaf) per axis (FP64
-// subtraction, one rounding to FP32), w = fl32(|a|^2) computed in FP64 from the FP32 values; returns
-// (to every lane) amax = max |a|^2 over the leaf rounded up, which bounds the FP32 rounding band, or
-// +inf when a coordinate is not finite in FP32 (the leaf's FP32 tests are then never trusted).
-__device__ inline float recentre_leaf(const double4 *__restrict__ p64, const NodeRec &r, uint32_t lane,
+// Leaf-recentred FP32 coordinates of one leaf, by a group of L lanes (L a power of two <= 64, the
+// groups aligned in the wave; lane g of the group takes points first + g, first + g + L, ...):
+// a = fl32(p - c_leaf) per axis (FP64 subtraction, one rounding to FP32), w = fl32(|a|^2) computed in
+// FP64 from the FP32 values; returns (to every lane of the group) amax = max |a|^2 over the leaf
+// rounded up, which bounds the FP32 rounding band, or +inf when a coordinate is not finite in FP32
+// (the leaf's FP32 tests are then never trusted). The results do not depend on L.
+template <int L>
+__device__ inline float recentre_leaf(const double4 *__restrict__ p64, const NodeRec &r, uint32_t g,
                                       float4 *__restrict__ p32) {
     double m = 0.0;
-    bool bad = false;
-    for (uint32_t i = r.first + lane; i < r.last; i += 64) {
+    uint32_t bad = 0;
+    for (uint32_t i = r.first + g; i < r.last; i += L) {
         double4 p = p64[i];
         float ax = (float)(p.x - r.cx), ay = (float)(p.y - r.cy), az = (float)(p.z - r.cz);
         double w = (double)ax * (double)ax + (double)ay * (double)ay + (double)az * (double)az;
         const float fw = (float)w;
         p32[i] = make_float4(ax, ay, az, fw);
-        bad |= !(fw < __builtin_huge_valf());
+        bad |= !(fw < __builtin_huge_valf()) ? 1u : 0u;
         m = w > m ? w : m;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        double o = __shfl_xor(m, off);
+    for (int off = L / 2; off > 0; off >>= 1) {  // within the group: xor of a lane index below L
+        const double o = __shfl_xor(m, off);
         m = o > m ? o : m;
+        bad |= __shfl_xor(bad, off);
     }
-    if (__ballot(bad)) m = __builtin_huge_val();
+    if (bad) m = __builtin_huge_val();
     return __double2float_ru(m);
 }
+constexpr int kRecentreLanes = 16;  // lanes per leaf (leaves hold at most maxLeaf points, ~30 on average)
 
 // Children of internal node (first child fc, level `level`) regrouped by corner bits. A child's bits
 // come from its decoded key at level + 1, so they are exact also on degenerate axes where two
