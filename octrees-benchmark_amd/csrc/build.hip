@@ -78,21 +78,21 @@ __global__ __launch_bounds__(256) void k_recentre(const double4 *__restrict__ p6
     if (g == 0) amax[u] = m;
 }
 
-// Per internal node u: its children regrouped by corner bits (make_node_ext).
+// Per internal node u: its children regrouped by corner bits, 8 lanes per node (node_ext_child).
 struct LevelTable {
     uint32_t start[kMaxDepth + 3];
 };
 __global__ __launch_bounds__(256) void k_node_ext(const NodeRec *__restrict__ nodes, const uint64_t *__restrict__ nkey,
                                                  const uint32_t *__restrict__ child, const float *__restrict__ amax,
                                                  uint32_t n_total, int encoder, LevelTable lt, NodeExt *__restrict__ ext) {
-    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t u = t >> 3;
     if (u >= n_total) return;
     const uint32_t fc = child[u];
     if (fc == 0) return;
     uint32_t level = 0;
     while (level + 1 < kMaxDepth + 2 && lt.start[level + 1] <= u) level++;
-    const NodeExt e = make_node_ext(nodes, nkey, child, amax, fc, level, encoder);
-    ext[(fc - 1) >> 3] = e;
+    node_ext_child(nodes, nkey, child, amax, fc, (uint32_t)(t & 7u), level, encoder, ext + ((fc - 1) >> 3));
 }
 
 template <class T>
@@ -202,7 +202,7 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t M, std::string &err) {
     if (n_internal) {
         LevelTable lt;
         for (uint32_t l = 0; l < kMaxDepth + 3; l++) lt.start[l] = ctx->level_start[l];
-        unsigned nb = (unsigned)((n_total + 255) / 256);
+        unsigned nb = (unsigned)(((size_t)n_total * 8 + 255) / 256);
         k_node_ext<<<nb, 256, 0, ctx->stream>>>(ctx->nodes.p, ctx->nkey.p, ctx->child.p, ctx->amax.p, n_total, ctx->encoder, lt,
                                                 ctx->next.p);
     }

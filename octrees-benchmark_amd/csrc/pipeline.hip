@@ -364,12 +364,14 @@ __global__ __launch_bounds__(256) void k_bp_node_ext(const NodeRec *__restrict__
                                                      const uint32_t *__restrict__ lv, const uint32_t *__restrict__ ncloud,
                                                      const CloudGeom *__restrict__ geom, NodeExt *__restrict__ ext) {
     const uint32_t n_total = lv[kLvCount - 1];
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_total; u += gridDim.x * blockDim.x) {
+    const uint32_t k = threadIdx.x & 7u;  // child of the node: 8 lanes per node
+    const uint32_t groups = (gridDim.x * blockDim.x) >> 3;
+    for (uint32_t u = (blockIdx.x * blockDim.x + threadIdx.x) >> 3; u < n_total; u += groups) {
         const uint32_t fc = child[u];
         if (fc == 0) continue;
         uint32_t level = 0;
         while (level + 1 < kLvCount && lv[level + 1] <= u) level++;
-        ext[(fc - 1) >> 3] = make_node_ext(nodes, nkey, child, amax, fc, level, (int)geom[ncloud[u]].encoder);
+        node_ext_child(nodes, nkey, child, amax, fc, k, level, (int)geom[ncloud[u]].encoder, ext + ((fc - 1) >> 3));
     }
 }
 

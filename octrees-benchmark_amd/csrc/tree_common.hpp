@@ -63,31 +63,30 @@ __device__ inline float recentre_leaf(const double4 *__restrict__ p64, const Nod
 }
 constexpr int kRecentreLanes = 16;  // lanes per leaf (leaves hold at most maxLeaf points, ~30 on average)
 
-// Children of internal node (first child fc, level `level`) regrouped by corner bits. A child's bits
-// come from its decoded key at level + 1, so they are exact also on degenerate axes where two
-// children share a centre value.
-__device__ inline NodeExt make_node_ext(const NodeRec *__restrict__ nodes, const uint64_t *__restrict__ nkey,
-                                        const uint32_t *__restrict__ child, const float *__restrict__ amax, uint32_t fc,
-                                        uint32_t level, int encoder) {
+// Children of internal node (first child fc, level `level`) regrouped by corner bits, one child per
+// lane of a group of 8 (lane k takes child fc + k and writes its entries of the node's record *e).
+// A child's bits come from its decoded key at level + 1, so they are exact also on degenerate axes
+// where two children share a centre value; the 4 children on one side of an axis share that centre
+// (the same corner coordinate), so their writes of it to e->ax are the same value.
+__device__ inline void node_ext_child(const NodeRec *__restrict__ nodes, const uint64_t *__restrict__ nkey,
+                                      const uint32_t *__restrict__ child, const float *__restrict__ amax, uint32_t fc,
+                                      uint32_t k, uint32_t level, int encoder, NodeExt *__restrict__ e) {
     const uint32_t sh = kMaxDepth - (level + 1);  // corner bit of the children's level
-    NodeExt e;
-    for (int k = 0; k < 4; k++) e.pad[k] = 0;
-    for (uint32_t k = 0; k < 8; k++) {
-        const uint32_t v = fc + k;
-        uint32_t x, y, z;
-        if (encoder == OCT_ENC_HILBERT3D)
-            hilbert_decode(nkey[v], x, y, z);
-        else
-            morton_decode(nkey[v], x, y, z);
-        const uint32_t bx = (x >> sh) & 1u, by = (y >> sh) & 1u, bz = (z >> sh) & 1u;
-        const uint32_t c = bx | (by << 1) | (bz << 2);
-        const NodeRec r = nodes[v];
-        e.ax[bx] = r.cx, e.ax[2 + by] = r.cy, e.ax[4 + bz] = r.cz;
-        e.first[c] = r.first, e.last[c] = r.last;
-        e.gc[c] = child[v];
-        e.amax[c] = child[v] == 0 ? amax[v] : 0.0f;
-    }
-    return e;
+    const uint32_t v = fc + k;
+    uint32_t x, y, z;
+    if (encoder == OCT_ENC_HILBERT3D)
+        hilbert_decode(nkey[v], x, y, z);
+    else
+        morton_decode(nkey[v], x, y, z);
+    const uint32_t bx = (x >> sh) & 1u, by = (y >> sh) & 1u, bz = (z >> sh) & 1u;
+    const uint32_t c = bx | (by << 1) | (bz << 2);
+    const NodeRec r = nodes[v];
+    const uint32_t gc = child[v];
+    e->ax[bx] = r.cx, e->ax[2 + by] = r.cy, e->ax[4 + bz] = r.cz;
+    e->first[c] = r.first, e->last[c] = r.last;
+    e->gc[c] = gc;
+    e->amax[c] = gc == 0 ? amax[v] : 0.0f;
+    if (k < 4) e->pad[k] = 0;
 }
 
 }  // namespace oct
