@@ -36,32 +36,35 @@ __global__ void k_root(NodeRec *nodes, uint64_t *nkey, uint32_t *flags, uint32_t
     }
 }
 
-// One thread per node of level `level` (indices [ls, le)). Flagged nodes write their 8 children.
+// Nodes of level `level` (indices [ls, le)), 8 lanes each. Flagged nodes write their 8 children.
 __global__ __launch_bounds__(256) void k_children(const uint64_t *__restrict__ keys, NodeRec *__restrict__ nodes,
                                                  uint64_t *__restrict__ nkey, uint32_t *__restrict__ child,
                                                  const uint32_t *__restrict__ flags, const uint32_t *__restrict__ rank,
                                                  uint32_t *__restrict__ next_flags, uint32_t ls, uint32_t le, uint32_t level,
                                                  uint32_t max_leaf, BuildGeom g) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // 8 lanes per node (lane k: child k); a node's lanes leave or stay together
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = t >> 3;
+    const uint32_t k = (uint32_t)(t & 7u);
     if (i >= le - ls) return;
-    uint32_t u = ls + i;
+    const uint32_t u = ls + (uint32_t)i;
     if (!flags[i]) {
-        child[u] = 0;
+        if (k == 0) child[u] = 0;
         return;
     }
     const uint32_t cb = le + 8u * rank[i];
-    child[u] = cb;
+    if (k == 0) child[u] = cb;
     const NodeRec pr = nodes[u];
     const uint64_t pkey = nkey[u];
     const uint32_t cl = level + 1;
     const uint64_t step = node_range(cl);
-    uint32_t lo = pr.first;
-    for (uint32_t c = 0; c < 8; c++) {
-        uint32_t hi = (c == 7) ? pr.last : lower_bound_keys(keys, lo, pr.last, pkey + (uint64_t)(c + 1) * step);
-        write_node(nodes, nkey, cb + c, pkey + (uint64_t)c * step, cl, lo, hi, g);
-        next_flags[cb - le + c] = (hi - lo > max_leaf && cl < kMaxDepth) ? 1u : 0u;
-        lo = hi;
-    }
+    // child k ends where the keys reach child k + 1's prefix; it starts where child k - 1 ends (the
+    // search over the whole parent range finds the same index as one starting at child k's start)
+    const uint32_t hi = (k == 7) ? pr.last : lower_bound_keys(keys, pr.first, pr.last, pkey + (uint64_t)(k + 1) * step);
+    const uint32_t prev = __shfl_up(hi, 1, 8);
+    const uint32_t lo = (k == 0) ? pr.first : prev;
+    write_node(nodes, nkey, cb + k, pkey + (uint64_t)k * step, cl, lo, hi, g);
+    next_flags[cb - le + k] = (hi - lo > max_leaf && cl < kMaxDepth) ? 1u : 0u;
 }
 
 // Leaf-recentred FP32 coordinates, kRecentreLanes lanes per node (recentre_leaf); internal nodes'
@@ -167,7 +170,7 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t M, std::string &err) {
         if ((e = grow_preserve(ctx, ctx->child, newle + 1, le)) != hipSuccess) return e;
         if ((e = grow_preserve(ctx, ctx->flags, newle, le)) != hipSuccess) return e;
         if ((e = grow_preserve(ctx, ctx->nrank, newle, le)) != hipSuccess) return e;
-        unsigned nb = (cnt + 255) / 256;
+        unsigned nb = (unsigned)(((size_t)cnt * 8 + 255) / 256);
         k_children<<<nb, 256, 0, ctx->stream>>>(ctx->sorted_keys, ctx->nodes.p, ctx->nkey.p, ctx->child.p, ctx->flags.p + ls,
                                                 ctx->nrank.p + ls, ctx->flags.p + le, ls, le, level, M, g);
         if ((e = hipGetLastError()) != hipSuccess) return e;
