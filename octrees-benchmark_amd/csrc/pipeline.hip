@@ -270,13 +270,15 @@ __global__ __launch_bounds__(kLevelBlocks) void k_bp_level_scan(uint32_t *__rest
 
 // Children of the split nodes of level l: block `blk` walks its slice in rounds of 256 nodes, ranks
 // the split flags (wave ballots + a block scan) on top of its exclusive block offset, and writes each
-// split node's 8 children (range, key, centre, flag, cloud) at level_start[l + 1] + 8 * rank.
+// split node's 8 children (range, key, centre, flag, cloud) at level_start[l + 1] + 8 * rank, one
+// lane per child.
 __global__ __launch_bounds__(kLevelThreads) void k_bp_level_children(
     const uint64_t *__restrict__ keys, NodeRec *__restrict__ nodes, uint64_t *__restrict__ nkey, uint32_t *__restrict__ child,
     uint32_t *__restrict__ flags, uint32_t *__restrict__ ncloud, const CloudGeom *__restrict__ geom,
     const uint32_t *__restrict__ lv, uint32_t l, const uint32_t *__restrict__ part, uint32_t *__restrict__ cl_internal,
     uint32_t *__restrict__ cl_depth) {
     __shared__ uint32_t s_w[kLevelThreads / 64];
+    __shared__ uint32_t s_u[kLevelThreads], s_cb[kLevelThreads], s_cid[kLevelThreads];  // a round's split nodes
     uint32_t b0, b1;
     level_slice(lv, l, blockIdx.x, b0, b1);
     const uint32_t le = lv[l + 1];
@@ -304,22 +306,28 @@ __global__ __launch_bounds__(kLevelThreads) void k_bp_level_children(
                 child[u] = 0;
             } else {
                 cid = ncloud[u];
-                const CloudGeom &g = geom[cid];
-                const uint32_t cb = le + 8u * (running + wpre + below);
+                const uint32_t j = wpre + below;  // the split node's slot in this round
+                const uint32_t cb = le + 8u * (running + j);
                 child[u] = cb;
-                const NodeRec pr = nodes[u];
-                const uint64_t pkey = nkey[u];
-                const uint64_t step = node_range(cl);
-                uint32_t lo = pr.first;
-                for (uint32_t c = 0; c < 8; c++) {
-                    const uint32_t hi = (c == 7) ? pr.last : lower_bound_keys(keys, lo, pr.last, pkey + (uint64_t)(c + 1) * step);
-                    put_node(nodes, nkey, cb + c, pkey + (uint64_t)c * step, cl, lo, hi, (int)g.encoder, g.boxmin, g.hl,
-                             g.rad[cl]);
-                    flags[cb + c] = (hi - lo > g.max_leaf && cl < kMaxDepth) ? 1u : 0u;
-                    ncloud[cb + c] = cid;
-                    lo = hi;
-                }
+                s_u[j] = u, s_cb[j] = cb, s_cid[j] = cid;
             }
+        }
+        __syncthreads();
+        // the round's split nodes, 8 lanes each: lane ck finds child ck's end and takes its start from
+        // lane ck - 1 (a search over the whole parent range finds the same index as one from the start)
+        for (uint32_t q = threadIdx.x >> 3; q < tot; q += blockDim.x >> 3) {
+            const uint32_t ck = threadIdx.x & 7u;
+            const uint32_t pu = s_u[q], cb = s_cb[q], cc = s_cid[q];
+            const CloudGeom &g = geom[cc];
+            const NodeRec pr = nodes[pu];
+            const uint64_t pkey = nkey[pu];
+            const uint64_t step = node_range(cl);
+            const uint32_t hi = (ck == 7) ? pr.last : lower_bound_keys(keys, pr.first, pr.last, pkey + (uint64_t)(ck + 1) * step);
+            const uint32_t prev = __shfl_up(hi, 1, 8);
+            const uint32_t lo = (ck == 0) ? pr.first : prev;
+            put_node(nodes, nkey, cb + ck, pkey + (uint64_t)ck * step, cl, lo, hi, (int)g.encoder, g.boxmin, g.hl, g.rad[cl]);
+            flags[cb + ck] = (hi - lo > g.max_leaf && cl < kMaxDepth) ? 1u : 0u;
+            ncloud[cb + ck] = cc;
         }
         // per-cloud internal-node counts and depth: one atomic per wave when its split nodes share a cloud
         if (m) {
@@ -337,6 +345,7 @@ __global__ __launch_bounds__(kLevelThreads) void k_bp_level_children(
             }
         }
         running += tot;
+        __syncthreads();  // the next round rewrites the slot lists
     }
 }
 
