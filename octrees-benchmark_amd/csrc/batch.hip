@@ -450,8 +450,8 @@ static void knn_shape(oct_batch *b) {
 }
 
 // the search pass: radius queries grouped and counted (work-queue count kernel), kNN queries grouped
-// and searched per size class (work-queue kNN kernel), kNN indices made cloud-local. only_class >= 0:
-// just that kNN class (autotuning), timed with ev0 / ev1 when given.
+// and searched per size class (work-queue kNN kernel, which writes indices local to each query's
+// cloud). only_class >= 0: just that kNN class (autotuning), timed with ev0 / ev1 when given.
 static hipError_t enqueue_search(oct_batch *b, int kernel, int algo, int precision, std::string &err, int only_class = -1,
                                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     oct_ctx *f = b->forest;
@@ -478,12 +478,11 @@ static hipError_t enqueue_search(oct_batch *b, int kernel, int algo, int precisi
             L.idx = b->p_kidx.p, L.d2 = b->p_kd2.p;
             L.tile_info = b->kpass.Q.tile_info[c], L.roots = b->roots.p, L.radtab = b->radtab.p, L.magtab = b->magtab.p;
             L.queue_tiles = b->kpass.Q.ctrl + kQCtrlTiles + c, L.queue_next = b->kpass.Q.ctrl + kQCtrlQueue + c;
+            if (only_class < 0) L.cloud_off = b->cloud_off.p;  // indices local to each query's cloud
             if (ev0) (void)hipEventRecord(ev0, s);
             if ((e = launch_knn(f, L, err)) != hipSuccess) return e;
             if (ev1) (void)hipEventRecord(ev1, s);
         }
-        if (only_class < 0 && (e = enqueue_knn_local(s, b->p_kidx.p, b->kpass.n, b->p_kmax, b->kpass.Q.cloud, b->cloud_off.p)) != hipSuccess)
-            return e;
     }
     return hipGetLastError();
 }

@@ -226,6 +226,9 @@ struct KnnLaunch {
     const double *magtab = nullptr;
     const uint32_t *kq = nullptr;
     uint32_t kclass = 0;  // the largest k of the launch when kq is set (0: k)
+    // forests: per-cloud start in the concatenated sorted clouds; set, indices are written local to
+    // each query's cloud (null: forest-wide indices)
+    const uint32_t *cloud_off = nullptr;
     // work-queue form: device-resident wave-tile count and a zeroed tile counter
     const uint32_t *queue_tiles = nullptr;
     uint32_t *queue_next = nullptr;

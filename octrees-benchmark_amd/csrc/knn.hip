@@ -65,6 +65,7 @@ struct KnnArgs {
     const uint32_t *__restrict__ tile_info;
     const uint32_t *__restrict__ roots;
     const uint32_t *__restrict__ kq;  // per-query k (null: k for every query); k is the row width
+    const uint32_t *__restrict__ cloud_off;  // set: indices written local to the tile's cloud
     // work-queue form: device-resident wave-tile count and the zeroed counter waves take tiles from
     const uint32_t *__restrict__ queue_tiles;
     uint32_t *__restrict__ queue_next;
@@ -406,9 +407,10 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
     select();
 
     if (valid) {
+        const uint32_t off = a.cloud_off ? a.cloud_off[cloud] : 0u;  // the tile's cloud holds every neighbour
         for (uint32_t t = g; t < kw; t += S) {
             const bool have = t < cnt;
-            a.idx_out[(size_t)qid * kw + t] = have ? (int32_t)bi[t] : -1;
+            a.idx_out[(size_t)qid * kw + t] = have ? (int32_t)(bi[t] - off) : -1;
             if (a.d2_out) a.d2_out[(size_t)qid * kw + t] = have ? bd[t] : __builtin_nan("");
             if (a.d2f_out) a.d2f_out[(size_t)qid * kw + t] = have ? (float)bd[t] : __builtin_nanf("");
         }
@@ -559,6 +561,7 @@ hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err) {
     a.tile_info = L.tile_info;
     a.roots = L.roots;
     a.kq = L.kq;
+    a.cloud_off = L.cloud_off;
     a.queue_tiles = L.queue_tiles;
     a.queue_next = L.queue_next;
     if (L.radtab) a.radtab = L.radtab, a.magtab = L.magtab;

@@ -496,15 +496,6 @@ __global__ __launch_bounds__(256) void k_q_tiles(const uint32_t *__restrict__ sv
     }
 }
 
-__global__ void k_knn_local(int32_t *__restrict__ idx, size_t rows, uint32_t kmax, const uint32_t *__restrict__ cloud,
-                            const uint32_t *__restrict__ cloud_off) {
-    const size_t total = rows * kmax;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int32_t v = idx[e];
-        if (v >= 0) idx[e] = v - (int32_t)cloud_off[cloud[e / kmax]];
-    }
-}
-
 static inline unsigned grid_for(size_t n, unsigned per, unsigned cap) {
     size_t b = (n + per - 1) / per;
     if (b < 1) b = 1;
@@ -569,13 +560,6 @@ hipError_t enqueue_query_groups(hipStream_t s, const QueryGroups &Q, uint32_t C,
     }
     k_q_tile_offsets<<<Q.nclass, 1024, 0, s>>>(Q.gstart, C, sh, Q.toff, Q.ctrl + kQCtrlTiles);
     k_q_tiles<<<dim3(grid_for(maxslots, 256, 4096), Q.nclass), 256, 0, s>>>(sv, Q.gstart, Q.toff, Q.ctrl + kQCtrlTiles, C, sh);
-    return hipGetLastError();
-}
-
-hipError_t enqueue_knn_local(hipStream_t s, int32_t *idx, size_t rows, uint32_t kmax, const uint32_t *cloud,
-                             const uint32_t *cloud_off) {
-    if (rows == 0) return hipSuccess;
-    k_knn_local<<<grid_for(rows * kmax, 256, 8192), 256, 0, s>>>(idx, rows, kmax, cloud, cloud_off);
     return hipGetLastError();
 }
 

@@ -101,9 +101,6 @@ struct QueryGroups {
 // the slot capacity a class needs whatever the queries: (ceil(n / G) + C) tiles of G slots
 inline uint32_t group_slots(uint32_t n, uint32_t C, uint32_t G) { return ((n + G - 1) / G + C) * G; }
 hipError_t enqueue_query_groups(hipStream_t s, const QueryGroups &Q, uint32_t C, const uint32_t *cloud_off);
-// kNN rows of forest indices -> indices in each query's own cloud (entries < 0 stay)
-hipError_t enqueue_knn_local(hipStream_t s, int32_t *idx, size_t rows, uint32_t kmax, const uint32_t *cloud,
-                             const uint32_t *cloud_off);
 // k -> size class (16 << class >= k)
 __host__ __device__ inline uint32_t knn_class_of(uint32_t k) {
     uint32_t c = 0;
