@@ -4,7 +4,7 @@
  *
  * Each entry point replaces one piece of the reference's header-only C++ API
  * (derivada/octrees-benchmark; paths relative to its root). The drop-in C++ facade
- * (octrees-benchmark_amd/facade/octree/ headers) keeps the reference's class names and signatures and
+ * (headers under octrees-benchmark_amd/facade/, laid out as the reference's inc/) keeps the reference's class names and signatures and
  * forwards to these calls.
  *
  * Conventions

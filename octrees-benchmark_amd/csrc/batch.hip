@@ -152,6 +152,13 @@ static int bfail(oct_batch *b, int code, const std::string &msg) {
         if (r_ != OCT_OK) return bfail(b, r_, std::string(what) + ": " + oct_last_error(ctx_for_error)); \
     } while (0)
 
+// Synchronises the forest's stream and fails on a set device error word (a search kernel's full
+// traversal stack).
+static int forest_check(oct_batch *b) {
+    const int r = oct_synchronize(b->forest);
+    return r ? bfail(b, r, std::string("search: ") + oct_last_error(b->forest)) : OCT_OK;
+}
+
 static void drop_graph(oct_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -544,6 +551,8 @@ static int run_device(oct_batch *b, int use_graph, bool search, int kernel, int 
         BCHK(hipMemcpyAsync(cint.data(), b->cl_internal.p, C * 4, hipMemcpyDeviceToHost, s), "copy status");
         BCHK(hipMemcpyAsync(cdep.data(), b->cl_depth.p, C * 4, hipMemcpyDeviceToHost, s), "copy status");
         BCHK(hipStreamSynchronize(s), "device build");
+        if (search)
+            if (int fr = forest_check(b)) return fr;
         if (ctrl[kCtrlSortErr] || rctrl[kCtrlSortErr] || kctrl[kCtrlSortErr])
             return bfail(b, OCT_ERR_STATE, "sort: look-back wait exceeded its bound");
         if (ctrl[kCtrlBuildErr] & kBuildErrDepth) return bfail(b, OCT_ERR_STATE, "build: split below the maximum depth");
@@ -908,6 +917,7 @@ int oct_batch_radius(oct_batch *b, int kernel, int algo, int precision, const ui
     BCHK(launch_radius(f, L), "batch radius");
     BCHK(hipMemcpyAsync(counts_out, b->counts.p, nq * 4, hipMemcpyDeviceToHost, f->stream), "copy counts");
     BCHK(hipStreamSynchronize(f->stream), "batch radius");
+    if (int fr = forest_check(b)) return fr;
     if (stats) return batch_stats(b, stats, ST_SEARCH);
     return OCT_OK;
 }
@@ -975,6 +985,7 @@ int oct_batch_knn(oct_batch *b, int precision, const uint32_t *cloud, const uint
     BCHK(hipMemcpyAsync(idx_out, b->kidx.p, nq * kmax * 4, hipMemcpyDeviceToHost, f->stream), "copy idx");
     if (d2_out) BCHK(hipMemcpyAsync(d2_out, b->kd2.p, nq * kmax * 8, hipMemcpyDeviceToHost, f->stream), "copy d2");
     BCHK(hipStreamSynchronize(f->stream), "batch knn");
+    if (int fr = forest_check(b)) return fr;
     // forest indices -> the clouds' own sorted indices
     for (size_t i = 0; i < nq; i++) {
         const int32_t off = (int32_t)b->clouds[cloud[i]].pt_off;

@@ -42,6 +42,19 @@ int hfail(oct_ctx *ctx, hipError_t e, const char *where, const std::string &extr
 
 bool valid_encoder(int e) { return e == OCT_ENC_MORTON3D || e == OCT_ENC_HILBERT3D; }
 
+// Reads the device error word after the stream's work so far (synchronises); a set bit fails the
+// call with OCT_ERR_STATE and the word is cleared for the next call.
+int check_device_error(oct_ctx *ctx) {
+    uint32_t w = 0;
+    HCHK(hipMemcpyAsync(&w, ctx->derr.p, 4, hipMemcpyDeviceToHost, ctx->stream), "copy error word");
+    HCHK(hipStreamSynchronize(ctx->stream), "copy error word");
+    if (!w) return OCT_OK;
+    HCHK(hipMemsetAsync(ctx->derr.p, 0, 4, ctx->stream), "clear error word");
+    HCHK(hipStreamSynchronize(ctx->stream), "clear error word");
+    if (w & kErrStack) return fail(ctx, OCT_ERR_STATE, "traversal stack full: a search dropped a pending node");
+    return fail(ctx, OCT_ERR_STATE, "device error word set");
+}
+
 __global__ void k_pad(const double *__restrict__ xyz, double4 *__restrict__ out, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0);
@@ -154,6 +167,11 @@ int oct_create(oct_ctx **out, int device) {
             return OCT_ERR_HIP;
         }
     }
+    if (ctx->derr.reserve(4) != hipSuccess || hipMemset(ctx->derr.p, 0, 4 * sizeof(uint32_t)) != hipSuccess) {
+        ctx->derr.release();
+        delete ctx;
+        return OCT_ERR_HIP;
+    }
     *out = ctx;
     return OCT_OK;
 }
@@ -173,7 +191,7 @@ int oct_destroy(oct_ctx *ctx) {
     ctx->entries.release(), ctx->stats.release(), ctx->kidx.release(), ctx->kd2.release();
     ctx->next.release(), ctx->qrad3.release(), ctx->kc_xyz.release(), ctx->kc_keys.release();
     ctx->kd2f.release(), ctx->radtab.release(), ctx->las_rec.release(), ctx->las_meta.release();
-    ctx->rec_in.release(), ctx->rec_out.release();
+    ctx->rec_in.release(), ctx->rec_out.release(), ctx->derr.release();
     for (int s = 0; s < ST_N; s++) {
         if (ctx->ev0[s]) (void)hipEventDestroy(ctx->ev0[s]);
         if (ctx->ev1[s]) (void)hipEventDestroy(ctx->ev1[s]);
@@ -188,7 +206,7 @@ const char *oct_last_error(const oct_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int oct_synchronize(oct_ctx *ctx) {
     if (!ctx) return OCT_ERR_ARG;
     HCHK(hipStreamSynchronize(ctx->stream), "synchronize");
-    return OCT_OK;
+    return check_device_error(ctx);
 }
 
 int oct_set_points(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_bytes) {
@@ -490,6 +508,7 @@ static int radius_impl(oct_ctx *ctx, int kernel, int algo, int precision, const 
     ctx->last_algo = algo;
     if (counts_out && nq) HCHK(hipMemcpyAsync(counts_out, ctx->counts.p, nq * 4, hipMemcpyDeviceToHost, ctx->stream), "copy counts");
     HCHK(hipStreamSynchronize(ctx->stream), "radius");
+    if ((s = check_device_error(ctx))) return s;
     if (stats) return read_stats(ctx, stats, ms);
     return OCT_OK;
 }
@@ -599,6 +618,7 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
     ctx->has_lists = false;
     if (stats) {
         HCHK(hipStreamSynchronize(ctx->stream), "radius");
+        if ((s = check_device_error(ctx))) return s;
         return read_stats(ctx, stats, stage_ms(ctx, ST_SEARCH));
     }
     return OCT_OK;
@@ -746,6 +766,7 @@ int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *
     if (idx_out && nq) HCHK(hipMemcpyAsync(idx_out, ctx->kidx.p, nq * k * 4, hipMemcpyDeviceToHost, ctx->stream), "copy idx");
     if (d2_out && nq) HCHK(hipMemcpyAsync(d2_out, ctx->kd2.p, nq * k * 8, hipMemcpyDeviceToHost, ctx->stream), "copy d2");
     HCHK(hipStreamSynchronize(ctx->stream), "knn");
+    if (int de = check_device_error(ctx)) return de;
     if (stats) return read_stats(ctx, stats, stage_ms(ctx, ST_KNN));
     return OCT_OK;
 }
@@ -785,6 +806,7 @@ int oct_knn_dev(oct_ctx *ctx, int precision, const uint32_t *center_idx_dev, siz
     if (keep) ctx->knn_nq = nq, ctx->knn_k = k;
     if (stats) {
         HCHK(hipStreamSynchronize(ctx->stream), "knn");
+        if (int de = check_device_error(ctx)) return de;
         return read_stats(ctx, stats, stage_ms(ctx, ST_KNN));
     }
     return OCT_OK;
@@ -798,7 +820,7 @@ int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d
     if (idx_out && cnt) HCHK(hipMemcpyAsync(idx_out, ctx->kidx.p + off, cnt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy idx");
     if (d2_out && cnt) HCHK(hipMemcpyAsync(d2_out, ctx->kd2f.p + off, cnt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy d2");
     HCHK(hipStreamSynchronize(ctx->stream), "knn fetch");
-    return OCT_OK;
+    return check_device_error(ctx);
 }
 
 int oct_autotune_knn(oct_ctx *ctx, int precision, uint32_t k, size_t sample_nq, int *best, double ms_out[3]) {

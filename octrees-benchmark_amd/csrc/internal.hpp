@@ -69,6 +69,10 @@ struct SortTile {
 // Per-cloud bin bases of a segmented sort: bases[cloud * kSegBaseStride + pass * 256 + digit].
 constexpr uint32_t kSegBaseStride = kRadixPassCount * 256;
 
+// Bits of the context's device error word (oct_ctx::derr): kernels set them with atomicOr, the host
+// reads and clears the word at its next synchronisation and fails the call.
+constexpr uint32_t kErrStack = 1u;  // a traversal stack was full: a pending node would have been dropped
+
 enum Stage { ST_BBOX = 0, ST_ENCODE, ST_SORT, ST_GATHER, ST_BUILD, ST_SEARCH, ST_KNN, ST_N };
 
 }  // namespace oct
@@ -152,6 +156,8 @@ struct oct_ctx {
     uint32_t knn_k = 0;
     int knn_lanes_override = -1;    // OCT_TUNE_KNN_LANES (-1: tuned / default)
     int knn_lanes_tuned[2][8];      // autotuned lanes per query per (precision, size class), -1 = not tuned
+
+    oct::DevBuf<uint32_t> derr;     // device error word (kErr* bits), zero when no kernel failed
 
     hipEvent_t ev0[oct::ST_N] = {}, ev1[oct::ST_N] = {};
     bool stage_events_off = false;  // launch sequences captured in graphs record no stage events

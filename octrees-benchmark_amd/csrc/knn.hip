@@ -57,6 +57,7 @@ struct KnnArgs {
     double *__restrict__ d2_out;
     float *__restrict__ d2f_out;
     unsigned long long *__restrict__ stats;
+    uint32_t *__restrict__ err;  // device error word: kErrStack when the traversal stack was full
     // tree geometry per cloud: level radii radtab[cloud * 66 + 3 * level + axis] and the box magnitude
     // magtab[cloud * 3 + axis] = |centre| + radius; for a forest of clouds, tile_info[wave tile] =
     // cloud | valid slots << 20 and roots[cloud] (null: one cloud, root node 0)
@@ -336,10 +337,13 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
     };
 
     // reference query for the child order: the middle valid query of the wave
-    const uint32_t nvalid = a.nq - slot0 < (uint32_t)G ? a.nq - slot0 : (uint32_t)G;
+    // (a forest's wave tile holds nvt valid slots: the rest are padding whose centre is not a query)
+    uint32_t nvalid = a.nq - slot0 < (uint32_t)G ? a.nq - slot0 : (uint32_t)G;
+    if (nvt < nvalid) nvalid = nvt;
     const int refl = (int)(S * (nvalid / 2));
 
     uint32_t sp = 0;
+    bool overflow = false;  // wave-uniform: a push found the stack full (cannot happen within 21 levels)
     {
         const uint32_t c0 = uniform_u32(a.child[root]);
         if (c0 == 0) {
@@ -395,16 +399,21 @@ __device__ __attribute__((always_inline)) inline void knn_tile(const KnnArgs &a,
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             const uint32_t c = ((kOrder >> (4 * t)) & 7u) ^ qc;
-            if (((needbits >> c) & 1u) && sp < (uint32_t)kKnnStack) {
-                const uint32_t gc = E.gc[c];
-                s_node[sp] = gc ? (gc - 1) >> 3 : ent * 8 + c;
-                s_lev[sp] = (uint8_t)(gc ? d : (d | 0x80u));
-                sp++;
+            if ((needbits >> c) & 1u) {
+                if (sp < (uint32_t)kKnnStack) {
+                    const uint32_t gc = E.gc[c];
+                    s_node[sp] = gc ? (gc - 1) >> 3 : ent * 8 + c;
+                    s_lev[sp] = (uint8_t)(gc ? d : (d | 0x80u));
+                    sp++;
+                } else {
+                    overflow = true;
+                }
             }
         }
         wsync();
     }
     select();
+    if (overflow && lane == 0) atomicOr(a.err, kErrStack);
 
     if (valid) {
         const uint32_t off = a.cloud_off ? a.cloud_off[cloud] : 0u;  // the tile's cloud holds every neighbour
@@ -556,6 +565,7 @@ hipError_t launch_knn(oct_ctx *ctx, const KnnLaunch &L, std::string &err) {
     a.d2_out = L.d2;
     a.d2f_out = L.d2f;
     a.stats = L.stats;
+    a.err = ctx->derr.p;
     a.radtab = ctx->radtab.p;
     a.magtab = ctx->radtab.p + 66;  // the context's magnitudes follow its level radii
     a.tile_info = L.tile_info;

@@ -238,6 +238,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
 
     // ---- root (singleTraversal, linear_octree.hpp:655-662)
     uint32_t sp = 0;
+    bool overflow = false;  // wave-uniform: a push found the stack full (cannot happen within 21 levels)
     {
         const NodeRec nr = a.nodes[0];
         const uint64_t m0 = ballot64(valid);
@@ -267,10 +268,11 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
             const uint32_t c = first_child + oct;
             const NodeRec nr = a.nodes[c];
             // An empty child cannot change counts or index lists whatever its class; only Struct
-            // range lists keep empty INSIDE ranges (neighbor_set.hpp:84-89).
+            // range lists keep empty INSIDE ranges (neighbor_set.hpp:84-89). It still counts as
+            // classified: the reference's traversal evaluates its criterion.
+            n_class += __popcll(m);
             if (!((ALGO == OCT_ALGO_STRUCT || ALGO == OCT_ALGO_APPROX) && MODE != 0) && uniform_u32(nr.first) == uniform_u32(nr.last))
                 continue;
-            n_class += __popcll(m);
             n_cslots += 64;
             const int j = judge(m, nr, d);
             if (j == J_INSIDE) sink.range(nr.first, nr.last);
@@ -282,11 +284,14 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
                 } else if (sp < kStack) {
                     s_node[w][sp] = c, s_mask[w][sp] = ov, s_depth[w][sp] = (uint8_t)d;
                     sp++;
+                } else {
+                    overflow = true;
                 }
             }
         }
     }
 
+    if (overflow && lane == 0) atomicOr(a.err, kErrStack);
     if (valid) {
         if (MODE != 2) {
             if (a.counts) a.counts[qid] = sink.cnt;
@@ -795,6 +800,7 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
     // Stack entries hold the FIRST CHILD of an OVERLAP internal node; its NodeExt record is one
     // uniform load.
     uint32_t sp = 0;
+    bool overflow = false;  // wave-uniform: a push found the stack full (cannot happen within 21 levels)
     {
         const NodeRec nr = a.nodes[root];
         const int jr = judge(valid, nr, 0);
@@ -840,6 +846,8 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
                 } else if (sp < kStack) {
                     s_node[w][sp] = gc, s_mask[w][sp] = ov, s_depth[w][sp] = (uint8_t)d;
                     sp++;
+                } else {
+                    overflow = true;
                 }
             }
         }
@@ -875,6 +883,7 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
         }
         cnt += tot;
     }
+    if (overflow && lane == 0) atomicOr(a.err, kErrStack);
     if (valid && g == 0 && a.counts) a.counts[qid] = cnt;
     if (STATS) {
         if (lane == 0) {
@@ -1032,6 +1041,7 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     a.offsets = L.offsets;
     a.entries = L.entries;
     a.stats = L.stats;
+    a.err = ctx->derr.p;
     for (int l = 0; l < 22; l++)
         for (int k = 0; k < 3; k++) a.rad[l][k] = ctx->level_radii[l][k];
     const size_t tiles = (L.nq + 63) / 64;

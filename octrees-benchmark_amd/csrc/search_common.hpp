@@ -48,6 +48,7 @@ struct SearchArgs {
     const uint64_t *__restrict__ offsets;
     uint32_t *__restrict__ entries;
     unsigned long long *__restrict__ stats;
+    uint32_t *__restrict__ err;  // device error word: kErrStack when a traversal stack was full
     double rad[22][3];
 };
 

@@ -209,6 +209,53 @@ def random_indices(nq, n, seed=42):
     return out
 
 
+class BatchWorkload:
+    """The batched workload: `clouds` independent clouds with sizes log-uniform in [1e4, 4e6], rescaled
+    to ~`total` points; odd clouds uniform boxes (5 pts/m^3 shape, z extent a fifth), even clouds
+    airborne tiles at 50 pts/m^2; each at a random origin in [-1e5, 1e5]. `per` centres per cloud
+    drawn uniformly from its points (sorted-order indices), a radius per query uniform in [0.5, 5] m
+    rounded to float, a fraction `knn_frac` of the queries kNN with k uniform in {8..256}.
+    Deterministic in `seed`; clouds are generated on demand by cloud(i)."""
+
+    def __init__(self, seed=4, clouds=256, total=200e6, per=50_000, knn_frac=0.2):
+        rng = np.random.default_rng(seed)
+        C_ = clouds
+        sizes = np.exp(rng.uniform(np.log(1e4), np.log(4e6), C_))
+        self.sizes = np.maximum(1e4, sizes * (total / sizes.sum())).astype(np.int64)
+        self.offsets = [rng.uniform(-1e5, 1e5, 3) for _ in range(C_)]
+        self.per = per
+        self.cloud_of = np.repeat(np.arange(C_, dtype=np.uint32), per)
+        self.idx = (rng.uniform(0.0, 1.0, C_ * per) * np.repeat(self.sizes, per)).astype(np.uint32)
+        self.is_knn = rng.uniform(0.0, 1.0, C_ * per) < knn_frac
+        self.radii = rng.uniform(0.5, 5.0, C_ * per).astype(np.float32).astype(np.float64)
+        self.k = rng.integers(8, 257, C_ * per).astype(np.uint32)
+        self.rq = np.nonzero(~self.is_knn)[0]
+        self.kq = np.nonzero(self.is_knn)[0]
+
+    @property
+    def count(self):
+        return len(self.sizes)
+
+    @property
+    def points(self):
+        return int(self.sizes.sum())
+
+    def cloud(self, i):
+        n, off = int(self.sizes[i]), self.offsets[i]
+        if i % 2:
+            L = float(n) ** (1.0 / 3.0)
+            return uniform_box(n, seed=1000 + i, size=(L, L, L / 5.0), offset=tuple(off))
+        return lidar_tile(n, seed=1000 + i, extent=1000.0 * (n / 50e6) ** 0.5, offset=(off[0], off[1]))
+
+    def radius_queries(self):
+        """(cloud, index, radius) of the radius queries"""
+        return self.cloud_of[self.rq], self.idx[self.rq], self.radii[self.rq]
+
+    def knn_queries(self):
+        """(cloud, index, k) of the kNN queries"""
+        return self.cloud_of[self.kq], self.idx[self.kq], self.k[self.kq]
+
+
 # ------------------------------------------------------------------ context
 class Batch:
     """Many independent clouds: each is sorted and built as on its own, then all are searched together
@@ -590,14 +637,17 @@ class Octree:
                   "oct_knn")
         return oi, od, st.as_dict()
 
-    def knn_dev(self, k, precision=PREC_FP32, first=0, nq=None, stats=False):
+    def knn_dev(self, k, precision=PREC_FP32, first=0, nq=None, stats=False, idx_ptr=None, d2_ptr=None):
         """All-points kNN for sorted points first .. first + nq - 1; results (int32 indices, FP32 d^2)
-        stay on the device until knn_fetch."""
+        stay on the device until knn_fetch, or go to the caller's device buffers idx_ptr / d2_ptr
+        (addresses of nq x k int32 / float32 arrays on the context's device, both given)."""
         nq = self.n - first if nq is None else nq
         st = SearchStats() if stats else None
-        self._chk(self.L.oct_knn_dev(self.h, precision, None, first, nq, k, None, None, C.byref(st) if stats else None),
-                  "oct_knn_dev")
-        self._knn_shape = (nq, k)
+        own = idx_ptr is None or d2_ptr is None
+        self._chk(self.L.oct_knn_dev(self.h, precision, None, first, nq, k, None if own else idx_ptr,
+                                     None if own else d2_ptr, C.byref(st) if stats else None), "oct_knn_dev")
+        if own:
+            self._knn_shape = (nq, k)
         return st.as_dict() if stats else None
 
     def knn_fetch(self, q0=0, nq=None):

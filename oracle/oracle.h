@@ -25,10 +25,10 @@
  *                            brute-force recursion on small inputs
  *   orc_radius             : brute force over all points with the same predicate (equal except in
  *                            rounding-pathological node classifications), reference KATs
- *                            test_octree.cpp:147-164, test_octree_advanced.cpp:213-295,
+ *                            test_octree.cpp:63-80, test_octree_advanced.cpp:69-131,
  *                            test_complex.cpp:37-94
- *   orc_knn / orc_knn_brute: brute-force (d2, index) order; reference KATs test_octree.cpp:126-145,
- *                            test_octree_advanced.cpp:213-231, test_complex.cpp:144-172
+ *   orc_knn / orc_knn_brute: brute-force (d2, index) order; reference KATs test_octree.cpp:42-61,
+ *                            test_octree_advanced.cpp:33-67, test_complex.cpp:144-172
  *   orc_las_*              : LAS files written field by field from the ASPRS LAS 1.2 / 1.3 / 1.4
  *                            layouts by an independent writer (tests/las_files.py) with known
  *                            integers and attributes, and power-of-two scales whose coordinates have a

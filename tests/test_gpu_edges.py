@@ -52,13 +52,13 @@ def check_counts(ctx, kernel, algo, centers, radius, ref_sizes):
 OCT8 = [[0, 0, 0], [0.1, 0, 0], [0, 0.1, 0], [10, 10, 10], [10.1, 10, 10], [10, 10.1, 10], [5, 5, 5], [5, 5, 5.1]]
 
 
-def test_kat_square_radius(ctx):  # test_octree.cpp:147-164
+def test_kat_square_radius(ctx):  # test_octree.cpp:63-80
     sx, lists = run_both(ctx, OCT8, A.ENC_MORTON, 128, A.K_SQUARE, A.A_PRUNE, [[5, 5, 5]], 0.25)
     assert len(lists[0]) == 2
     assert sx[lists[0][0], 2] == 5.0 and sx[lists[0][1], 2] == 5.1
 
 
-def test_kat_sphere_and_cube(ctx):  # test_octree_advanced.cpp:267-295
+def test_kat_sphere_and_cube(ctx):  # test_octree_advanced.cpp:103-131
     _, l1 = run_both(ctx, [[1, 1, 1], [2, 0, 0]], A.ENC_MORTON, 128, A.K_SPHERE, A.A_PRUNE, [[0, 0, 0]], 1.5)
     assert len(l1[0]) == 0
     _, l2 = run_both(ctx, [[1, 1, 1], [2, 0, 0]], A.ENC_MORTON, 128, A.K_SPHERE, A.A_PRUNE, [[0, 0, 0]], 2.5)
@@ -67,7 +67,7 @@ def test_kat_sphere_and_cube(ctx):  # test_octree_advanced.cpp:267-295
     assert len(l3[0]) == 1 and sx[l3[0][0], 0] == 1.0
 
 
-def test_kat_zero_radius_and_boundary(ctx):  # test_octree_advanced.cpp:233-265
+def test_kat_zero_radius_and_boundary(ctx):  # test_octree_advanced.cpp:69-101
     sx, l = run_both(ctx, [[0, 0, 0], [0, 0, 0], [1, 1, 1]], A.ENC_MORTON, 128, A.K_SQUARE, A.A_PRUNE, [[0, 0, 0]], 0.0)
     assert len(l[0]) == 2
     _, l = run_both(ctx, [[0, 0, 0], [10, 10, 10], [5, 5, 5]], A.ENC_MORTON, 128, A.K_SQUARE, A.A_PRUNE,
