@@ -11,6 +11,7 @@ query sample on the host's cores.
 Prints ONE JSON line.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -27,36 +28,59 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 
 
+# sources that the all-points count kernel is compiled from: a committed counter profile of that
+# kernel applies to this build only when their digest matches
+SEARCH_SOURCES = ("search.hip", "search_common.hpp", "device.hpp", "tree_common.hpp", "internal.hpp")
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def source_digest(names=SEARCH_SOURCES):
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "octrees-benchmark_amd", "csrc")
+    for f in names:
+        h.update(f.encode())
+        h.update(open(os.path.join(d, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def cpu_model():
+    """The host CPU's model name, as lscpu reports it (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_of(fn, reps):
+    """One warm-up call, then the median wall time of `reps` timed calls (and the last result)."""
+    out = fn()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), out
+
+
 def sync_all(ctx):
     ctx.synchronize()
-    try:
-        import torch
-
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
 
 
 def profile(args):
     """Small fixed workloads for rocprofv3 runs (kernel statistics and counters)."""
     if args.profile == "copy":
-        import torch
-
-        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
-        dst = torch.empty_like(src)
+        ctx = A.Octree(0)
         for _ in range(3):
-            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            t0.record()
-            dst.copy_(src)
-            t1.record()
-            torch.cuda.synchronize()
-            ms = t0.elapsed_time(t1)
-            log(f"[profile] copy 1 GiB: {ms:.3f} ms = {2 * (1 << 30) / ms / 1e6:.0f} GB/s (read + write)")
+            gbs, ms = ctx.copy_bandwidth(1 << 30)
+            log(f"[profile] copy 1 GiB: {ms:.3f} ms = {gbs:.0f} GB/s (read + write)")
+        ctx.close()
         return
     n = args.n
     extent = args.extent if n == 50_000_000 else args.extent * (n / 50_000_000) ** 0.5
@@ -283,13 +307,16 @@ def main():
     ap.add_argument("--max-leaf", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--knn-k", type=int, default=10, help="k of the all-points kNN measured beside the search (0: skip)")
+    ap.add_argument("--knn-k", default="10,50,250,1000",
+                    help="k values of the all-points kNN measured beside the search, comma-separated (0: skip); "
+                         "k <= 50 runs every point, k = 250 the first 10 M and k = 1000 the first 2 M SFC positions")
+    ap.add_argument("--cpu-reps", type=int, default=2, help="timed repeats of each CPU-baseline stage (median)")
     ap.add_argument("--workload", default="tile50m", choices=("tile50m", "mls400m", "batch256"),
                     help="tile50m: the benchmark line (default); mls400m: the mobile-laser-scan stress run; batch256: "
                          "256 independent clouds searched together (each prints one JSON line of its own)")
     ap.add_argument("--no-verify", action="store_true")
-    ap.add_argument("--competitor", action="store_true",
-                    help="also time the reference's competitor, its vendored nanoflann KD-tree, on the CPU")
+    ap.add_argument("--no-competitor", action="store_true",
+                    help="skip the reference's competitor, its vendored nanoflann KD-tree, on the CPU")
     ap.add_argument("--las", default=None, help="run the benchmark line on the points of this (uncompressed) LAS file "
                                                  "instead of the synthetic tile")
     ap.add_argument("--profile", default=None,
@@ -381,30 +408,34 @@ def main():
     fp32, fp64, fp32v = results["fp32"], results["fp64"], results["fp32_alt"]
     ctx.set_mfma(None)
 
-    # kNN (KNNV2) on the same tile: every point as a query, int32 indices + FP32 squared distances kept
-    # on the device; each precision with its autotuned lanes-per-query shape
+    # kNN (KNNV2) on the same tile: every point as a query (the first nq SFC positions for large k, the
+    # per-query work being additive), int32 indices + FP32 squared distances kept on the device; each
+    # precision with its autotuned lanes-per-query shape
     knn = {}
-    if args.knn_k > 0:
-        k = args.knn_k
+    kvals = [int(v) for v in str(args.knn_k).split(",") if int(v) > 0]
+    for k in kvals:
+        nq = min(n, n if k <= 50 else (10_000_000 if k <= 250 else 2_000_000))
+        ent = {"k": k, "queries": nq}
         for name, prec in (("fp32", A.PREC_FP32), ("fp64", A.PREC_FP64)):
-            lanes, cand = ctx.autotune_knn(k, prec, sample_nq=n)  # tuned on the full workload
-            ctx.knn_dev(k, prec, nq=n)
+            lanes, cand = ctx.autotune_knn(k, prec, sample_nq=min(nq, 4_000_000))
+            ctx.knn_dev(k, prec, nq=nq)
             sync_all(ctx)
             t = time.perf_counter()
-            ctx.knn_dev(k, prec, nq=n)
+            ctx.knn_dev(k, prec, nq=nq)
             sync_all(ctx)
             wall = time.perf_counter() - t
             kern = ctx.stage_times()["knn"]
-            st = ctx.knn_dev(k, prec, nq=n, stats=True)  # counters in a run of their own (global atomics)
-            knn[name] = {"queries_per_s": n / wall, "kernel_ms": kern, "lanes_per_query": lanes,
-                         "candidate_ms": cand.tolist(), "points_tested_per_query": st["points_tested"] / n,
-                         "rechecked_per_query": st["rechecked"] / n}
-            log(f"[bench] knn k={k} {name}: {n / wall:.3e} queries/s, kernel {kern:.1f} ms, lanes {lanes}")
-            ksamp = np.arange(0, n, max(1, n // 2000), dtype=np.int64)[:2000]
+            st = ctx.knn_dev(k, prec, nq=nq, stats=True)  # counters in a run of their own (global atomics)
+            ent[name] = {"queries_per_s": nq / wall, "kernel_ms": kern, "lanes_per_query": lanes,
+                         "candidate_ms": cand.tolist(), "points_tested_per_query": st["points_tested"] / nq,
+                         "rechecked_per_query": st["rechecked"] / nq}
+            log(f"[bench] knn k={k} {name}: {nq / wall:.3e} queries/s over {nq} queries, kernel {kern:.1f} ms, "
+                f"lanes {lanes}")
+            ksamp = np.linspace(0, nq - 1, 400 if k <= 250 else 100).astype(np.int64)
             rows = [ctx.knn_fetch(int(q), 1) for q in ksamp]
-            knn[name]["_sample"] = (np.vstack([r[0] for r in rows]), np.vstack([r[1] for r in rows]))
-        knn["k"] = k
-        knn["fp32_vs_fp64_speedup"] = knn["fp64"]["kernel_ms"] / knn["fp32"]["kernel_ms"]
+            ent[name]["_sample"] = (ksamp, np.vstack([r[0] for r in rows]), np.vstack([r[1] for r in rows]))
+        ent["fp32_vs_fp64_speedup"] = ent["fp64"]["kernel_ms"] / ent["fp32"]["kernel_ms"]
+        knn[k] = ent
     qps = n / fp32["wall_s"]
     verify = {}
     if not args.no_verify:
@@ -439,83 +470,129 @@ def main():
                 ok &= bool(np.array_equal(cnt, ref_sizes)) and all(np.array_equal(a, b) for a, b in zip(lists, ref))
             verify["lists_eq_oracle_sample"] = ok
             verify["sample_queries"] = int(len(samp))
-            if knn:
-                ksamp = np.arange(0, n, max(1, n // 2000), dtype=np.int64)[:2000]
+            knn_ok, knn_err = True, 0.0
+            for k, ent in knn.items():
+                ksamp = ent["fp32"]["_sample"][0]
                 t = time.time()
-                oi, od = O.knn(tree, sx, sx[ksamp], knn["k"])  # the reference's best-first search
-                knn["cpu_oracle_queries_per_s"] = len(ksamp) / max(time.time() - t, 1e-9)
+                oi, od = O.knn(tree, sx, sx[ksamp], k)  # the reference's best-first search
+                ent["cpu_oracle_queries_per_s"] = len(ksamp) / max(time.time() - t, 1e-9)
                 # distances against the best-first search on the whole sample; index lists against brute
-                # force (order (d^2, index)) on its first 16 queries (the best-first order of ties is free)
-                bi, _ = O.knn_brute(sx, sx[ksamp[:16]], knn["k"])
-                ok, err = True, 0.0
+                # force (order (d^2, index)) on its first 8 queries (the best-first order of ties is free)
+                bi, _ = O.knn_brute(sx, sx[ksamp[:8]], k)
+                err = 0.0
                 for name in ("fp32", "fp64"):
-                    gi, gf = knn[name]["_sample"]
-                    ok &= bool(np.array_equal(gf, od.astype(np.float32)))
-                    ok &= bool(np.array_equal(gi[:16].astype(np.int64), bi))
+                    _, gi, gf = ent[name]["_sample"]
+                    knn_ok &= bool(np.array_equal(gf, od.astype(np.float32)))
+                    knn_ok &= bool(np.array_equal(gi[:8].astype(np.int64), bi))
                     err = max(err, float(np.max(np.abs(gf.astype(np.float64) - od))))
-                verify["knn_eq_oracle_sample"] = ok
-                knn["max_abs_d2_error_fp32_output"] = err
+                ent["max_abs_d2_error_fp32_output"] = err
+                ent["sample_queries"] = int(len(ksamp))
+                knn_err = max(knn_err, err)
+            if knn:
+                verify["knn_eq_oracle_sample"] = knn_ok
         if not args.no_cpu:
-            threads = O.lib().orc_max_threads()
-            # calibrate, then time a sample sized for ~cpu_seconds of CPU work
+            threads = int(O.lib().orc_max_threads())
+            reps = max(2, args.cpu_reps)
+            # the reference's CPU path restated (the oracle, OpenMP on every host core): encode + sort
+            # (enc_build_benchmarks.hpp:183-227) and the octree build (:300-345) on the same cloud, one
+            # warm-up then the median of `reps` timed runs per stage
+            cpu_stages = {}
+            cpu_stages["bbox"], (cc, rr) = median_of(lambda: O.mbb(xyz), reps)
+            cpu_stages["encode"], ckeys = median_of(lambda: O.encode(O.ENC_HILBERT, xyz, cc, rr), reps)
+            cpu_stages["sort"], (csk, cperm) = median_of(lambda: O.sort(ckeys, nthreads=threads), reps)
+            cpu_stages["gather"], _ = median_of(lambda: xyz[cperm.astype(np.int64)], reps)
+            del ckeys
+            verify["cpu_keys_eq_gpu"] = bool(np.array_equal(csk, keys0) and np.array_equal(cperm, perm0))
+            del csk, cperm
+            cpu_stages["build"], _ = median_of(lambda: O.Tree(keys0, args.max_leaf, A.ENC_HILBERT, c, r), reps)
+            log(f"[bench] cpu stages (s) {cpu_stages}")
+            # the search: a strided sample of the all-points queries sized for ~cpu_seconds per run, the
+            # reference harness loop (neighbor_benchmarks.hpp:507-561); the calibration run is the warm-up
             cal = np.arange(0, n, max(1, n // 20000), dtype=np.int64)
             t = time.time()
             O.radius_harness(tree, sx, O.K_SPHERE, O.A_STRUCT, sx[cal], args.radius)
             per_q = max((time.time() - t) / len(cal), 1e-9)
-            nq_cpu = int(min(n, max(len(cal), args.cpu_seconds / per_q)))
+            nq_cpu = int(min(n, max(len(cal), args.cpu_seconds / reps / per_q)))
             stride = max(1, n // nq_cpu)
             samp = np.arange(0, n, stride, dtype=np.int64)
-            t = time.time()
-            O.radius_harness(tree, sx, O.K_SPHERE, O.A_STRUCT, sx[samp], args.radius)
-            dt = time.time() - t
-            cpu = {"value": len(samp) / dt, "unit": "queries/s", "cores": int(threads), "kind": "port",
+            centres = sx[samp]
+            ts = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                O.radius_harness(tree, sx, O.K_SPHERE, O.A_STRUCT, centres, args.radius)
+                ts.append(time.perf_counter() - t)
+            dt = float(np.median(ts))
+            cpu = {"value": len(samp) / dt, "unit": "queries/s", "cores": threads, "kind": "port",
+                   "cpu_model": cpu_model(), "repeats": reps, "repeat_s": ts,
                    "sample": f"{len(samp)} of the {n} all-points queries (every {stride}th SFC index), "
                              f"neighborsStruct sphere r={args.radius} with the reference harness loop "
-                             f"(build each range list, keep its size), OpenMP dynamic schedule, {dt:.1f} s"}
+                             f"(build each range list, keep its size), OpenMP dynamic schedule; median of {reps} "
+                             f"timed runs after a warm-up",
+                   "stages_s": cpu_stages,
+                   "encode_sort_s": cpu_stages["bbox"] + cpu_stages["encode"] + cpu_stages["sort"] + cpu_stages["gather"],
+                   "build_s": cpu_stages["build"],
+                   "compile_flags": "gcc -O2 -mavx2 -mbmi2 -fopenmp -ffp-contract=off (the reference's Release flags)"}
             log(f"[bench] cpu baseline {cpu}")
-        if args.competitor and O.nanoflann_available():
+        if not args.no_competitor and O.nanoflann_available():
             # the reference's competitor: its vendored nanoflann KD-tree (leaf size maxLeaf) over the sorted
             # cloud, radiusSearch per centre as its benchmark drives it, on a sample of the centres
             t = time.time()
             nf = O.NanoflannTree(sx, args.max_leaf)
             nf_build = time.time() - t
-            nsamp = np.arange(0, n, 10, dtype=np.int64)
-            t = time.time()
-            nfc = nf.radius_counts(sx[nsamp], args.radius)
-            nf_dt = time.time() - t
+            nsamp = np.arange(0, n, 25, dtype=np.int64)
+            centres = sx[nsamp]
+            nf_dt, nfc = median_of(lambda: nf.radius_counts(centres, args.radius), 2)
             gpu_c, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, idx=nsamp.astype(np.uint32), radius=args.radius)
             competitor = {"name": "nanoflann KD-tree (reference's vendored 1.7.1), CPU", "build_s": nf_build,
                           "radius_queries_per_s": len(nsamp) / nf_dt, "cores": int(O.lib().orc_max_threads()),
-                          "sample": f"{len(nsamp)} centres (every 10th SFC index), sphere r={args.radius}",
+                          "sample": f"{len(nsamp)} centres (every 25th SFC index), sphere r={args.radius}, median of "
+                                    f"2 runs after a warm-up",
                           "counts_eq_gpu": bool(np.array_equal(nfc, gpu_c.astype(np.uint64)))}
             log(f"[bench] competitor {competitor}")
             del nf
 
     # Roofline of the dominant kernel (the FP32 search): 9 flops (3 sub + 3 mul + 2 add + 1 compare) per
     # point test of the reference traversal (T, identical for both precisions), over the kernel's
-    # device time (HIP events on the context's stream), against the FP32 compute roof. traffic: the
-    # kernel's HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
-    # this workload (calibrated on a copy kernel), when the measured configuration matches it.
+    # device time (HIP events on the context's stream), against MI355X's FP32 compute roof; the kernel
+    # runs on the vector ALUs. traffic: the kernel's HBM bytes per launch from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated on a copy kernel), used only when the
+    # summary was taken from this kernel's current sources and tile shape.
     T = fp32["stats"]["points_tested"]
     kern_s = fp32["kernel_ms"] * 1e-3
     achieved = 9.0 * T / kern_s / 1e12
-    traffic = None
+    traffic, traffic_note = None, "no counter summary"
+    digest = source_digest()
     try:
         tr = json.load(open(os.path.join(ROOT, "profiles", "search_c2_50M_traffic.json")))
         cfg = f"k_count<1, 3, 1, {tuned['fp32']['queries_per_wave']}, false, {str(tuned['fp32']['mfma']).lower()}>"
-        if n == 50_000_000 and tr["kernel"].startswith(cfg):
-            traffic = tr["hbm_bytes_per_launch"]
+        if n != 50_000_000 or las is not None:
+            traffic_note = "counter summary is of the 50 M-point tile"
+        elif not tr["kernel"].startswith(cfg):
+            traffic_note = f"counter summary is of {tr['kernel']}, tuned shape is {cfg}"
+        elif tr.get("source_digest") != digest:
+            traffic_note = f"counter summary is of sources {tr.get('source_digest')}, these are {digest}"
+        else:
+            traffic, traffic_note = tr["hbm_bytes_per_launch"], f"rocprofv3 counters, sources {digest}"
     except (OSError, ValueError, KeyError):
-        traffic = None
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+        pass
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "traffic_source": traffic_note,
                 "kernel": "radius count (FP32 + FP64 re-check)",
-                "compute_unit": "matrix cores" if tuned["fp32"]["mfma"] else "vector ALUs (the FP32 roof is the same 157.3 TFLOP/s figure)"}
+                "compute_unit": "matrix cores" if tuned["fp32"]["mfma"] else "vector ALUs",
+                "work": f"9 flops x {T} point tests of the reference traversal"}
 
     es_ms = med(bbox_ms) + med(enc_ms) + med(sort_ms) + med(gather_ms)
     # algorithmic bytes of bbox + encode + sort + gather: 24 + 32 + 8 (histogram) + P * 24 (key + index read
     # and written per pass) + 52 (gather) B/pt with 8-bit digits, P = 8 passes
     es_bytes = (116 + 24 * 8) * n
+    es_gbs = es_bytes / (es_ms * 1e-3) / 1e9
+    copy_gbs, _ = ctx.copy_bandwidth(1 << 30)  # the practical streaming ceiling, measured in this run
+    log(f"[bench] streaming copy ceiling {copy_gbs:.0f} GB/s")
+    roofline_es = {"bound": "hbm", "achieved": es_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": es_gbs / PEAK_HBM_GBS,
+                   "traffic": None, "kernel": "bbox + encode + 8 radix passes + gather",
+                   "work": f"{116 + 24 * 8} B/pt x {n} points", "copy_ceiling_gbs": copy_gbs,
+                   "frac_of_copy_ceiling": es_gbs / copy_gbs}
+
     out = {
         "metric": "radius queries/s @50M pts, FP32+recheck vs FP64 GPU (exact sets); build ms",
         "value": qps,
@@ -553,9 +630,10 @@ def main():
         "autotune": tuned,
         "union_efficiency": {"fp32_test_slots_used": fp32["stats"]["points_tested"] / max(1, fp32["stats"]["test_slots"]),
                              "fp32_class_slots_used": fp32["stats"]["nodes_classified"] / max(1, fp32["stats"]["class_slots"])},
-        "knn": {kk: v for kk, v in knn.items() if kk not in ("fp32", "fp64")} | {
-            name: {kk: v for kk, v in knn[name].items() if kk != "_sample"} for name in ("fp32", "fp64") if name in knn},
+        "knn": {str(k): {kk: ({x: y for x, y in v.items() if x != "_sample"} if kk in ("fp32", "fp64") else v)
+                         for kk, v in ent.items()} for k, ent in knn.items()},
         "roofline": roofline,
+        "roofline_encode_sort": roofline_es,
         "cpu_baseline": cpu,
         "competitor_cpu": competitor,
     }

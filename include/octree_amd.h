@@ -87,6 +87,9 @@ const char *oct_version(void);
  * radii = (max - min) / 2; the encode/octree Box is Box(center, radii) (box.hpp:17). */
 int oct_set_points(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_bytes);
 int oct_set_points_dev(oct_ctx *ctx, const double *xyz_dev, size_t n, size_t stride_bytes);
+/* The structure-of-arrays container PointsSoA (inc/geometry/point_containers.hpp:72-163): n points
+ * as three host coordinate arrays (its dataX / dataY / dataZ). Same effect as oct_set_points. */
+int oct_set_points_soa(oct_ctx *ctx, const double *x, const double *y, const double *z, size_t n);
 int oct_bbox(oct_ctx *ctx, double center[3], double radii[3]);
 
 /* ---- H2 encode: PointEncoder::encodePoints (point_encoder.hpp:54-63) with getAnchorCoords
@@ -119,6 +122,10 @@ int oct_sort_points(oct_ctx *ctx, int encoder, double center_out[3], double radi
  * linear_octree.hpp:614-618), replacing what oct_sort left on the device. */
 int oct_set_sorted(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_bytes, const uint64_t *codes,
                    int encoder, const double center[3], const double radii[3]);
+/* oct_set_sorted for a LinearOctree<PointsSoA> (linear_octree.hpp:614-618 over
+ * point_containers.hpp:72-163): the sorted cloud as three host coordinate arrays. */
+int oct_set_sorted_soa(oct_ctx *ctx, const double *x, const double *y, const double *z, size_t n, const uint64_t *codes,
+                       int encoder, const double center[3], const double radii[3]);
 
 /* ---- H4 build: LinearOctree(points, codes, box, enc) -> buildOctree (linear_octree.hpp:567-623);
  * maxPointsLeaf is the reference's global mainOptions.maxPointsLeaf (main_options.hpp:210). */
@@ -187,6 +194,13 @@ int oct_knn(oct_ctx *ctx, int precision, const double *centers, const uint32_t *
 int oct_knn_dev(oct_ctx *ctx, int precision, const uint32_t *center_idx_dev, size_t first, size_t nq, uint32_t k,
                 int32_t *idx_dev, float *d2_dev, oct_search_stats *stats);
 int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d2_out);
+/* Verification of kNN results that stay on the device (no reference counterpart: it compares result
+ * sets too large to copy out, e.g. FP32 + recheck against FP64 for all points at k = 1000). Over nq
+ * rows of k entries -- the caller's device buffers, or with both NULL the context's kept results of
+ * the last oct_knn_dev -- out[0] = order-dependent 64-bit checksum of (position, index, FP32 d^2
+ * bits), out[1] = rows whose distances are not ascending or whose indices lie outside the cloud,
+ * out[2] = sum of the indices (mod 2^64). */
+int oct_knn_checksum(oct_ctx *ctx, const int32_t *idx_dev, const float *d2_dev, size_t nq, uint32_t k, uint64_t out[3]);
 
 /* ---- tuning (no counterpart in the reference: its per-query loop has no tile shape to choose).
  * OCT_TUNE_RADIUS_GROUP: queries per wave of the count-mode radius kernel (16, 32 or 64 queries
@@ -317,6 +331,9 @@ int oct_las_fetch(oct_ctx *ctx, double *xyz, oct_point_meta *meta);
 /* ---- timing: device milliseconds of the stages of the last calls (HIP events on the context's
  * stream): [0] bbox, [1] encode, [2] sort, [3] gather, [4] build, [5] search, [6] knn. */
 int oct_stage_times(const oct_ctx *ctx, double ms[8]);
+/* Measurement aid (no reference counterpart): the best of 5 timed device copies of `bytes` bytes
+ * (16-byte words), in ms -- the practical read + write ceiling for the HBM-bound stages. */
+int oct_copy_bandwidth(oct_ctx *ctx, size_t bytes, double *ms_out);
 
 #ifdef __cplusplus
 }

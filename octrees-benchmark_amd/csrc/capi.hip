@@ -68,6 +68,86 @@ __global__ void k_unstride(const double *__restrict__ src, size_t stride_d, doub
     }
 }
 
+// Interleaves three coordinate arrays (x[0..n), y[0..n), z[0..n) back to back in src) into packed xyz.
+__global__ void k_interleave(const double *__restrict__ src, double *__restrict__ xyz, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        xyz[3 * i] = src[i];
+        xyz[3 * i + 1] = src[n + i];
+        xyz[3 * i + 2] = src[2 * n + i];
+    }
+}
+
+// Checksum of kNN result rows (nq rows of k int32 indices and FP32 squared distances): the wrapping
+// sum over entries of a 64-bit mix of (position, index, distance bits) -- equal sums mean equal
+// arrays up to a 2^-64 chance -- plus the rows whose distances are not ascending or whose indices
+// lie outside [0, n), and the sum of the indices. One wave-reduced atomic per wave.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__global__ void k_knn_checksum(const int32_t *__restrict__ idx, const float *__restrict__ d2, size_t nq, uint32_t k,
+                               uint32_t n, unsigned long long *__restrict__ out) {
+    uint64_t h = 0, bad = 0, isum = 0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (size_t)gridDim.x * blockDim.x) {
+        const int32_t *ri = idx + q * k;
+        const float *rd = d2 + q * k;
+        bool ok = true;
+        float prev = -INFINITY;
+        for (uint32_t j = 0; j < k; j++) {
+            const int32_t i = ri[j];
+            const float d = rd[j];
+            ok &= (i >= 0 && (uint32_t)i < n && !(d < prev));
+            prev = d;
+            isum += (uint32_t)i;
+            h += mix64((q * k + j) * 0x100000001b3ull ^ (((uint64_t)(uint32_t)i << 32) | __float_as_uint(d)));
+        }
+        bad += !ok;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        h += __shfl_xor(h, o);
+        bad += __shfl_xor(bad, o);
+        isum += __shfl_xor(isum, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], (unsigned long long)h);
+        atomicAdd(&out[1], (unsigned long long)bad);
+        atomicAdd(&out[2], (unsigned long long)isum);
+    }
+}
+
+// Streaming copy of 16-byte words, four in flight per thread: the device's practical read + write
+// ceiling that the HBM-bound stages are measured against.
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t words) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < words; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a, dst[i + stride] = b, dst[i + 2 * stride] = c, dst[i + 3 * stride] = d;
+    }
+    for (; i < words; i += stride) dst[i] = src[i];
+}
+
+// Copies n points given as three host coordinate arrays into the packed device array.
+static int upload_xyz_soa(oct_ctx *ctx, const double *x, const double *y, const double *z, size_t n) {
+    if (n >= (1ull << 30)) return fail(ctx, OCT_ERR_ARG, "at most 2^30 - 1 points");
+    ctx->las_loaded = false;
+    HCHK(ctx->xyz.reserve(3 * n), "alloc points");
+    if (n) {
+        if (!x || !y || !z) return fail(ctx, OCT_ERR_ARG, "null coordinate array");
+        HCHK(ctx->qxyz.reserve(3 * n), "alloc staging");
+        HCHK(hipMemcpyAsync(ctx->qxyz.p, x, n * 8, hipMemcpyHostToDevice, ctx->stream), "copy x");
+        HCHK(hipMemcpyAsync(ctx->qxyz.p + n, y, n * 8, hipMemcpyHostToDevice, ctx->stream), "copy y");
+        HCHK(hipMemcpyAsync(ctx->qxyz.p + 2 * n, z, n * 8, hipMemcpyHostToDevice, ctx->stream), "copy z");
+        unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+        k_interleave<<<nb, 256, 0, ctx->stream>>>(ctx->qxyz.p, ctx->xyz.p, n);
+        HCHK(hipGetLastError(), "interleave points");
+    }
+    ctx->n = n;
+    return OCT_OK;
+}
+
 // Copies n points (x, y, z leading doubles, stride bytes apart) into the packed device array.
 // Packed input is one flat copy; strided input (e.g. the 32-byte Point) is copied flat into a
 // staging buffer and repacked on the device.
@@ -135,6 +215,37 @@ void query_order(oct_ctx *ctx, const double *centers, const uint32_t *idx, size_
 }
 
 }  // namespace
+
+// Upload of an already sorted cloud (points through `upload`) with its keys and Box.
+template <class Upload>
+static int set_sorted_impl(oct_ctx *ctx, size_t n, const uint64_t *codes, int encoder, const double center[3],
+                           const double radii[3], Upload &&upload) {
+    if (!ctx || !center || !radii || (n && !codes)) return fail(ctx, OCT_ERR_ARG, "null argument");
+    if (!valid_encoder(encoder)) return fail(ctx, OCT_ERR_UNSUPPORTED, "only the 3-D encoders build octrees");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    for (size_t i = 1; i < n; i++)
+        if (codes[i] < codes[i - 1]) return fail(ctx, OCT_ERR_ARG, "codes must be sorted ascending");
+    int r = upload();
+    if (r) return r;
+    HCHK(ctx->keys[0].reserve(n), "alloc keys");
+    HCHK(ctx->p64.reserve(n), "alloc points");
+    if (n) {
+        HCHK(hipMemcpyAsync(ctx->keys[0].p, codes, n * 8, hipMemcpyHostToDevice, ctx->stream), "copy codes");
+        unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+        k_pad<<<nb, 256, 0, ctx->stream>>>(ctx->xyz.p, ctx->p64.p, n);
+        HCHK(hipGetLastError(), "pad points");
+    }
+    HCHK(hipStreamSynchronize(ctx->stream), "set sorted");
+    ctx->sorted_keys = ctx->keys[0].p;
+    ctx->perm = nullptr;
+    ctx->encoder = encoder;
+    for (int a = 0; a < 3; a++) ctx->center[a] = center[a], ctx->radii[a] = radii[a];
+    ctx->has_points = true;
+    ctx->has_keys = false;
+    ctx->has_sorted = true;
+    ctx->has_tree = ctx->has_lists = false;
+    return OCT_OK;
+}
 
 extern "C" {
 
@@ -213,6 +324,17 @@ int oct_set_points(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_byte
     if (!ctx) return OCT_ERR_ARG;
     HCHK(hipSetDevice(ctx->device), "set device");
     int r = upload_xyz(ctx, xyz, n, stride_bytes, hipMemcpyHostToDevice);
+    if (r) return r;
+    HCHK(hipStreamSynchronize(ctx->stream), "copy points");
+    ctx->has_points = true;
+    ctx->has_keys = ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
+    return OCT_OK;
+}
+
+int oct_set_points_soa(oct_ctx *ctx, const double *x, const double *y, const double *z, size_t n) {
+    if (!ctx) return OCT_ERR_ARG;
+    HCHK(hipSetDevice(ctx->device), "set device");
+    int r = upload_xyz_soa(ctx, x, y, z, n);
     if (r) return r;
     HCHK(hipStreamSynchronize(ctx->stream), "copy points");
     ctx->has_points = true;
@@ -318,31 +440,13 @@ int oct_sort_points(oct_ctx *ctx, int encoder, double center_out[3], double radi
 
 int oct_set_sorted(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_bytes, const uint64_t *codes, int encoder,
                    const double center[3], const double radii[3]) {
-    if (!ctx || !center || !radii || (n && !codes)) return fail(ctx, OCT_ERR_ARG, "null argument");
-    if (!valid_encoder(encoder)) return fail(ctx, OCT_ERR_UNSUPPORTED, "only the 3-D encoders build octrees");
-    HCHK(hipSetDevice(ctx->device), "set device");
-    for (size_t i = 1; i < n; i++)
-        if (codes[i] < codes[i - 1]) return fail(ctx, OCT_ERR_ARG, "codes must be sorted ascending");
-    int r = upload_xyz(ctx, xyz, n, stride_bytes, hipMemcpyHostToDevice);
-    if (r) return r;
-    HCHK(ctx->keys[0].reserve(n), "alloc keys");
-    HCHK(ctx->p64.reserve(n), "alloc points");
-    if (n) {
-        HCHK(hipMemcpyAsync(ctx->keys[0].p, codes, n * 8, hipMemcpyHostToDevice, ctx->stream), "copy codes");
-        unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-        k_pad<<<nb, 256, 0, ctx->stream>>>(ctx->xyz.p, ctx->p64.p, n);
-        HCHK(hipGetLastError(), "pad points");
-    }
-    HCHK(hipStreamSynchronize(ctx->stream), "set sorted");
-    ctx->sorted_keys = ctx->keys[0].p;
-    ctx->perm = nullptr;
-    ctx->encoder = encoder;
-    for (int a = 0; a < 3; a++) ctx->center[a] = center[a], ctx->radii[a] = radii[a];
-    ctx->has_points = true;
-    ctx->has_keys = false;
-    ctx->has_sorted = true;
-    ctx->has_tree = ctx->has_lists = false;
-    return OCT_OK;
+    return set_sorted_impl(ctx, n, codes, encoder, center, radii,
+                           [&] { return upload_xyz(ctx, xyz, n, stride_bytes, hipMemcpyHostToDevice); });
+}
+
+int oct_set_sorted_soa(oct_ctx *ctx, const double *x, const double *y, const double *z, size_t n, const uint64_t *codes,
+                       int encoder, const double center[3], const double radii[3]) {
+    return set_sorted_impl(ctx, n, codes, encoder, center, radii, [&] { return upload_xyz_soa(ctx, x, y, z, n); });
 }
 
 int oct_build(oct_ctx *ctx, uint32_t max_points_leaf, oct_tree_info *info) {
@@ -820,6 +924,65 @@ int oct_knn_fetch(oct_ctx *ctx, size_t q0, size_t nq, int32_t *idx_out, float *d
     if (idx_out && cnt) HCHK(hipMemcpyAsync(idx_out, ctx->kidx.p + off, cnt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy idx");
     if (d2_out && cnt) HCHK(hipMemcpyAsync(d2_out, ctx->kd2f.p + off, cnt * 4, hipMemcpyDeviceToHost, ctx->stream), "copy d2");
     HCHK(hipStreamSynchronize(ctx->stream), "knn fetch");
+    return check_device_error(ctx);
+}
+
+int oct_copy_bandwidth(oct_ctx *ctx, size_t bytes, double *ms_out) {
+    if (!ctx || !ms_out) return OCT_ERR_ARG;
+    if (bytes < 16 || bytes % 16) return fail(ctx, OCT_ERR_ARG, "copy size must be a positive multiple of 16 bytes");
+    HCHK(hipSetDevice(ctx->device), "set device");
+    HCHK(ctx->rec_in.reserve(bytes), "alloc copy source");
+    HCHK(ctx->rec_out.reserve(bytes), "alloc copy target");
+    HCHK(hipMemsetAsync(ctx->rec_in.p, 1, bytes, ctx->stream), "fill copy source");
+    int cus = 0;
+    HCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "device attribute");
+    const size_t words = bytes / 16;
+    const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((words + 255) / 256, (size_t)cus * 8));
+    hipEvent_t e0, e1;
+    HCHK(hipEventCreate(&e0), "event");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(ctx, OCT_ERR_HIP, "event");
+    }
+    double best = 1e300;
+    hipError_t err = hipSuccess;
+    for (int rep = 0; rep < 6 && err == hipSuccess; rep++) {  // the first repetition warms up
+        err = hipEventRecord(e0, ctx->stream);
+        if (err == hipSuccess) {
+            k_copy16<<<nb, 256, 0, ctx->stream>>>((const uint4 *)ctx->rec_in.p, (uint4 *)ctx->rec_out.p, words);
+            err = hipGetLastError();
+        }
+        if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err == hipSuccess && rep) best = std::min(best, (double)ms);
+    }
+    (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return hfail(ctx, err, "copy");
+    *ms_out = best;
+    return OCT_OK;
+}
+
+int oct_knn_checksum(oct_ctx *ctx, const int32_t *idx_dev, const float *d2_dev, size_t nq, uint32_t k, uint64_t out[3]) {
+    if (!ctx || !out) return OCT_ERR_ARG;
+    if (!idx_dev != !d2_dev) return fail(ctx, OCT_ERR_ARG, "give both result buffers or neither");
+    if (!idx_dev) {
+        if (nq > ctx->knn_nq || (nq && k != ctx->knn_k)) return fail(ctx, OCT_ERR_ARG, "checksum beyond the kept results");
+        idx_dev = ctx->kidx.p, d2_dev = ctx->kd2f.p;
+    }
+    HCHK(hipSetDevice(ctx->device), "set device");
+    HCHK(ctx->stats.reserve(8), "alloc stats");
+    HCHK(hipMemsetAsync(ctx->stats.p, 0, 3 * sizeof(unsigned long long), ctx->stream), "clear checksum");
+    if (nq && k) {
+        const unsigned nb = (unsigned)std::min<size_t>((nq + 255) / 256, 4096);
+        k_knn_checksum<<<nb, 256, 0, ctx->stream>>>(idx_dev, d2_dev, nq, k, (uint32_t)ctx->n, ctx->stats.p);
+        HCHK(hipGetLastError(), "knn checksum");
+    }
+    unsigned long long h[3];
+    HCHK(hipMemcpyAsync(h, ctx->stats.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream), "copy checksum");
+    HCHK(hipStreamSynchronize(ctx->stream), "knn checksum");
+    for (int i = 0; i < 3; i++) out[i] = h[i];
     return check_device_error(ctx);
 }
 

@@ -3,6 +3,7 @@
 // are the reference's integer key algebra, kept for callers that use them.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <optional>
@@ -16,6 +17,7 @@
 #include "../geometry/point_containers.hpp"
 #include "../geometry/point_metadata.hpp"
 #include "../main_options.hpp"
+#include "../octree_amd_points.hpp"
 #include "../octree_amd_runtime.hpp"
 
 struct EncodingLog {
@@ -54,6 +56,64 @@ public:
         run_codec(c, &k, 1, true);
         x = c[0], y = c[1], z = c[2];
     }
+    // encodeVectorized (point_encoder.hpp:24-35): keys[i .. i + 8) for the coordinates x/y/z[0 .. 8)
+    virtual void encodeVectorized(const uint32_t *x, const uint32_t *y, const uint32_t *z, std::vector<key_t> &keys,
+                                  size_t i) const {
+        uint32_t c[24];
+        uint64_t k[8];
+        for (int j = 0; j < 8; j++) c[3 * j] = x[j], c[3 * j + 1] = y[j], c[3 * j + 2] = z[j];
+        run_codec(c, k, 8, false);
+        for (int j = 0; j < 8; j++) keys[i + j] = (key_t)k[j];
+    }
+
+    // getAnchorCoords (point_encoder.hpp:37-49): the point's cell in the 2^maxDepth grid over bbox. The
+    // 3-D encoders quantise on the device (the key of the point, decoded back: encode is a bijection
+    // on the grid). The 2-D encoders' grid has 2^31 cells per axis; the reference forms that scale
+    // as the int `1 << 31`, which overflows, so here it is the double 2^31.
+    void getAnchorCoords(const Point &p, const Box &bbox, coords_t &x, coords_t &y, coords_t &z) const {
+        if (is3D()) {
+            decode(encodeFromPoint(p, bbox), x, y, z);
+            return;
+        }
+        const double tx = ((p.getX() - bbox.center().getX()) + bbox.radii().getX()) / (2 * bbox.radii().getX());
+        const double ty = ((p.getY() - bbox.center().getY()) + bbox.radii().getY()) / (2 * bbox.radii().getY());
+        const double tz = ((p.getZ() - bbox.center().getZ()) + bbox.radii().getZ()) / (2 * bbox.radii().getZ());
+        const double scale = (double)(1ull << maxDepth());
+        const coords_t top = (coords_t(1) << maxDepth()) - 1;
+        x = std::min((coords_t)(tx * scale), top);
+        y = std::min((coords_t)(ty * scale), top);
+        z = std::min((coords_t)(tz * scale), top);
+    }
+    // encodeFromPoint (point_encoder.hpp:52): one point's key under bbox
+    virtual key_t encodeFromPoint(const Point &p, const Box &bbox) const {
+        if (!is3D()) {
+            coords_t x, y, z;
+            getAnchorCoords(p, bbox, x, y, z);
+            return encode(x, y, z);
+        }
+        const Point one[1] = {p};
+        return encode_cloud(reinterpret_cast<const double *>(one), 1, sizeof(Point), bbox)[0];
+    }
+    // encodePoints (point_encoder.hpp:54-63): keys of every point under bbox, in the container's order
+    template <class Container>
+    std::vector<key_t> encodePoints(const Container &points, const Box &bbox) const {
+        const std::vector<double> xyz = octamd::packed_xyz(points);
+        return encode_cloud(xyz.data(), points.size(), 24, bbox);
+    }
+
+    // getCenter (point_encoder.hpp:517-539): centre of the level-`level` cell holding `code`, in the
+    // reference's FP64 operation order (low corner of the cell, then + the level's radii)
+    Point getCenter(key_t code, uint32_t level, const Box &bbox, const double *halfLengths,
+                    const std::vector<Vector> &precomputedRadii) const {
+        coords_t x, y, z;
+        decode(code, x, y, z);
+        const coords_t mask = ((coords_t(1) << maxDepth()) - 1) ^ ((coords_t(1) << (maxDepth() - level)) - 1);
+        x &= mask, y &= mask, z &= mask;
+        return Point(bbox.minX() + x * halfLengths[0] * 2, bbox.minY() + y * halfLengths[1] * 2,
+                     bbox.minZ() + z * halfLengths[2] * 2) +
+               precomputedRadii[level];
+    }
+
     void encodeCoords(const uint32_t *xyz, size_t n, uint64_t *keys) const {
         run_codec(const_cast<uint32_t *>(xyz), keys, n, false);
     }
@@ -68,7 +128,7 @@ public:
         auto &c = octamd::shared();
         std::lock_guard<std::mutex> g(c.mutex());
         const size_t n = points.size();
-        octamd::check(c.get(), oct_set_points(c.get(), xyz_ptr(points), n, sizeof(Point)), "oct_set_points");
+        octamd::check(c.get(), octamd::set_points(c.get(), points), "oct_set_points");
         const double ctr[3] = {bbox.center().getX(), bbox.center().getY(), bbox.center().getZ()};
         const double rad[3] = {bbox.radii().getX(), bbox.radii().getY(), bbox.radii().getZ()};
         octamd::check(c.get(), oct_encode(c.get(), (int)getEncoder(), ctr, rad, nullptr), "oct_encode");
@@ -91,7 +151,7 @@ public:
         auto &c = octamd::shared();
         std::lock_guard<std::mutex> g(c.mutex());
         const size_t n = points.size();
-        octamd::check(c.get(), oct_set_points(c.get(), xyz_ptr(points), n, sizeof(Point)), "oct_set_points");
+        octamd::check(c.get(), octamd::set_points(c.get(), points), "oct_set_points");
         double ctr[3], rad[3];
         std::vector<uint32_t> perm(n);
         std::vector<key_t> keys(n);
@@ -126,6 +186,18 @@ public:
     }
 
 private:
+    std::vector<key_t> encode_cloud(const double *xyz, size_t n, size_t stride, const Box &bbox) const {
+        std::vector<key_t> keys(n);
+        if (n == 0) return keys;
+        auto &c = octamd::shared();
+        std::lock_guard<std::mutex> g(c.mutex());
+        octamd::check(c.get(), oct_set_points(c.get(), xyz, n, stride), "oct_set_points");
+        const double ctr[3] = {bbox.center().getX(), bbox.center().getY(), bbox.center().getZ()};
+        const double rad[3] = {bbox.radii().getX(), bbox.radii().getY(), bbox.radii().getZ()};
+        octamd::check(c.get(), oct_encode(c.get(), (int)getEncoder(), ctr, rad, reinterpret_cast<uint64_t *>(keys.data())),
+                      "oct_encode");
+        return keys;
+    }
     void run_codec(uint32_t *xyz, uint64_t *keys, size_t n, bool decode) const {
         auto &c = octamd::shared();
         std::lock_guard<std::mutex> g(c.mutex());
@@ -135,20 +207,12 @@ private:
             octamd::check(c.get(), oct_encode_coords(c.get(), (int)getEncoder(), xyz, n, keys), "oct_encode_coords");
     }
 
-    // the points and (when present) their metadata reordered as the keys: records of trivially
-    // copyable types are gathered on the device by the permutation the context holds
+    // the points and (when present) their metadata reordered as the keys, by the permutation the
+    // context holds (records gathered on the device)
     template <class Container>
     static void apply(oct_ctx *ctx, Container &points, std::optional<std::vector<PointMetadata>> &meta_opt,
                       const std::vector<uint32_t> &perm) {
-        using T = typename Container::value_type;
-        Container sorted(points.size());
-        if constexpr (std::is_trivially_copyable<T>::value) {
-            if (!points.empty())
-                octamd::check(ctx, oct_gather_records(ctx, points.data(), sorted.data(), sizeof(T), 0), "oct_gather_records");
-        } else {
-            for (size_t i = 0; i < perm.size(); i++) sorted[i] = points[perm[i]];
-        }
-        points.swap(sorted);
+        octamd::reorder(ctx, points, perm);
         if (meta_opt) {
             std::vector<PointMetadata> m(meta_opt->size());
             if (m.size() == perm.size()) {

@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "../octree_amd_points.hpp"
 #include "../octree_amd_runtime.hpp"
 #include "point.hpp"
 #include "point_containers.hpp"
@@ -44,7 +45,7 @@ template <class Container>
 inline Point mbb(const Container &points, Vector &maxRadius) {
     auto &c = octamd::shared();
     std::lock_guard<std::mutex> g(c.mutex());
-    octamd::check(c.get(), oct_set_points(c.get(), xyz_ptr(points), points.size(), sizeof(Point)), "oct_set_points");
+    octamd::check(c.get(), octamd::set_points(c.get(), points), "oct_set_points");
     double ctr[3], rad[3];
     octamd::check(c.get(), oct_bbox(c.get(), ctr, rad), "oct_bbox");
     maxRadius = Vector(rad[0], rad[1], rad[2]);

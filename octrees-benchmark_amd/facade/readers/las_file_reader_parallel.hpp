@@ -37,7 +37,7 @@ private:
         octamd::check(ctx.get(), oct_las_fetch(ctx.get(), n ? xyz.data() : nullptr, (with_meta && n) ? m.data() : nullptr),
                       "oct_las_fetch");
         Container points(n);
-        for (size_t i = 0; i < n; i++) points[i] = Point(i, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        for (size_t i = 0; i < n; i++) points.set(i, Point(i, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
         std::vector<PointMetadata> meta;
         if (with_meta) {
             meta.resize(n);

@@ -7,15 +7,19 @@
 // the reference's: indices into the sorted cloud, lists in the reference's emission order.
 #pragma once
 
+#include <concepts>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <utility>
 #include <vector>
 
 #include "../encoding/point_encoder.hpp"
 #include "../geometry/box.hpp"
+#include "../kernels/kernel_factory.hpp"
 #include "../main_options.hpp"
+#include "../octree_amd_points.hpp"
 #include "../octree_amd_runtime.hpp"
 #include "neighbor_set.hpp"
 
@@ -39,8 +43,8 @@ public:
         if (codes.size() != points.size()) throw std::invalid_argument("octree_amd: codes and points differ in size");
         const double ctr[3] = {box.center().getX(), box.center().getY(), box.center().getZ()};
         const double rad[3] = {box.radii().getX(), box.radii().getY(), box.radii().getZ()};
-        octamd::check(ctx(), oct_set_sorted(ctx(), xyz_ptr(points), points.size(), sizeof(Point),
-                                            reinterpret_cast<const uint64_t *>(codes.data()), (int)enc.getEncoder(), ctr, rad),
+        octamd::check(ctx(), octamd::set_sorted(ctx(), points, reinterpret_cast<const uint64_t *>(codes.data()),
+                                                (int)enc.getEncoder(), ctr, rad),
                       "oct_set_sorted");
         octamd::check(ctx(), oct_build(ctx(), (uint32_t)mainOptions.maxPointsLeaf, &info_), "oct_build");
         if (log) {
@@ -52,6 +56,32 @@ public:
             log->octreeLeaves = info_.n_leaf;
             log->octreeTotalNodes = info_.n_total;
         }
+    }
+
+    // ---- searches with a kernel object (linear_octree.hpp:695-816, 980-1035): the kernel's shape,
+    // centre and radii go to the device, which evaluates the same predicates as the object
+    template <class Kernel>
+        requires std::derived_from<Kernel, KernelAbstract>
+    NeighborSet<Container> neighborsStruct(const Kernel &k) const {
+        return as_set(list_k(k, OCT_ALGO_STRUCT));
+    }
+    template <class Kernel>
+        requires std::derived_from<Kernel, KernelAbstract>
+    std::vector<size_t> neighborsPrune(const Kernel &k) const {
+        return as_indices(list_k(k, OCT_ALGO_PRUNE));
+    }
+    template <class Kernel, class Function>
+        requires std::derived_from<Kernel, KernelAbstract>
+    std::vector<size_t> neighbors(const Kernel &k, Function &&condition) const {
+        return filter(as_indices(list_k(k, OCT_ALGO_NEIGHBORS)), condition);
+    }
+    template <class Kernel>
+        requires std::derived_from<Kernel, KernelAbstract>
+    NeighborSet<Container> neighborsApprox(const Kernel &k, uint32_t precisionLevel, bool upperBound) const {
+        const KernelAbstract &ka = k;
+        const bool box = is_box(ka.type());
+        return as_set(approx((int)ka.type(), ka.center(), ka.radii().getX(), box ? &ka.radii() : nullptr, precisionLevel,
+                             upperBound));
     }
 
     // ---- radius searches (linear_octree.hpp:1038-1087)
@@ -219,8 +249,10 @@ private:
     mutable std::vector<std::pair<size_t, size_t>> ranges_;
     mutable std::vector<uint32_t> offsets_;
     mutable std::vector<double> radii_;  // precomputedRadii, 22 x 3
+    std::unique_ptr<std::once_flag> tree_once_ = std::make_unique<std::once_flag>();
 
     oct_ctx *ctx() const { return ctx_->get(); }
+    static bool is_box(Kernel_t t) { return t == Kernel_t::cube || t == Kernel_t::square; }
 
     struct Lists {
         int algo;
@@ -243,6 +275,11 @@ private:
         if (!l.entries.empty())
             octamd::check(ctx(), oct_radius_fetch(ctx(), offs, l.entries.data(), l.entries.size()), "oct_radius_fetch");
         return l;
+    }
+    // a kernel object's search: the ball shapes by their radius, the box shapes by their radii
+    Lists list_k(const KernelAbstract &k, int algo) const {
+        if (is_box(k.type())) return list((int)k.type(), algo, k.center(), 0.0, &k.radii());
+        return list((int)k.type(), algo, k.center(), k.radii().getX(), nullptr);
     }
     Lists approx(int kernel, const Point &p, double radius, const Vector *radii, uint32_t level, bool upper) const {
         const double c[3] = {p.getX(), p.getY(), p.getZ()};
@@ -273,20 +310,27 @@ private:
             if (cond(points_[i])) out.push_back(i);
         return out;
     }
+    // Copies the device tree once; concurrent first callers wait until the copy is complete (a copy
+    // that throws leaves the flag unset, so the next call tries again).
     void fetch_tree() const {
-        if (!offsets_.empty()) return;
-        const size_t nt = info_.n_total;
-        std::vector<uint64_t> rg(2 * nt);
-        std::vector<double> ce(3 * nt);
-        offsets_.resize(nt);
-        std::lock_guard<std::mutex> g(ctx_->mutex());
-        radii_.resize(66);
-        octamd::check(ctx(), oct_get_tree(ctx(), nullptr, offsets_.data(), rg.data(), ce.data(), radii_.data()), "oct_get_tree");
-        centers_.resize(nt);
-        ranges_.resize(nt);
-        for (size_t i = 0; i < nt; i++) {
-            centers_[i] = Point(ce[3 * i], ce[3 * i + 1], ce[3 * i + 2]);
-            ranges_[i] = {(size_t)rg[2 * i], (size_t)rg[2 * i + 1]};
-        }
+        std::call_once(*tree_once_, [this] {
+            const size_t nt = info_.n_total;
+            std::vector<uint64_t> rg(2 * nt);
+            std::vector<double> ce(3 * nt);
+            std::vector<uint32_t> off(nt);
+            std::vector<double> rad(66);
+            {
+                std::lock_guard<std::mutex> g(ctx_->mutex());
+                octamd::check(ctx(), oct_get_tree(ctx(), nullptr, off.data(), rg.data(), ce.data(), rad.data()), "oct_get_tree");
+            }
+            centers_.resize(nt);
+            ranges_.resize(nt);
+            for (size_t i = 0; i < nt; i++) {
+                centers_[i] = Point(ce[3 * i], ce[3 * i + 1], ce[3 * i + 2]);
+                ranges_[i] = {(size_t)rg[2 * i], (size_t)rg[2 * i + 1]};
+            }
+            radii_ = std::move(rad);
+            offsets_ = std::move(off);
+        });
     }
 };

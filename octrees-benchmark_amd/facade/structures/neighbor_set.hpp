@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <iterator>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -29,10 +30,14 @@ public:
     size_t size() const { return numberOfPoints; }
     bool empty() const { return numberOfPoints == 0; }
 
+    // (index, point): the point by reference for containers that store Points (PointsAoS), by value
+    // for those that assemble them on access (PointsSoA)
+    using point_ref = std::conditional_t<std::is_reference_v<decltype(std::declval<const Container &>()[0])>, const Point &, Point>;
+
     class Iterator {
     public:
         using iterator_category = std::forward_iterator_tag;
-        using value_type = std::pair<size_t, const Point &>;
+        using value_type = std::pair<size_t, point_ref>;
         using difference_type = std::ptrdiff_t;
         using pointer = value_type *;
         using reference = value_type;

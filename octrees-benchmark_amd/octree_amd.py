@@ -29,7 +29,8 @@ EXPORTS = (
     "oct_batch_build", "oct_batch_info", "oct_batch_stage_times", "oct_batch_radius", "oct_batch_knn",
     "oct_batch_build_device", "oct_batch_set_points", "oct_batch_cloud_sorted", "oct_batch_cloud_tree",
     "oct_batch_queries", "oct_batch_pipeline", "oct_batch_pipeline_results", "oct_batch_autotune_knn",
-    "oct_las_read_header", "oct_las_load", "oct_las_fetch", "oct_gather_records",
+    "oct_las_read_header", "oct_las_load", "oct_las_fetch", "oct_gather_records", "oct_set_points_soa",
+    "oct_set_sorted_soa", "oct_knn_checksum", "oct_copy_bandwidth",
 )
 
 # oct_point_meta (per-point LAS attributes, PointMetadata)
@@ -107,6 +108,10 @@ def lib():
         L.oct_knn.argtypes = [_vp, C.c_int, _vp, _vp, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_knn_dev.argtypes = [_vp, C.c_int, _vp, C.c_size_t, C.c_size_t, C.c_uint32, _vp, _vp, C.POINTER(SearchStats)]
         L.oct_knn_fetch.argtypes = [_vp, C.c_size_t, C.c_size_t, _vp, _vp]
+        L.oct_knn_checksum.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_uint32, _vp]
+        L.oct_copy_bandwidth.argtypes = [_vp, C.c_size_t, C.POINTER(C.c_double)]
+        L.oct_set_points_soa.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t]
+        L.oct_set_sorted_soa.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t, _vp, C.c_int, _vp, _vp]
         L.oct_radius_approx.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_double, _vp, C.c_size_t, C.c_uint32,
                                         C.c_int, C.c_int, _vp, C.POINTER(SearchStats)]
         L.oct_batch_create.argtypes = [C.POINTER(_vp), C.c_int]
@@ -657,6 +662,28 @@ class Octree:
         od = np.empty((nq, k), np.float32)
         self._chk(self.L.oct_knn_fetch(self.h, q0, nq, _ptr(oi), _ptr(od)), "oct_knn_fetch")
         return oi, od
+
+    def knn_checksum(self, nq=None, k=None, idx_ptr=None, d2_ptr=None):
+        """(checksum, bad rows, index sum) of kNN results on the device: the kept results of the last
+        knn_dev (first nq rows), or the caller's device buffers idx_ptr / d2_ptr of nq x k entries."""
+        if idx_ptr is None:
+            total, kk = self._knn_shape
+            nq, k = (total if nq is None else nq), kk
+        out = np.zeros(3, np.uint64)
+        self._chk(self.L.oct_knn_checksum(self.h, idx_ptr, d2_ptr, nq, k, _ptr(out)), "oct_knn_checksum")
+        return int(out[0]), int(out[1]), int(out[2])
+
+    def copy_bandwidth(self, nbytes=1 << 30):
+        """Read + write GB/s of the library's streaming copy over nbytes (best of 5 timed runs)."""
+        ms = C.c_double(0.0)
+        self._chk(self.L.oct_copy_bandwidth(self.h, nbytes, C.byref(ms)), "oct_copy_bandwidth")
+        return 2 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
+
+    def set_points_soa(self, x, y, z):
+        """The cloud as three coordinate arrays (PointsSoA)."""
+        x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+        self._chk(self.L.oct_set_points_soa(self.h, _ptr(x), _ptr(y), _ptr(z), x.shape[0]), "oct_set_points_soa")
+        self.n = x.shape[0]
 
     def set_remap(self, on):
         """Count-mode leaf remapping of sparsely wanted leaves (None: default, on)."""

@@ -171,18 +171,21 @@ void orc_encode(int enc, const double *xyz, size_t n, const double center[3], co
 }
 
 /* sortPoints radix passes (point_encoder.hpp:293-376): 8 passes x 8 bits; per-thread histograms
- * over static contiguous chunks; bucket-major, thread-minor offsets; stable scatter. */
+ * over static contiguous chunks; bucket-major, thread-minor offsets; stable scatter. The chunks run
+ * on `nthreads` OpenMP threads, as the reference's; the result does not depend on it. */
 void orc_sort(uint64_t *keys, uint32_t *perm, size_t n, int nthreads) {
     if (nthreads < 1) nthreads = 1;
     uint64_t *kb = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
     uint32_t *pb = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
     uint64_t *hist = (uint64_t *)calloc((size_t)nthreads * 256, sizeof(uint64_t));
+#pragma omp parallel for schedule(static) num_threads(nthreads)
     for (size_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
     uint64_t *ks = keys, *kd = kb;
     uint32_t *ps = perm, *pd = pb;
     for (int pass = 0; pass < 8; pass++) {
         int shift = pass * 8;
         memset(hist, 0, (size_t)nthreads * 256 * sizeof(uint64_t));
+#pragma omp parallel for schedule(static, 1) num_threads(nthreads)
         for (int t = 0; t < nthreads; t++) {
             size_t lo = n * (size_t)t / (size_t)nthreads, hi = n * (size_t)(t + 1) / (size_t)nthreads;
             for (size_t i = lo; i < hi; i++) hist[t * 256 + ((ks[i] >> shift) & 255)]++;
@@ -194,6 +197,7 @@ void orc_sort(uint64_t *keys, uint32_t *perm, size_t n, int nthreads) {
                 hist[t * 256 + b] = off;
                 off += v;
             }
+#pragma omp parallel for schedule(static, 1) num_threads(nthreads)
         for (int t = 0; t < nthreads; t++) {
             size_t lo = n * (size_t)t / (size_t)nthreads, hi = n * (size_t)(t + 1) / (size_t)nthreads;
             for (size_t i = lo; i < hi; i++) {

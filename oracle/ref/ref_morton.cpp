@@ -13,3 +13,8 @@ extern "C" void ref_morton3d_decode(uint64_t key, uint64_t out[3]) {
     libmorton::morton3D_64_decode((uint_fast64_t)key, x, y, z);
     out[0] = x, out[1] = y, out[2] = z;
 }
+
+// 2-D keys as MortonEncoder2D uses them (morton_encoder_2d.hpp:27-40 -> libmorton morton2D_64_encode)
+extern "C" uint64_t ref_morton2d_encode(uint64_t x, uint64_t y) {
+    return libmorton::morton2D_64_encode((uint_fast32_t)x, (uint_fast32_t)y);
+}

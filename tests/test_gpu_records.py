@@ -33,7 +33,8 @@ def test_gather_records_host(ctx):
 def test_gather_records_device(ctx):
     """Device buffers allocated with the HIP runtime itself (hipMalloc through ctypes)."""
     import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
+    A.lib()
+    hip = C.CDLL("libamdhip64.so.7")  # the HIP runtime the library itself loaded
     hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     hip.hipFree.argtypes = [C.c_void_p]
