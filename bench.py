@@ -91,6 +91,10 @@ def profile(args):
             ctx.sort_points(A.ENC_HILBERT)
             ctx.build(args.max_leaf)
             log(f"[profile] {ctx.stage_times()}")
+        # the streaming copy in the same process: the practical ceiling the stages are read against
+        for _ in range(2):
+            gbs, ms = ctx.copy_bandwidth(1 << 30)
+            log(f"[profile] copy 1 GiB: {ms:.3f} ms = {gbs:.0f} GB/s (read + write)")
         ctx.close()
         return
     kind, _, spec = args.profile.partition(":")
@@ -323,7 +327,8 @@ def main():
                     help="profiling driver instead of the benchmark: 'search:v64,d64' (one all-points search per "
                          "configuration: v = FP32 vector ALUs, m = FP32 matrix cores, d = FP64, then queries per "
                          "wave), 'knn:10' (one all-points kNN per precision), 'sort' (bbox + encode + sort + gather "
-                         "+ build, 3 times) or 'copy' (a 1 GiB copy kernel, the counter calibration)")
+                         "+ build, 3 times, then the 1 GiB copy kernel) or 'copy' (the 1 GiB copy kernel alone, the "
+                         "counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
@@ -500,7 +505,7 @@ def main():
             cpu_stages["bbox"], (cc, rr) = median_of(lambda: O.mbb(xyz), reps)
             cpu_stages["encode"], ckeys = median_of(lambda: O.encode(O.ENC_HILBERT, xyz, cc, rr), reps)
             cpu_stages["sort"], (csk, cperm) = median_of(lambda: O.sort(ckeys, nthreads=threads), reps)
-            cpu_stages["gather"], _ = median_of(lambda: xyz[cperm.astype(np.int64)], reps)
+            cpu_stages["gather"], _ = median_of(lambda: O.gather3(xyz, cperm), reps)
             del ckeys
             verify["cpu_keys_eq_gpu"] = bool(np.array_equal(csk, keys0) and np.array_equal(cperm, perm0))
             del csk, cperm
@@ -533,6 +538,26 @@ def main():
                    "build_s": cpu_stages["build"],
                    "compile_flags": "gcc -O2 -mavx2 -mbmi2 -fopenmp -ffp-contract=off (the reference's Release flags)"}
             log(f"[bench] cpu baseline {cpu}")
+        competitors = []
+        if not args.no_competitor:
+            # the reference's pointer-based octree (restated in the oracle): built by inserting the sorted
+            # points one by one from the cloud box (serial, as the reference), searched with its
+            # neighbors() (boxOverlap descent) per centre, OpenMP over the centres
+            t = time.time()
+            pt = O.PointerOctree(sx, args.max_leaf, c, r)
+            pt_build = time.time() - t
+            nsamp = np.arange(0, n, 25, dtype=np.int64)
+            centres = sx[nsamp]
+            pt_dt, ptc = median_of(lambda: pt.radius_counts(O.K_SPHERE, centres, args.radius), 2)
+            gpu_c, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, idx=nsamp.astype(np.uint32), radius=args.radius)
+            competitors.append({"name": "pointer octree (the reference's Octree, restated), CPU", "build_s": pt_build,
+                                "build_threads": 1, "radius_queries_per_s": len(nsamp) / pt_dt,
+                                "cores": int(O.lib().orc_max_threads()), "leaves": pt.n_leaf, "depth": pt.depth,
+                                "sample": f"{len(nsamp)} centres (every 25th SFC index), sphere r={args.radius}, "
+                                          f"median of 2 runs after a warm-up",
+                                "counts_eq_gpu": bool(np.array_equal(ptc, gpu_c.astype(np.uint64)))})
+            log(f"[bench] competitor {competitors[-1]}")
+            del pt
         if not args.no_competitor and O.nanoflann_available():
             # the reference's competitor: its vendored nanoflann KD-tree (leaf size maxLeaf) over the sorted
             # cloud, radiusSearch per centre as its benchmark drives it, on a sample of the centres
@@ -543,13 +568,14 @@ def main():
             centres = sx[nsamp]
             nf_dt, nfc = median_of(lambda: nf.radius_counts(centres, args.radius), 2)
             gpu_c, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, idx=nsamp.astype(np.uint32), radius=args.radius)
-            competitor = {"name": "nanoflann KD-tree (reference's vendored 1.7.1), CPU", "build_s": nf_build,
-                          "radius_queries_per_s": len(nsamp) / nf_dt, "cores": int(O.lib().orc_max_threads()),
-                          "sample": f"{len(nsamp)} centres (every 25th SFC index), sphere r={args.radius}, median of "
-                                    f"2 runs after a warm-up",
-                          "counts_eq_gpu": bool(np.array_equal(nfc, gpu_c.astype(np.uint64)))}
-            log(f"[bench] competitor {competitor}")
+            competitors.append({"name": "nanoflann KD-tree (reference's vendored 1.7.1), CPU", "build_s": nf_build,
+                                "radius_queries_per_s": len(nsamp) / nf_dt, "cores": int(O.lib().orc_max_threads()),
+                                "sample": f"{len(nsamp)} centres (every 25th SFC index), sphere r={args.radius}, "
+                                          f"median of 2 runs after a warm-up",
+                                "counts_eq_gpu": bool(np.array_equal(nfc, gpu_c.astype(np.uint64)))})
+            log(f"[bench] competitor {competitors[-1]}")
             del nf
+        competitor = competitors or None
 
     # Roofline of the dominant kernel (the FP32 search): 9 flops (3 sub + 3 mul + 2 add + 1 compare) per
     # point test of the reference traversal (T, identical for both precisions), over the kernel's

@@ -645,7 +645,7 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
             const uint32_t lim = __float_as_uint(qf(__uint_as_float(sum < 1e30f ? __float_as_uint(2.0f * hw) : 0xffffffffu)));
             const float ux = qf(-2.0f * bx), uy = qf(-2.0f * by);
             const float uz = KERNEL == OCT_KERNEL_SPHERE ? qf(-2.0f * bz) : 0.0f;
-            uint32_t c8 = 0, mnb = 0xffffffffu;
+            uint32_t c8 = 0, c8b = 0, mnb = 0xffffffffu;
             for (uint32_t base = first; base < last; base += CAP) {
                 const uint32_t P = (last - base) < (uint32_t)CAP ? (last - base) : (uint32_t)CAP;
                 const uint32_t iters = stage(base, P, lgs);
@@ -666,15 +666,16 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
                     }
                     const uint32_t d0 = __float_as_uint(t0), d1 = __float_as_uint(t1);
                     const uint32_t d2 = __float_as_uint(t2), d3 = __float_as_uint(t3);
+                    // two accumulators: each v_bcnt adds into its own (one add3 fewer per 4 tests)
                     c8 = __builtin_popcount(__builtin_amdgcn_perm(d0, d1, 0x0c0c0b09u)) + c8;
-                    c8 = __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u)) + c8;
+                    c8b = __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u)) + c8b;
                     mnb = min(min(mnb, d0), d1);
                     mnb = min(min(mnb, d2), d3);
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
-            lc = c8 >> 3;
+            lc = (c8 + c8b) >> 3;
             const bool band = ract && mnb <= lim;
             if (ballot64(band)) {  // rare: exact FP64 re-count of this lane's share of the leaf
                 const Query qr = fetch_query();

@@ -10,6 +10,7 @@
 #include <iterator>
 #include <optional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "encoding/hilbert_encoder_3d.hpp"
@@ -46,6 +47,37 @@ static void run_kernel(const LinearOctree<PointsAoS> &oct, const std::vector<Poi
         for (auto e : s) it += (e.second.getX() == e.second.getX());
         if (it != s.size()) std::printf("iterator mismatch %zu %zu\n", it, s.size());
     }
+}
+
+// Per-query calls from 8 threads at once (the reference harness' OpenMP loop over queries): the
+// batched launches must give every caller its own results, the same as sequential calls.
+static void check_concurrent(const LinearOctree<PointsAoS> &oct, const std::vector<Point> &centers, double r) {
+    const size_t reps = 40, nq = centers.size() * reps;
+    std::vector<std::vector<std::pair<size_t, size_t>>> st(nq);
+    std::vector<std::vector<size_t>> pr(nq), kn(nq);
+    auto one = [&](size_t i, std::vector<std::pair<size_t, size_t>> &a, std::vector<size_t> &b, std::vector<size_t> &c) {
+        const Point &p = centers[i % centers.size()];
+        const double ri = r * (1.0 + 0.25 * (double)(i % 3));  // radii differ within a batch
+        a = oct.neighborsStruct<Kernel_t::sphere>(p, ri).ranges;
+        b = oct.neighborsPrune<Kernel_t::cube>(p, Vector(ri, 0.5 * ri, ri));
+        std::vector<double> d2(5 + i % 4);
+        c.assign(d2.size(), 0);
+        oct.knn(p, d2.size(), c, d2);
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < 8; t++)
+        th.emplace_back([&, t] {
+            for (size_t i = (size_t)t; i < nq; i += 8) one(i, st[i], pr[i], kn[i]);
+        });
+    for (auto &x : th) x.join();
+    size_t bad = 0;
+    for (size_t i = 0; i < nq; i += 7) {
+        std::vector<std::pair<size_t, size_t>> a;
+        std::vector<size_t> b, c;
+        one(i, a, b, c);
+        bad += (a != st[i]) + (b != pr[i]) + (c != kn[i]);
+    }
+    std::printf("concurrent %zu %zu\n", (nq + 6) / 7, bad);
 }
 
 static bool same_set(const NeighborSet<PointsAoS> &a, const NeighborSet<PointsAoS> &b) {
@@ -227,6 +259,7 @@ int main(int argc, char **argv) {
     check_kernel_objects<Kernel_t::cube>(oct, points, centers, r, "cube");
     check_kernel_objects<Kernel_t::square>(oct, points, centers, r, "square");
     if (!metadata) check_soa(read_f64(argv[3]), points, codes, oct, centers, r, enc);
+    check_concurrent(oct, centers, r);
     PointEncoding::key_t k = enc.encode(1000, 2000, 3000);
     PointEncoding::coords_t x, y, z;
     enc.decode(k, x, y, z);

@@ -2,8 +2,9 @@
 // and search signatures, backed by the gfx950 build and searches through the C ABI. Each octree
 // owns one device context (its cloud, keys and tree stay resident on the GPU).
 //
-// Per-query calls run as batches of one (exact, as the reference); the batch calls at the end
-// (neighborsCount, neighborsCountAll, knnBatch) are the throughput API. Results are identical to
+// Per-query calls made concurrently from several threads (the reference harness' OpenMP loop) are
+// coalesced into batched launches (octree_amd_batcher.hpp); the batch calls at the end
+// (neighborsCount, neighborsCountAll) are the throughput API. Results are identical to
 // the reference's: indices into the sorted cloud, lists in the reference's emission order.
 #pragma once
 
@@ -19,6 +20,7 @@
 #include "../geometry/box.hpp"
 #include "../kernels/kernel_factory.hpp"
 #include "../main_options.hpp"
+#include "../octree_amd_batcher.hpp"
 #include "../octree_amd_points.hpp"
 #include "../octree_amd_runtime.hpp"
 #include "neighbor_set.hpp"
@@ -39,7 +41,8 @@ class LinearOctree {
 public:
     explicit LinearOctree(Container &points, std::vector<key_t> &codes, Box box, PointEncoding::PointEncoder &enc,
                           std::shared_ptr<BuildLog> log = nullptr)
-        : points_(points), enc_(enc), codes_(codes), box_(box), ctx_(new octamd::Context()) {
+        : points_(points), enc_(enc), codes_(codes), box_(box), ctx_(new octamd::Context()),
+          batcher_(new octamd::SearchBatcher(ctx_->get(), ctx_->mutex())) {
         if (codes.size() != points.size()) throw std::invalid_argument("octree_amd: codes and points differ in size");
         const double ctr[3] = {box.center().getX(), box.center().getY(), box.center().getZ()};
         const double rad[3] = {box.radii().getX(), box.radii().getY(), box.radii().getZ()};
@@ -158,17 +161,15 @@ public:
     size_t knn(const Point &p, const size_t k, std::vector<size_t> &indexes, std::vector<double> &dists) const {
         const size_t m = std::min(points_.size(), k);
         if (m == 0) return 0;
-        std::vector<int32_t> idx(k);
-        std::vector<double> d2(k);
-        const double c[3] = {p.getX(), p.getY(), p.getZ()};
-        {
-            std::lock_guard<std::mutex> g(ctx_->mutex());
-            octamd::check(ctx(), oct_knn(ctx(), OCT_PREC_FP64, c, nullptr, 1, (uint32_t)k, idx.data(), d2.data(), nullptr),
-                          "oct_knn");
-        }
+        octamd::SearchRequest r;
+        r.kind = octamd::SearchRequest::KNN;
+        r.precision = OCT_PREC_FP64;
+        r.k = (uint32_t)k;
+        r.c[0] = p.getX(), r.c[1] = p.getY(), r.c[2] = p.getZ();
+        batcher_->run(r);
         if (indexes.size() < m) indexes.resize(m);
         if (dists.size() < m) dists.resize(m);
-        for (size_t i = 0; i < m; i++) indexes[i] = (size_t)idx[i], dists[i] = d2[i];
+        for (size_t i = 0; i < m; i++) indexes[i] = (size_t)r.knn_idx[i], dists[i] = r.knn_d2[i];
         return m;
     }
 
@@ -244,6 +245,7 @@ private:
     std::vector<key_t> &codes_;
     Box box_;
     std::unique_ptr<octamd::Context> ctx_;
+    std::unique_ptr<octamd::SearchBatcher> batcher_;  // per-query calls from many threads share launches
     oct_tree_info info_{};
     mutable std::vector<Point> centers_;
     mutable std::vector<std::pair<size_t, size_t>> ranges_;
@@ -258,23 +260,16 @@ private:
         int algo;
         std::vector<uint32_t> entries;
     };
+    // one query's lists, through the batcher: concurrent callers share one launch
     Lists list(int kernel, int algo, const Point &p, double radius, const Vector *radii) const {
-        const double c[3] = {p.getX(), p.getY(), p.getZ()};
-        std::lock_guard<std::mutex> g(ctx_->mutex());
-        if (radii) {
-            const double r3[3] = {radii->getX(), radii->getY(), radii->getZ()};
-            octamd::check(ctx(), oct_radius_box(ctx(), kernel, algo, c, nullptr, nullptr, r3, 1, 1, nullptr, nullptr),
-                          "oct_radius_box");
-        } else {
-            octamd::check(ctx(), oct_radius(ctx(), kernel, algo, kPrecision, c, nullptr, nullptr, radius, 1, 1, nullptr, nullptr),
-                          "oct_radius");
-        }
-        uint64_t offs[2];
-        octamd::check(ctx(), oct_radius_fetch(ctx(), offs, nullptr, 0), "oct_radius_fetch");
-        Lists l{algo, std::vector<uint32_t>((size_t)offs[1] * (algo == OCT_ALGO_STRUCT ? 2 : 1))};
-        if (!l.entries.empty())
-            octamd::check(ctx(), oct_radius_fetch(ctx(), offs, l.entries.data(), l.entries.size()), "oct_radius_fetch");
-        return l;
+        octamd::SearchRequest r;
+        r.kind = radii ? octamd::SearchRequest::BOX : octamd::SearchRequest::RADIUS;
+        r.kernel = kernel, r.algo = algo, r.precision = kPrecision;
+        r.c[0] = p.getX(), r.c[1] = p.getY(), r.c[2] = p.getZ();
+        r.radius = radius;
+        if (radii) r.r3[0] = radii->getX(), r.r3[1] = radii->getY(), r.r3[2] = radii->getZ();
+        batcher_->run(r);
+        return Lists{algo, std::move(r.entries)};
     }
     // a kernel object's search: the ball shapes by their radius, the box shapes by their radii
     Lists list_k(const KernelAbstract &k, int algo) const {

@@ -215,6 +215,16 @@ void orc_sort(uint64_t *keys, uint32_t *perm, size_t n, int nthreads) {
     free(hist);
 }
 
+/* The points reordered by the sort's permutation (sortPoints moves the Points with their keys,
+ * point_encoder.hpp:359-375): out[i] = xyz[perm[i]], OpenMP over the output. */
+void orc_gather3(const double *xyz, const uint32_t *perm, size_t n, double *out) {
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; i++) {
+        const double *p = &xyz[3 * (size_t)perm[i]];
+        out[3 * i] = p[0], out[3 * i + 1] = p[1], out[3 * i + 2] = p[2];
+    }
+}
+
 /* ------------------------------------------------------------------ key algebra
  * point_encoder.hpp:542-645 */
 static inline uint32_t clz64(uint64_t x) { return x ? (uint32_t)__builtin_clzll(x) : 64u; }
@@ -661,6 +671,152 @@ static int kern_box_overlap(const kern_t *k, const double *nc, const double *nr)
     if (nc[0] - nr[0] > k->bmax[0] || nc[1] - nr[1] > k->bmax[1] || (is3d && nc[2] - nr[2] > k->bmax[2]))
         return 0;
     return 1;
+}
+
+/* ------------------------------------------------------------------ pointer octree (competitor)
+ * The reference's pointer-based Octree (inc/structures/octree.hpp:20-396, src/octree.cpp:17-322),
+ * restated as a CPU baseline: built by inserting the points one by one (buildOctree -> insertPoints,
+ * octree.cpp:207-243); a leaf already holding maxPointsLeaf points splits into 8 octants
+ * (createOctants, :245-257: child i has x +/- radii.x * 0.5F by bit 4, y by bit 2, z by bit 1, radii
+ * halved) and hands its points to them (fillOctants, :259-270; octantIdx, :272-283: x >= centre.x ->
+ * bit 4, ...). Searches (neighbors, octree.hpp:156-188): depth first over the octants whose box
+ * passes the kernel's boxOverlap, every point of a reached leaf tested with isInside. The reference
+ * splits with no depth limit (MIN_OCTANT_RADIUS = 0), so more than maxPointsLeaf identical points
+ * would recurse without end; here splitting stops at depth 64 (the leaf then keeps every point). */
+typedef struct {
+    double c[3], r[3];
+    int64_t child;  /* index of the first of 8 children, -1 for a leaf */
+    uint32_t *pts, n, cap;
+} pnode_t;
+
+struct orc_ptree {
+    pnode_t *nodes;
+    size_t nn, capn;
+    const double *xyz;
+    uint32_t max_leaf;
+    uint32_t depth;
+};
+
+static size_t ptree_new_node(orc_ptree *t, const double c[3], const double r[3]) {
+    if (t->nn == t->capn) {
+        t->capn = t->capn ? 2 * t->capn : 1024;
+        t->nodes = (pnode_t *)realloc(t->nodes, t->capn * sizeof(pnode_t));
+    }
+    pnode_t *p = &t->nodes[t->nn];
+    memcpy(p->c, c, sizeof p->c);
+    memcpy(p->r, r, sizeof p->r);
+    p->child = -1, p->pts = NULL, p->n = p->cap = 0;
+    return t->nn++;
+}
+static void ptree_push(pnode_t *p, uint32_t i) {
+    if (p->n == p->cap) {
+        p->cap = p->cap ? 2 * p->cap : 8;
+        p->pts = (uint32_t *)realloc(p->pts, p->cap * sizeof(uint32_t));
+    }
+    p->pts[p->n++] = i;
+}
+static inline unsigned ptree_octant(const orc_ptree *t, size_t node, uint32_t i) {
+    const double *p = &t->xyz[3 * (size_t)i];
+    const pnode_t *n = &t->nodes[node];
+    return (p[0] >= n->c[0] ? 4u : 0u) | (p[1] >= n->c[1] ? 2u : 0u) | (p[2] >= n->c[2] ? 1u : 0u);
+}
+static void ptree_insert(orc_ptree *t, size_t node, uint32_t i, uint32_t depth) {
+    for (;;) {
+        pnode_t *n = &t->nodes[node];
+        if (n->child >= 0) {
+            node = (size_t)n->child + ptree_octant(t, node, i);
+            depth++;
+            continue;
+        }
+        if (n->n == 0 || n->n < t->max_leaf || depth >= 64) {
+            ptree_push(n, i);
+            if (depth > t->depth) t->depth = depth;
+            return;
+        }
+        /* createOctants + fillOctants, then insert into the octant of i */
+        double c[3] = {n->c[0], n->c[1], n->c[2]}, r[3] = {n->r[0], n->r[1], n->r[2]};
+        const size_t first = t->nn;
+        for (unsigned o = 0; o < 8; o++) {
+            double cc[3], rr[3];
+            cc[0] = c[0] + r[0] * ((o & 4u) ? 0.5f : -0.5f);
+            cc[1] = c[1] + r[1] * ((o & 2u) ? 0.5f : -0.5f);
+            cc[2] = c[2] + r[2] * ((o & 1u) ? 0.5f : -0.5f);
+            rr[0] = r[0] * 0.5f, rr[1] = r[1] * 0.5f, rr[2] = r[2] * 0.5f;
+            ptree_new_node(t, cc, rr);
+        }
+        n = &t->nodes[node]; /* the array may have moved */
+        n->child = (int64_t)first;
+        uint32_t *old = n->pts, cnt = n->n;
+        n->pts = NULL, n->n = n->cap = 0;
+        for (uint32_t k = 0; k < cnt; k++) ptree_insert(t, first + ptree_octant(t, node, old[k]), old[k], depth + 1);
+        free(old);
+        node = first + ptree_octant(t, node, i);
+        depth++;
+    }
+}
+
+orc_ptree *orc_ptree_build(const double *xyz, size_t n, uint32_t max_points_leaf, const double center[3],
+                           const double radii[3]) {
+    orc_ptree *t = (orc_ptree *)calloc(1, sizeof(orc_ptree));
+    t->xyz = xyz;
+    t->max_leaf = max_points_leaf;
+    ptree_new_node(t, center, radii);
+    for (size_t i = 0; i < n; i++) ptree_insert(t, 0, (uint32_t)i, 0);
+    return t;
+}
+
+void orc_ptree_free(orc_ptree *t) {
+    if (!t) return;
+    for (size_t i = 0; i < t->nn; i++) free(t->nodes[i].pts);
+    free(t->nodes);
+    free(t);
+}
+
+void orc_ptree_info(const orc_ptree *t, uint64_t info[3]) {
+    uint64_t leaves = 0;
+    for (size_t i = 0; i < t->nn; i++) leaves += t->nodes[i].child < 0;
+    info[0] = leaves, info[1] = t->nn - leaves, info[2] = t->depth;
+}
+
+/* neighbors(kernel, condition) (octree.hpp:156-188) for nq centres of one radius; sizes out, and the
+ * index lists when out_idx != NULL (offsets from the exclusive prefix of sizes, filled by the caller's
+ * second call). Returns the total. */
+uint64_t orc_ptree_radius(const orc_ptree *t, int kernel, const double *centers, size_t nq, double radius,
+                          uint64_t *sizes, const uint64_t *offsets, uint64_t *out_idx) {
+    uint64_t total = 0;
+#pragma omp parallel reduction(+ : total)
+    {
+        size_t cap = 256, sp;
+        size_t *stack = (size_t *)malloc(cap * sizeof(size_t));
+#pragma omp for schedule(dynamic, 64)
+        for (size_t q = 0; q < nq; q++) {
+            kern_t k;
+            kern_init(&k, kernel, &centers[3 * q], radius);
+            uint64_t cnt = 0;
+            sp = 0;
+            stack[sp++] = 0;
+            while (sp) {
+                const pnode_t *n = &t->nodes[stack[--sp]];
+                if (n->child < 0) {
+                    for (uint32_t j = 0; j < n->n; j++)
+                        if (kern_inside(&k, &t->xyz[3 * (size_t)n->pts[j]])) {
+                            if (out_idx) out_idx[offsets[q] + cnt] = n->pts[j];
+                            cnt++;
+                        }
+                } else {
+                    if (sp + 8 > cap) cap *= 2, stack = (size_t *)realloc(stack, cap * sizeof(size_t));
+                    for (unsigned o = 0; o < 8; o++) {
+                        const pnode_t *ch = &t->nodes[n->child + o];
+                        if (kern_box_overlap(&k, ch->c, ch->r)) stack[sp++] = (size_t)n->child + o;
+                    }
+                }
+            }
+            sizes[q] = cnt;
+            total += cnt;
+        }
+        free(stack);
+    }
+    return total;
 }
 
 /* ------------------------------------------------------------------ traversal + searches */

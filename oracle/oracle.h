@@ -76,6 +76,8 @@ void orc_encode(int enc, const double *xyz, size_t n, const double center[3], co
 /* sortPoints radix passes (point_encoder.hpp:285-383): keys sorted in place, perm[i] = original
  * index of the point now at position i. nthreads emulates the per-thread histogram layout. */
 void orc_sort(uint64_t *keys, uint32_t *perm, size_t n, int nthreads);
+/* out[i] = xyz[perm[i]] (n x 3 doubles): the points moved with their keys (point_encoder.hpp:359-375). */
+void orc_gather3(const double *xyz, const uint32_t *perm, size_t n, double *out);
 
 /* ---------------- linear octree (inc/structures/linear_octree.hpp) ---------------- */
 typedef struct orc_tree orc_tree;
@@ -134,6 +136,20 @@ uint64_t orc_knn_brute(const double *sorted_xyz, size_t n, const double *centers
                        uint32_t k, int64_t *idx, double *d2);
 
 int orc_max_threads(void);
+
+/* Competitor baseline: the reference's pointer-based Octree (inc/structures/octree.hpp:20-396,
+ * src/octree.cpp:17-322) over xyz (n x 3, kept by reference), insertion-built from the root box
+ * (center, radii); orc_ptree_info: leaves, internal nodes, depth. orc_ptree_radius: its neighbors()
+ * search (boxOverlap descent, isInside per leaf point) for nq centres of one radius, OpenMP over the
+ * centres; sizes[q] per centre; with out_idx and offsets (exclusive prefix of sizes) the index lists
+ * in the reference's emission order. Returns the total. */
+typedef struct orc_ptree orc_ptree;
+orc_ptree *orc_ptree_build(const double *xyz, size_t n, uint32_t max_points_leaf, const double center[3],
+                           const double radii[3]);
+void orc_ptree_free(orc_ptree *t);
+void orc_ptree_info(const orc_ptree *t, uint64_t info[3]);
+uint64_t orc_ptree_radius(const orc_ptree *t, int kernel, const double *centers, size_t nq, double radius,
+                          uint64_t *sizes, const uint64_t *offsets, uint64_t *out_idx);
 
 /* LAS ingest as the reference's reader sees it (readers/las_file_reader_parallel.hpp:52-64, 74-95,
  * 112-137 through LASlib). orc_las_header: info[0] version minor, [1] point format, [2] record length,

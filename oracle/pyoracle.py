@@ -39,6 +39,7 @@ def lib():
             getattr(L, f).argtypes = [C.c_uint64, _U64P]
         L.orc_encode.argtypes = [C.c_int, _F64P, C.c_size_t, _F64P, _F64P, _U64P]
         L.orc_sort.argtypes = [_U64P, _U32P, C.c_size_t, C.c_int]
+        L.orc_gather3.argtypes = [_F64P, _U32P, C.c_size_t, _F64P]
         L.orc_build.argtypes = [_U64P, C.c_size_t, C.c_uint32, C.c_int, _F64P, _F64P]
         L.orc_build.restype = _P
         L.orc_tree_free.argtypes = [_P]
@@ -74,6 +75,12 @@ def lib():
         L.orc_knn_brute.argtypes = [_F64P, C.c_size_t, _F64P, C.c_size_t, C.c_uint32, _I64P, _F64P]
         L.orc_knn_brute.restype = C.c_uint64
         L.orc_max_threads.restype = C.c_int
+        L.orc_ptree_build.argtypes = [_F64P, C.c_size_t, C.c_uint32, _F64P, _F64P]
+        L.orc_ptree_build.restype = _P
+        L.orc_ptree_free.argtypes = [_P]
+        L.orc_ptree_info.argtypes = [_P, _U64P]
+        L.orc_ptree_radius.argtypes = [_P, C.c_int, _F64P, C.c_size_t, C.c_double, _U64P, C.c_void_p, C.c_void_p]
+        L.orc_ptree_radius.restype = C.c_uint64
         L.orc_las_header.argtypes = [C.c_char_p, _U64P, _F64P]
         L.orc_las_read.argtypes = [C.c_char_p, _F64P, _F64P, np.ctypeslib.ndpointer(np.uint8, flags="C"),
                                    np.ctypeslib.ndpointer(np.uint16, flags="C"), _U32P]
@@ -124,6 +131,15 @@ def sort(keys, nthreads=1):
     perm = np.zeros(k.shape[0], np.uint32)
     lib().orc_sort(k, perm, k.shape[0], nthreads)
     return k, perm
+
+
+def gather3(xyz, perm):
+    """xyz[perm] (n x 3), on every host core."""
+    xyz = _f64(xyz).reshape(-1, 3)
+    perm = np.ascontiguousarray(perm, np.uint32)
+    out = np.empty((perm.shape[0], 3))
+    lib().orc_gather3(xyz, perm, perm.shape[0], out)
+    return out
 
 
 def sort_points(enc, xyz):
@@ -351,4 +367,39 @@ class NanoflannTree:
     def __del__(self):
         if getattr(self, "h", None):
             _nf().refnf_free(self.h)
+            self.h = None
+
+
+# ------------------------------------------------------------------ competitor baseline (reference's pointer octree, restated)
+class PointerOctree:
+    """The reference's pointer-based Octree (inc/structures/octree.hpp, src/octree.cpp) restated in the
+    oracle (orc_ptree_*): insertion-built with leaves of at most max_leaf points, searched by its
+    neighbors() (boxOverlap descent, isInside per point); a CPU competitor baseline."""
+
+    def __init__(self, xyz, max_leaf, center, radii):
+        self.xyz = _f64(xyz).reshape(-1, 3)  # kept alive: the tree reads the caller's points
+        self.h = lib().orc_ptree_build(self.xyz, self.xyz.shape[0], max_leaf, _f64(center), _f64(radii))
+        info = np.zeros(3, np.uint64)
+        lib().orc_ptree_info(self.h, info)
+        self.n_leaf, self.n_internal, self.depth = (int(v) for v in info)
+
+    def radius_counts(self, kernel, centers, radius):
+        centers = _f64(centers).reshape(-1, 3)
+        sizes = np.zeros(centers.shape[0], np.uint64)
+        lib().orc_ptree_radius(self.h, kernel, centers, centers.shape[0], float(radius), sizes, None, None)
+        return sizes
+
+    def radius(self, kernel, centers, radius):
+        """Index lists in the reference's emission order."""
+        centers = _f64(centers).reshape(-1, 3)
+        sizes = self.radius_counts(kernel, centers, radius)
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        out = np.zeros(max(int(offs[-1]), 1), np.uint64)
+        lib().orc_ptree_radius(self.h, kernel, centers, centers.shape[0], float(radius), sizes.copy(),
+                               offs.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p))
+        return [out[offs[q]:offs[q + 1]].astype(np.int64) for q in range(centers.shape[0])]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_ptree_free(self.h)
             self.h = None

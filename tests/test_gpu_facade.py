@@ -56,7 +56,7 @@ def parse(text):
             raise AssertionError(line)
         elif f[0] == "kobj_ok":
             out.setdefault("kobj_ok", {})[f[1]] = int(f[2])
-        elif f[0] in ("encodepoints", "getcenter", "soa"):
+        elif f[0] in ("encodepoints", "getcenter", "soa", "concurrent"):
             out[f[0]] = tuple(int(v) for v in f[1:])
     return out
 
@@ -125,6 +125,10 @@ def test_facade_matches_oracle(enc):
                               "square": 10 * len(centers)}
     # the same cloud as PointsSoA: same order, keys, tree and lists
     assert got["soa"] == (len(pts), len(pts), 1, 1, len(centers))
+    # per-query calls from 8 threads, coalesced into batched launches: every caller gets its own
+    # results (Struct ranges, Prune indices, kNN indices), equal to sequential calls
+    checked, bad = got["concurrent"]
+    assert checked > 0 and bad == 0
 
 
 def test_facade_reads_las(tmp_path):

@@ -559,3 +559,33 @@ def test_nanoflann_baseline_agrees_with_oracle():
     bi, bd = O.knn_brute(sx, centers, 12)
     ni, nd = nf.knn(centers, 12)
     np.testing.assert_array_equal(nd, bd)
+
+
+def test_pointer_octree_competitor_agrees_with_oracle():
+    """The reference's pointer octree restated as a competitor (orc_ptree_*): its neighbors() lists
+    hold exactly the brute-force sets for all four kernels, its counts equal the linear octree's, its
+    leaves hold at most maxPointsLeaf points (the insertion rule of octree.cpp:217-243) and every point
+    sits in exactly one leaf; more identical points than a leaf holds stop splitting at depth 64."""
+    rng = np.random.default_rng(33)
+    pts = np.vstack([rng.uniform(0, 30, (8000, 3)) * [1, 1, 0.2], rng.normal(size=(800, 3)) * 0.3 + 5,
+                     np.repeat([[7.0, 7.0, 1.0]], 50, axis=0)])
+    sx, keys, perm, c, r = O.sort_points(O.ENC_HILBERT, pts)
+    tree = O.Tree(keys, 32, O.ENC_HILBERT, c, r)
+    pt = O.PointerOctree(sx, 32, c, r)
+    assert pt.n_leaf > 0 and pt.n_internal > 0 and pt.n_leaf == 7 * pt.n_internal + 1
+    centers = np.vstack([sx[rng.integers(0, len(sx), 200)], [[7.0, 7.0, 1.0]]])
+    for kernel in (O.K_SPHERE, O.K_CUBE, O.K_CIRCLE, O.K_SQUARE):
+        for radius in (0.4, 1.5):
+            lists = pt.radius(kernel, centers, radius)
+            brute, _ = O.radius_brute(sx, kernel, centers, radius)
+            for q in range(len(centers)):
+                np.testing.assert_array_equal(np.sort(lists[q]), np.sort(brute[q]), err_msg=f"kernel {kernel} q {q}")
+            np.testing.assert_array_equal(pt.radius_counts(kernel, centers, radius),
+                                          O.radius_count(tree, sx, kernel, O.A_NEIGHBORS, centers, radius))
+    # 40 identical points over a leaf size of 8: splitting stops at depth 64, the leaf keeps them all
+    dup = np.vstack([rng.uniform(0, 1, (100, 3)), np.repeat([[0.5, 0.5, 0.5]], 40, axis=0)])
+    c2, r2 = O.mbb(dup)
+    pd = O.PointerOctree(dup, 8, c2, r2)
+    assert pd.depth == 64
+    got = pd.radius_counts(O.K_SPHERE, np.array([[0.5, 0.5, 0.5]]), 1e-9)
+    assert int(got[0]) == 40
