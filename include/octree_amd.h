@@ -151,6 +151,21 @@ int oct_get_tree(oct_ctx *ctx, uint64_t *prefixes, uint32_t *offsets, uint64_t *
 int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *centers, const uint32_t *center_idx,
                const double *radii, double radius, size_t nq, int want_lists, uint32_t *counts_out,
                oct_search_stats *stats);
+/* Allocation-free variant of oct_radius(want_lists = 1) + oct_radius_fetch, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference. It calls no hipMalloc / hipFree: the queries
+ * (center_idx_dev, nq sorted-cloud indices on the device, or NULL for points 0 .. nq-1) and every
+ * output are the caller's device buffers -- counts_dev (nq neighbour counts), offsets_dev (nq + 1
+ * list offsets, in entries: indices, or [first, last) ranges for OCT_ALGO_STRUCT, u64), entries_dev
+ * (u32 words: one per index, two per range) -- and the scan of the list sizes keeps its partial sums
+ * in a fixed device array of the library (64 KiB per GPU). Limit: nq <= 67,108,864 per call. The
+ * list total (indices or ranges) is returned in *total_out; an entries_capacity (u32 words) below
+ * what the lists need fails with OCT_ERR_ARG after counts, offsets and *total_out are written, so
+ * the caller can size its buffer and call again. Calls that overlap on one GPU (several contexts on
+ * different streams) race on the shared scratch array. */
+int oct_radius_lists_noalloc(oct_ctx *ctx, int kernel, int algo, int precision, const uint32_t *center_idx_dev,
+                             double radius, size_t nq, uint32_t *counts_dev, uint64_t *offsets_dev, uint32_t *entries_dev,
+                             size_t entries_capacity, uint64_t *total_out);
 /* Vector-radii overloads (linear_octree.hpp:1044-1060, 1069-1075, 1082-1087; kernel_factory.hpp:33-41):
  * cube or square kernels with per-axis half extents, box = centre -/+ radii (kernel_abstract.hpp:39-45).
  * radii3: nq x 3 per-query half extents, or NULL to use radius3 for every query. Queries and lists

@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstdio>
+#include <mutex>
 #include <numeric>
+#include <string>
 
 #include "device.hpp"
 
@@ -128,6 +130,88 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
     }
     for (; i < words; i += stride) dst[i] = src[i];
 }
+
+// ---- scratch of the allocation-free entry point (oct_radius_lists_noalloc)
+// The in-place exclusive scan of the per-query entry counts keeps one partial sum per block of
+// kNaChunk queries in `scratch`, a file-scope device array: each GPU holds its own copy, whose address
+// oct_create resolves once per device (hipGetSymbolAddress, that device current) into g_scratch. Every
+// call on a GPU shares that GPU's array; calls that overlap on one GPU race on it.
+constexpr uint32_t kNaChunk = 8192;                                          // queries per scan block
+constexpr uint32_t kNaMaxBlocks = 8192;                                      // partial sums held
+constexpr size_t kNoAllocMaxQueries = (size_t)kNaChunk * kNaMaxBlocks;       // 67,108,864 queries per call
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint64_t scratch[kNaMaxBlocks];
+std::mutex g_scratch_mu;
+uint64_t *g_scratch[kMaxDevices] = {};
+
+// Resolves this device's scratch address (the device must be current). Idempotent.
+hipError_t resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipSuccess;  // such a device has no allocation-free entry
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch[device] = (uint64_t *)p;
+    return e;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x, int lane) {
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+// Exclusive prefix over a block of 256 threads (4 waves): returns the thread's exclusive prefix and
+// the block total.
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, uint64_t &total) {
+    __shared__ uint64_t wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t inc = wave_incl_scan(x, lane);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint64_t base = 0;
+    for (int i = 0; i < w; i++) base += wsum[i];
+    total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return base + inc - x;
+}
+// pass 1: part[b] = sum of v[b * kNaChunk .. ) (32 consecutive values per thread)
+__global__ __launch_bounds__(256) void k_na_sums(const uint64_t *__restrict__ v, size_t n, uint64_t *__restrict__ part) {
+    const size_t lo = (size_t)blockIdx.x * kNaChunk + (size_t)threadIdx.x * 32;
+    uint64_t x = 0;
+    for (size_t i = lo; i < lo + 32 && i < n; i++) x += v[i];
+    uint64_t total;
+    (void)block_excl_scan(x, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+// pass 2 (one block): part[0 .. nb) -> exclusive prefixes in place; the grand total to *total
+__global__ __launch_bounds__(256) void k_na_parts(uint64_t *__restrict__ part, uint32_t nb, uint64_t *__restrict__ total) {
+    uint64_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint64_t x = b < nb ? part[b] : 0;
+        uint64_t t;
+        const uint64_t e = block_excl_scan(x, t);
+        if (b < nb) part[b] = carry + e;
+        carry += t;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+// pass 3: each block's values -> exclusive prefixes in place, offset by its partial
+__global__ __launch_bounds__(256) void k_na_scan(uint64_t *__restrict__ v, size_t n, const uint64_t *__restrict__ part) {
+    const size_t lo = (size_t)blockIdx.x * kNaChunk + (size_t)threadIdx.x * 32;
+    uint64_t x = 0;
+    for (size_t i = lo; i < lo + 32 && i < n; i++) x += v[i];
+    uint64_t total;
+    uint64_t run = part[blockIdx.x] + block_excl_scan(x, total);
+    for (size_t i = lo; i < lo + 32 && i < n; i++) {
+        const uint64_t c = v[i];
+        v[i] = run;
+        run += c;
+    }
+}
+__global__ void k_na_zero(uint64_t *p) { *p = 0; }
 
 // Copies n points given as three host coordinate arrays into the packed device array.
 static int upload_xyz_soa(oct_ctx *ctx, const double *x, const double *y, const double *z, size_t n) {
@@ -277,6 +361,10 @@ int oct_create(oct_ctx **out, int device) {
             delete ctx;
             return OCT_ERR_HIP;
         }
+    }
+    if (resolve_scratch(device) != hipSuccess) {
+        delete ctx;
+        return OCT_ERR_HIP;
     }
     if (ctx->derr.reserve(4) != hipSuccess || hipMemset(ctx->derr.p, 0, 4 * sizeof(uint32_t)) != hipSuccess) {
         ctx->derr.release();
@@ -624,6 +712,62 @@ int oct_radius(oct_ctx *ctx, int kernel, int algo, int precision, const double *
     if (algo == OCT_ALGO_APPROX) return fail(ctx, OCT_ERR_ARG, "neighborsApprox runs through oct_radius_approx");
     return radius_impl(ctx, kernel, algo, precision, centers, center_idx, radii, radius, nullptr, nullptr, nq, want_lists,
                        counts_out, stats);
+}
+
+int oct_radius_lists_noalloc(oct_ctx *ctx, int kernel, int algo, int precision, const uint32_t *center_idx_dev,
+                             double radius, size_t nq, uint32_t *counts_dev, uint64_t *offsets_dev, uint32_t *entries_dev,
+                             size_t entries_capacity, uint64_t *total_out) {
+    if (!ctx) return OCT_ERR_ARG;
+    int s = check_search_args(ctx, kernel, algo, precision);
+    if (s) return s;
+    if (!counts_dev || !offsets_dev || !total_out) return fail(ctx, OCT_ERR_ARG, "counts, offsets and total are required");
+    if (nq > kNoAllocMaxQueries)
+        return fail(ctx, OCT_ERR_ARG, "at most " + std::to_string(kNoAllocMaxQueries) + " queries per allocation-free call");
+    if (!center_idx_dev && nq > ctx->n) return fail(ctx, OCT_ERR_ARG, "all-points search with nq > n");
+    if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch[ctx->device])
+        return fail(ctx, OCT_ERR_STATE, "no scratch resolved for this device");
+    uint64_t *part = g_scratch[ctx->device];
+    HCHK(hipSetDevice(ctx->device), "set device");
+    *total_out = 0;
+    // sizes: per-query neighbour counts and entry counts (indices, or ranges for Struct)
+    RadiusLaunch L;
+    L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 1;
+    L.qidx = center_idx_dev;
+    L.radius = radius;
+    L.nq = nq;
+    L.counts = counts_dev;
+    L.ecounts64 = offsets_dev;
+    HCHK(launch_radius(ctx, L), "radius sizes");
+    // offsets: exclusive prefix of the entry counts, in place; offsets[nq] = total
+    if (nq) {
+        const uint32_t nb = (uint32_t)((nq + kNaChunk - 1) / kNaChunk);
+        k_na_sums<<<nb, 256, 0, ctx->stream>>>(offsets_dev, nq, part);
+        k_na_parts<<<1, 256, 0, ctx->stream>>>(part, nb, offsets_dev + nq);
+        k_na_scan<<<nb, 256, 0, ctx->stream>>>(offsets_dev, nq, part);
+    } else {
+        k_na_zero<<<1, 1, 0, ctx->stream>>>(offsets_dev);
+    }
+    HCHK(hipGetLastError(), "offsets");
+    uint64_t total = 0;
+    HCHK(hipMemcpyAsync(&total, offsets_dev + nq, 8, hipMemcpyDeviceToHost, ctx->stream), "copy total");
+    HCHK(hipStreamSynchronize(ctx->stream), "radius sizes");
+    if ((s = check_device_error(ctx))) return s;
+    *total_out = total;
+    const size_t words = (size_t)total * (algo == OCT_ALGO_STRUCT ? 2 : 1);
+    if (words > entries_capacity)
+        return fail(ctx, OCT_ERR_ARG, "entries_capacity " + std::to_string(entries_capacity) + " is below the " +
+                                          std::to_string(words) + " words the lists need (counts and offsets are valid)");
+    if (words && !entries_dev) return fail(ctx, OCT_ERR_ARG, "null entries buffer");
+    // fill: every query's list at its offset
+    L.mode = 2;
+    L.counts = nullptr;
+    L.ecounts64 = nullptr;
+    L.offsets = offsets_dev;
+    L.entries = entries_dev;
+    if (words) HCHK(launch_radius(ctx, L), "radius fill");
+    HCHK(hipStreamSynchronize(ctx->stream), "radius fill");
+    ctx->has_lists = false;
+    return check_device_error(ctx);
 }
 
 int oct_radius_box(oct_ctx *ctx, int kernel, int algo, const double *centers, const uint32_t *center_idx,

@@ -202,6 +202,7 @@ struct RadiusLaunch {
     size_t nq = 0;
     uint32_t *counts = nullptr;
     uint32_t *ecounts = nullptr;
+    uint64_t *ecounts64 = nullptr;
     const uint64_t *offsets = nullptr;
     uint32_t *entries = nullptr;
     unsigned long long *stats = nullptr;

@@ -296,6 +296,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
         if (MODE != 2) {
             if (a.counts) a.counts[qid] = sink.cnt;
             if (MODE == 1 && a.ecounts) a.ecounts[qid] = sink.ecnt;
+            if (MODE == 1 && a.ecounts64) a.ecounts64[qid] = sink.ecnt;
         }
     }
     if (a.stats && lane == 0) {
@@ -1039,6 +1040,7 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     a.nq = (uint32_t)L.nq;
     a.counts = L.counts;
     a.ecounts = L.ecounts;
+    a.ecounts64 = L.ecounts64;
     a.offsets = L.offsets;
     a.entries = L.entries;
     a.stats = L.stats;

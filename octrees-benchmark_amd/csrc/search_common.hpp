@@ -45,6 +45,7 @@ struct SearchArgs {
     uint32_t nq;
     uint32_t *__restrict__ counts;
     uint32_t *__restrict__ ecounts;
+    uint64_t *__restrict__ ecounts64;  // list sizes pass: entries per query as u64 (scanned in place)
     const uint64_t *__restrict__ offsets;
     uint32_t *__restrict__ entries;
     unsigned long long *__restrict__ stats;

@@ -30,7 +30,7 @@ EXPORTS = (
     "oct_batch_build_device", "oct_batch_set_points", "oct_batch_cloud_sorted", "oct_batch_cloud_tree",
     "oct_batch_queries", "oct_batch_pipeline", "oct_batch_pipeline_results", "oct_batch_autotune_knn",
     "oct_las_read_header", "oct_las_load", "oct_las_fetch", "oct_gather_records", "oct_set_points_soa",
-    "oct_set_sorted_soa", "oct_knn_checksum", "oct_copy_bandwidth",
+    "oct_set_sorted_soa", "oct_knn_checksum", "oct_copy_bandwidth", "oct_radius_lists_noalloc",
 )
 
 # oct_point_meta (per-point LAS attributes, PointMetadata)
@@ -110,6 +110,8 @@ def lib():
         L.oct_knn_fetch.argtypes = [_vp, C.c_size_t, C.c_size_t, _vp, _vp]
         L.oct_knn_checksum.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_uint32, _vp]
         L.oct_copy_bandwidth.argtypes = [_vp, C.c_size_t, C.POINTER(C.c_double)]
+        L.oct_radius_lists_noalloc.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_double, C.c_size_t, _vp, _vp,
+                                               _vp, C.c_size_t, C.POINTER(C.c_uint64)]
         L.oct_set_points_soa.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t]
         L.oct_set_sorted_soa.argtypes = [_vp, _vp, _vp, _vp, C.c_size_t, _vp, C.c_int, _vp, _vp]
         L.oct_radius_approx.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_double, _vp, C.c_size_t, C.c_uint32,
