@@ -199,6 +199,17 @@ __device__ inline uint64_t lanemask_lt() {
     uint32_t l = lane_id();
     return l ? (~0ull >> (64 - l)) : 0ull;
 }
+// v of lane (lane ^ OFF) for OFF < 32 (the xor mode of ds_swizzle within 32-lane halves: no lane-index
+// arithmetic), and of lane (lane ^ 32) with a ds_bpermute at the address (lane ^ 32) * 4. Every lane
+// of the wave active.
+template <int OFF>
+__device__ inline uint32_t lane_xor(uint32_t v) {
+    static_assert(OFF > 0 && OFF < 32 && (OFF & (OFF - 1)) == 0, "a power of two below 32");
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (OFF << 10) | 0x1f);
+}
+__device__ inline uint32_t lane_xor32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lane_id() ^ 32u) << 2), (int)v);
+}
 __device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ inline uint32_t readlane_u32(uint32_t v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 __device__ inline double readlane_f64(double v, int lane) {

@@ -710,7 +710,11 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
         }
         if (!ract) lc = 0;
         if (rm) {
-            for (uint32_t off = 1u << lg; off < 64; off <<= 1) lc += __shfl_xor(lc, (int)off);
+            // butterfly over the lanes that served one compacted query (lane % 2^lg equal, lg >= 2)
+            if (lg <= 2) lc += lane_xor<4>(lc);
+            if (lg <= 3) lc += lane_xor<8>(lc);
+            if (lg <= 4) lc += lane_xor<16>(lc);
+            lc += lane_xor32(lc);
             if (ract && gp == 0) s_rtot[w][lane] = lc;  // lane = r for the group's first lane
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -870,8 +874,9 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
             leaf(ov, E.amax[c], ur);
         }
     }
-#pragma unroll
-    for (int off = G; off < 64; off <<= 1) cnt += __shfl_xor(cnt, off);
+    // the S lanes of a query hold parts of its count: lanes j, j + G, ... (G >= 16)
+    if constexpr (G <= 16) cnt += lane_xor<16>(cnt);
+    if constexpr (G <= 32) cnt += lane_xor32(cnt);
     if (MX) {
         // column totals: lanes c16, c16 + 16, c16 + 32, c16 + 48 hold query 16h + c16's row groups;
         // lane j (< G) has c16 = j % 16 and takes block h = j / 16
