@@ -238,9 +238,19 @@ int oct_knn_checksum(oct_ctx *ctx, const int32_t *idx_dev, const float *d2_dev, 
  * restores "autotuned, else the default". oct_autotune_knn times the offered shapes on sample_nq
  * all-points queries (16 strided chunks, as above; 0 = 2^18), stores the fastest for (precision,
  * size class of k), returns it in *best and the best milliseconds per shape in ms_out[0..2].
- * oct_tuned_knn: the lanes per query a kNN call with this k would use now. Results never depend on
- * the tuning. */
-enum oct_tune_knob { OCT_TUNE_RADIUS_GROUP = 1, OCT_TUNE_RADIUS_MFMA = 2, OCT_TUNE_KNN_LANES = 3, OCT_TUNE_RADIUS_REMAP = 4 };
+ * oct_tuned_knn: the lanes per query a kNN call with this k would use now.
+ * OCT_TUNE_SORT: how oct_sort sorts more than 4096 (8192) keys: 1 (default, also -1) partitions
+ * them by their top 16 bits in two device-wide passes and sorts each segment inside one workgroup's
+ * LDS (4096 keys per LDS sort), 2 the same with 8192-key LDS sorts, 0 runs the device-wide LSD pass
+ * per 8-bit digit; a segment above 32768 keys, or a key at or above 2^63, selects the LSD passes.
+ * Results never depend on the tuning. */
+enum oct_tune_knob {
+    OCT_TUNE_RADIUS_GROUP = 1,
+    OCT_TUNE_RADIUS_MFMA = 2,
+    OCT_TUNE_KNN_LANES = 3,
+    OCT_TUNE_RADIUS_REMAP = 4,
+    OCT_TUNE_SORT = 5
+};
 int oct_set_tuning(oct_ctx *ctx, int knob, int value);
 int oct_autotune_radius(oct_ctx *ctx, int kernel, int algo, int precision, double radius, size_t sample_nq, int *best,
                         double ms_out[4]);

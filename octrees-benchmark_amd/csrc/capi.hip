@@ -884,6 +884,11 @@ int oct_set_tuning(oct_ctx *ctx, int knob, int value) {
         ctx->remap_override = value;
         return OCT_OK;
     }
+    if (knob == OCT_TUNE_SORT) {
+        if (value < -1 || value > 2) return fail(ctx, OCT_ERR_ARG, "sort switch must be -1, 0, 1 or 2");
+        ctx->sort_override = value;
+        return OCT_OK;
+    }
     if (knob == OCT_TUNE_KNN_LANES) {
         if (value != -1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32 && value != 64)
             return fail(ctx, OCT_ERR_ARG, "knn lanes per query must be -1, 2, 4, 8, 16, 32 or 64");

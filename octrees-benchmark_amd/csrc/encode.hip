@@ -2,8 +2,8 @@
 //
 // k_bbox:   FP64 min/max per axis (box.hpp:94-137); 24 B/pt streamed once.
 // k_encode: per point FP64 quantisation (point_encoder.hpp:37-49) + Morton / Hilbert key, and the
-//           digit histograms of all radix passes in the same read (so the sort needs no separate
-//           histogram pass). 24 B/pt read + 8 B/pt written.
+//           digit histograms of all radix passes, and of bits 55..62 for the partition sort, in the
+//           same read (so the sort needs no separate histogram pass). 24 B/pt read + 8 B/pt written.
 #include <cfloat>
 
 #include "device.hpp"
@@ -91,6 +91,7 @@ hipError_t launch_bbox(oct_ctx *ctx, double mn[3], double mx[3]) {
 // ------------------------------------------------------------------ encode + digit histograms
 constexpr int kEncBlock = 256;
 constexpr int kRadixPasses = 8;
+static_assert(kRadixPasses == kRadixPassCount, "one histogram per LSD pass");
 
 struct EncodeBox {
     double c[3], r[3];
@@ -99,10 +100,10 @@ struct EncodeBox {
 template <int ENC>
 __global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__ xyz, size_t n, EncodeBox box,
                                                       uint64_t *__restrict__ keys, uint32_t *__restrict__ hist) {
-    __shared__ uint32_t sh[kRadixPasses * 256];
+    __shared__ uint32_t sh[kEncHists * 256];
     __shared__ uint16_t stab[48 * 8];
     __shared__ uint16_t stab2[kHilbertTab2];
-    for (int i = threadIdx.x; i < kRadixPasses * 256; i += blockDim.x) sh[i] = 0;
+    for (int i = threadIdx.x; i < kEncHists * 256; i += blockDim.x) sh[i] = 0;
     if (ENC == OCT_ENC_HILBERT3D)
         for (int i = threadIdx.x; i < 48 * 8; i += blockDim.x) stab[i] = kHilbert.t[i];
     __syncthreads();
@@ -118,9 +119,10 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__
         keys[i] = k;
 #pragma unroll
         for (int p = 0; p < kRadixPasses; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+        atomicAdd(&sh[kRadixPasses * 256 + ((k >> kMsdShift1) & 255u)], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kRadixPasses * 256; i += blockDim.x)
+    for (int i = threadIdx.x; i < kEncHists * 256; i += blockDim.x)
         if (sh[i]) atomicAdd(&hist[i], sh[i]);
 }
 
@@ -163,11 +165,11 @@ hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], cons
     size_t n = ctx->n;
     hipError_t e;
     if ((e = ctx->keys[0].reserve(n)) != hipSuccess) return e;
-    if ((e = ctx->hist.reserve(kRadixPasses * 256)) != hipSuccess) return e;
+    if ((e = ctx->hist.reserve(kEncHists * 256)) != hipSuccess) return e;
     EncodeBox box;
     for (int a = 0; a < 3; a++) box.c[a] = center[a], box.r[a] = radii[a];
     stage_begin(ctx, ST_ENCODE);
-    e = hipMemsetAsync(ctx->hist.p, 0, kRadixPasses * 256 * sizeof(uint32_t), ctx->stream);
+    e = hipMemsetAsync(ctx->hist.p, 0, kEncHists * 256 * sizeof(uint32_t), ctx->stream);
     if (e != hipSuccess) return e;
     if (n) {
         int nb = (int)((n + kEncBlock * 16 - 1) / (kEncBlock * 16));

@@ -68,6 +68,11 @@ struct SortTile {
 };
 // Per-cloud bin bases of a segmented sort: bases[cloud * kSegBaseStride + pass * 256 + digit].
 constexpr uint32_t kSegBaseStride = kRadixPassCount * 256;
+// Partition sort of one cloud (keys below 2^63): device-wide stable passes on bits 55..62 and 47..54,
+// then block sorts of the 16-bit segments, splitting big ones from bits 39..46 down. The encode kernel
+// writes the histogram of bits 55..62 after the 8 LSD digit histograms.
+constexpr uint32_t kMsdShift1 = 55, kMsdShift2 = 47, kMsdShift3 = 39;
+constexpr int kEncHists = kRadixPassCount + 1;
 
 // Bits of the context's device error word (oct_ctx::derr): kernels set them with atomicOr, the host
 // reads and clears the word at its next synchronisation and fails the call.
@@ -98,9 +103,13 @@ struct oct_ctx {
     double center[3] = {0, 0, 0}, radii[3] = {0, 0, 0};
     oct::DevBuf<uint64_t> keys[2];
     oct::DevBuf<uint32_t> vals[2];
-    oct::DevBuf<uint32_t> hist;      // 8 x 256 digit histograms (+ scratch)
+    oct::DevBuf<uint32_t> hist;      // 8 x 256 digit histograms, then bits 55..62
     oct::DevBuf<uint32_t> lookback;  // per pass, per tile, per bin
     oct::DevBuf<uint32_t> misc;      // tile counters, flags, scan partials
+    oct::DevBuf<oct::SortTile> msd_tiles;  // partition sort: tiles of the second pass
+    oct::DevBuf<uint32_t> msd;             // partition sort: histograms, bases, segment jobs, counters
+    int sort_override = -1;                // OCT_TUNE_SORT
+    bool p64_sorted = false;               // the last sort also wrote the sorted cloud (p64)
     uint64_t *sorted_keys = nullptr; // points into keys[0|1]
     uint32_t *perm = nullptr;        // points into vals[0|1]
     bool has_keys = false, has_sorted = false;

@@ -11,6 +11,8 @@
 //  - passes whose digit is the same for every key are skipped (the permutation is unchanged);
 //  - every look-back wait is bounded: on timeout the kernel raises an error flag and still exits.
 // Traffic per executed pass: 12 B/pt read + 12 B/pt written (+ 1 KB of look-back words per tile).
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace oct {
@@ -31,16 +33,17 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-// SEG = false: one key range [0, n), tile t covers [t * kSortTile, ...). SEG = true (batches): tiles
-// come from a table, each inside one cloud's range; the look-back stops at the cloud's first tile and
-// bin offsets are the cloud's own (bin_base + cloud * kSegBaseStride), so every cloud is sorted on
-// its own, stably, in one launch.
+// SEG = false: one key range [0, n), tile t covers [t * kSortTile, ...). SEG = true (batches, and the
+// second partition pass below): tiles come from a table, each inside one segment's range; the
+// look-back stops at the segment's first tile and bin offsets are the segment's own
+// (bin_base + segment * base_stride), so every segment is sorted on its own, stably, in one launch.
 template <bool SEG>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                          uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, size_t n,
                                                          uint32_t shift, const uint32_t *__restrict__ bin_base,
                                                          uint32_t *__restrict__ lookback, uint32_t *__restrict__ tile_counter,
-                                                         uint32_t *__restrict__ err, const SortTile *__restrict__ tiles) {
+                                                         uint32_t *__restrict__ err, const SortTile *__restrict__ tiles,
+                                                         uint32_t base_stride) {
     __shared__ uint64_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_wave[kSortBlock / 64][256];
@@ -86,13 +89,13 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
             uint64_t m = ballot64(bit);
             peers &= bit ? m : ~m;
         }
-        uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
-        uint32_t cnt = (uint32_t)__popcll(peers);
-        uint32_t prev = s_wave[w][d];
-        __builtin_amdgcn_wave_barrier();
-        if (valid && below == 0) s_wave[w][d] = prev + cnt;
-        __builtin_amdgcn_wave_barrier();
-        rank[r] = prev + below;
+        // the lowest lane of each digit group adds the group to the wave's digit count and hands the
+        // count before it to its group; a wave's LDS atomics run in program order, so rows rank in order
+        const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+        uint32_t prev = 0;
+        if (valid && below == 0) prev = atomicAdd(&s_wave[w][d], (uint32_t)__popcll(peers));
+        const uint32_t leader = peers ? (uint32_t)__builtin_ctzll(peers) : 0u;
+        rank[r] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)prev) + below;
     }
     __syncthreads();
 
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
             }
             store_flag(&lb[b], kFlagInc | ((excl + tile_count) & kValMask));
         }
-        s_global[b] = bin_base[(size_t)cloud * kSegBaseStride + b] + excl;
+        s_global[b] = bin_base[(size_t)cloud * base_stride + b] + excl;
     }  // owner
     __syncthreads();
 
@@ -182,6 +185,537 @@ __global__ void k_iota_copy(const uint64_t *__restrict__ kin, uint64_t *__restri
     }
 }
 
+// ------------------------------------------------------------------ partition sort
+//
+// For more keys than one workgroup sorts in LDS, all below 2^63 (both 3-D encoders' keys), the same
+// stable sort runs as two device-wide passes and one workgroup pass instead of eight device-wide passes:
+//  1. the onesweep pass above on bits 55..62 (its histogram comes from the encode kernel);
+//  2. a histogram of bits 47..54 inside each of those bins (k_msd_hist), their bin offsets and the
+//     workgroup jobs (k_msd_jobs), then a segmented onesweep pass on bits 47..54 inside each bin: the
+//     keys are now ordered by their top 16 bits;
+//  3. k_seg_sort: one workgroup per job sorts its range completely. A job is one 16-bit segment larger
+//     than the workgroup's LDS capacity, or a run of consecutive smaller segments of one bin that fits.
+//     A range that fits is sorted in LDS by LSD passes over only the digits, and inside a digit only
+//     the bits, that vary in it, each ranked stably with wave ballots. A larger segment is first split
+//     stably by its next 8 bits (39..46, then 31..38, ...) in LDS-sized tiles with running bin offsets
+//     (one workgroup owns the segment, so no look-back), until every piece fits.
+// Every step keeps equal keys in input order, so the permutation is the stable sort by key, as with
+// the 8 LSD passes. Traffic per key: 20 B (pass 1) + 8 B (histogram) + 24 B (pass 2) + 24 B (LDS
+// sort), plus 32 B for keys in segments that are split, against 188 B for eight passes.
+
+struct SegJob {
+    uint32_t start, len;
+};
+constexpr int kSegStack = 64;  // pieces awaiting a split: at most 7 per level x 6 levels for 2^15 keys
+
+template <int W, int IPT>
+struct SegShared {
+    static constexpr uint32_t kCap = (uint32_t)(W * 64 * IPT);
+    uint64_t keys[kCap];
+    uint32_t vals[kCap];
+    uint32_t cnt[W][256];  // per wave: digit counts, then the wave's first position inside each digit
+    uint32_t start[256];   // first position of each digit in the ranked range
+    uint32_t tot[256];     // keys per digit in the ranked range
+    uint32_t hist[256];    // split: keys per digit of the whole piece
+    uint32_t off[256];     // split: next output position of each digit
+    uint32_t wsum[W];
+    uint32_t mask_lo, mask_hi;  // OR of (key ^ first key) over a range
+    uint32_t or_d, and_d;       // split: OR / AND of the digit values present
+    uint32_t sp, nchunks, kind;
+    SegJob job;
+    uint4 stack[kSegStack];  // start, len, shift + 1 (0: no bits left, keys equal), buffer (0: a, 1: b)
+    SegJob chunks[256];      // split: runs of small pieces, each one LDS sort
+};
+
+// Greedy packing, by one wave, of 256 consecutive digit ranges (digit d = 4 * lane + slot: count
+// c[slot], start st[slot]), in digit order: a range longer than cap is reported alone (on_big), runs of
+// consecutive shorter ranges up to cap together (on_run); empty digits are skipped. Every lane runs the
+// same scalar loop, so the callbacks are wave-uniform.
+template <class OnBig, class OnRun>
+__device__ inline void pack_digit_ranges(const uint32_t (&c)[4], const uint32_t (&st)[4], uint32_t cap, OnBig &&on_big,
+                                         OnRun &&on_run) {
+    uint64_t nz[4];
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) nz[sl] = ballot64(c[sl] != 0u);
+    uint32_t rs = 0, rl = 0;
+    for (uint32_t ln = 0; ln < 64; ln++) {
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++) {
+            if (!((nz[sl] >> ln) & 1u)) continue;
+            const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)c[sl], (int)ln);
+            const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)st[sl], (int)ln);
+            if (l > cap) {
+                if (rl) on_run(rs, rl), rl = 0;
+                on_big(a, l);
+            } else {
+                if (rl + l > cap) on_run(rs, rl), rl = 0;
+                if (!rl) rs = a;
+                rl += l;
+            }
+        }
+    }
+    if (rl) on_run(rs, rl);
+}
+
+// Stable rank of the n keys a workgroup holds (thread (w, lane), row r: position w*64*IPT + r*64 + lane)
+// by the digit (key >> shift) & 255, with ballots over the digit bits in dbits only (the others are
+// equal for all n keys). Afterwards S.cnt[w][d] + rank[r] is the key's position in digit order,
+// S.start[d] the first position of digit d and S.tot[d] the number of keys with digit d.
+template <int W, int IPT>
+__device__ inline void seg_rank(SegShared<W, IPT> &S, const uint64_t (&k)[IPT], uint32_t n, uint32_t shift,
+                                uint32_t dbits, uint32_t (&rank)[IPT]) {
+    static_assert(W >= 4, "256 threads own the 256 digits");
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63, base = w * 64 * IPT;
+    for (uint32_t i = lane; i < 256; i += 64) S.cnt[w][i] = 0u;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        if (base + (uint32_t)r * 64 >= n) break;
+        const bool valid = base + (uint32_t)r * 64 + lane < n;
+        const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
+        uint64_t peers = ballot64(valid);
+        for (uint32_t bm = dbits; bm; bm &= bm - 1u) {
+            const bool bit = (d >> __builtin_ctz(bm)) & 1u;
+            const uint64_t m = ballot64(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+        uint32_t prev = 0;
+        if (valid && below == 0) prev = atomicAdd(&S.cnt[w][d], (uint32_t)__popcll(peers));
+        const uint32_t leader = peers ? (uint32_t)__builtin_ctzll(peers) : 0u;
+        rank[r] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)prev) + below;
+    }
+    __syncthreads();
+    uint32_t c[W], tot = 0, incl = 0;
+    if (tid < 256) {
+#pragma unroll
+        for (int ww = 0; ww < W; ww++) c[ww] = S.cnt[ww][tid], tot += c[ww];
+        S.tot[tid] = tot;
+        incl = wave_incl_scan(tot);
+        if (lane == 63) S.wsum[w] = incl;
+    }
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t pre = 0;
+        for (uint32_t ww = 0; ww < w; ww++) pre += S.wsum[ww];
+        uint32_t st = pre + incl - tot;
+        S.start[tid] = st;
+#pragma unroll
+        for (int ww = 0; ww < W; ww++) S.cnt[ww][tid] = st, st += c[ww];
+    }
+    __syncthreads();
+}
+
+// Stable sort of [start, start + n) of (ks, vs), 1 <= n <= kCap, into the same range of (kd, vd), which
+// may be the source: LSD passes over only the 8-bit digits, and inside a digit only the bits, that
+// vary in the range. With xyz, also writes the sorted cloud's records of the range to p64. The caller
+// synchronises the workgroup after it.
+template <int W, int IPT>
+__device__ void seg_local_sort(SegShared<W, IPT> &S, const uint64_t *ks, const uint32_t *vs, uint64_t *kd, uint32_t *vd,
+                               uint32_t start, uint32_t n, const double *xyz, double4 *p64) {
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63, base = w * 64 * IPT;
+    uint64_t k[IPT];
+    uint32_t v[IPT], rank[IPT];
+    if (tid == 0) S.mask_lo = S.mask_hi = 0u;
+    const uint64_t k0 = ks[start];
+    uint64_t m = 0;
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const uint32_t p = base + (uint32_t)r * 64 + lane;
+        k[r] = p < n ? ks[start + p] : 0ull;
+        v[r] = p < n ? vs[start + p] : 0u;
+        if (p < n) m |= k[r] ^ k0;
+    }
+    uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+    lo |= lane_xor<1>(lo), hi |= lane_xor<1>(hi);
+    lo |= lane_xor<2>(lo), hi |= lane_xor<2>(hi);
+    lo |= lane_xor<4>(lo), hi |= lane_xor<4>(hi);
+    lo |= lane_xor<8>(lo), hi |= lane_xor<8>(hi);
+    lo |= lane_xor<16>(lo), hi |= lane_xor<16>(hi);
+    lo |= lane_xor32(lo), hi |= lane_xor32(hi);
+    __syncthreads();
+    if (lane == 0) {
+        if (lo) atomicOr(&S.mask_lo, lo);
+        if (hi) atomicOr(&S.mask_hi, hi);
+    }
+    __syncthreads();
+    const uint64_t mask = ((uint64_t)S.mask_hi << 32) | S.mask_lo;
+    for (uint32_t sh = 0; sh < 64; sh += 8) {
+        const uint32_t dbits = (uint32_t)(mask >> sh) & 255u;
+        if (!dbits) continue;
+        seg_rank(S, k, n, sh, dbits, rank);
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const uint32_t p = base + (uint32_t)r * 64 + lane;
+            if (p < n) {
+                const uint32_t pos = S.cnt[w][(uint32_t)(k[r] >> sh) & 255u] + rank[r];
+                S.keys[pos] = k[r];
+                S.vals[pos] = v[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const uint32_t p = base + (uint32_t)r * 64 + lane;
+            if (p < n) k[r] = S.keys[p], v[r] = S.vals[p];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const uint32_t p = base + (uint32_t)r * 64 + lane;
+        if (p < n) kd[start + p] = k[r], vd[start + p] = v[r];
+    }
+    if (xyz) {  // the sorted cloud's records of the range, read through the permutation
+        double x[IPT], y[IPT], z[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            if (base + (uint32_t)r * 64 + lane < n) {
+                const double *q = xyz + 3 * (size_t)v[r];
+                x[r] = q[0], y[r] = q[1], z[r] = q[2];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const uint32_t p = base + (uint32_t)r * 64 + lane;
+            if (p < n) p64[start + p] = make_double4(x[r], y[r], z[r], 0.0);
+        }
+    }
+}
+
+// Complete sort of one 16-bit segment longer than kCap, [start, start + len) of buffer a, into the
+// same range of buffer b (buffer a's range is scratch): stable splits by the next 8 bits until every
+// piece fits, then LDS sorts of runs of consecutive pieces. A piece whose bits are used up holds
+// equal keys, already in input order.
+template <int W, int IPT>
+__device__ void seg_split_sort(SegShared<W, IPT> &S, uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb,
+                               uint32_t start, uint32_t len, uint32_t *err, const double *xyz, double4 *p64) {
+    constexpr uint32_t kCap = SegShared<W, IPT>::kCap;
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63, base = w * 64 * IPT;
+    if (tid == 0) {
+        S.stack[0] = make_uint4(start, len, kMsdShift3 + 1u, 0u);
+        S.sp = 1u;
+    }
+    __syncthreads();
+    for (;;) {
+        const uint32_t sp = S.sp;
+        if (sp == 0) break;
+        const uint4 e = S.stack[sp - 1];
+        __syncthreads();
+        if (tid == 0) S.sp = sp - 1;
+        uint64_t *ks = e.w ? kb : ka, *kt = e.w ? ka : kb;
+        uint32_t *vs = e.w ? vb : va, *vt = e.w ? va : vb;
+        if (e.z == 0) {
+            for (uint32_t i = tid; i < e.y; i += W * 64) {
+                const uint32_t val = e.w ? vb[e.x + i] : va[e.x + i];
+                if (e.w == 0) kb[e.x + i] = ka[e.x + i], vb[e.x + i] = val;
+                if (xyz) {
+                    const double *q = xyz + 3 * (size_t)val;
+                    p64[e.x + i] = make_double4(q[0], q[1], q[2], 0.0);
+                }
+            }
+            __syncthreads();
+            continue;
+        }
+        const uint32_t shift = e.z - 1u;
+        const uint32_t next1 = shift >= 8u ? shift - 7u : (shift > 0u ? 1u : 0u);  // next shift + 1
+        if (tid < 256) S.hist[tid] = 0u;
+        if (tid == 0) S.or_d = 0u, S.and_d = 255u;
+        __syncthreads();
+        for (uint32_t t0 = 0; t0 < e.y; t0 += kCap) {
+            uint32_t dg[IPT];
+#pragma unroll
+            for (int r = 0; r < IPT; r++) {
+                const uint32_t p = t0 + base + (uint32_t)r * 64 + lane;
+                dg[r] = p < e.y ? (uint32_t)(ks[e.x + p] >> shift) & 255u : 256u;
+            }
+#pragma unroll
+            for (int r = 0; r < IPT; r++)
+                if (dg[r] < 256u) atomicAdd(&S.hist[dg[r]], 1u);
+        }
+        __syncthreads();
+        uint32_t c = 0, incl = 0;
+        if (tid < 256) {
+            c = S.hist[tid];
+            if (c) atomicOr(&S.or_d, tid), atomicAnd(&S.and_d, tid);
+            incl = wave_incl_scan(c);
+            if (lane == 63) S.wsum[w] = incl;
+        }
+        __syncthreads();
+        if (tid < 256) {
+            uint32_t pre = 0;
+            for (uint32_t ww = 0; ww < w; ww++) pre += S.wsum[ww];
+            S.off[tid] = e.x + pre + incl - c;
+        }
+        __syncthreads();
+        const uint32_t dbits = S.or_d & ~S.and_d;
+        if (dbits == 0) {  // one digit value: nothing moves, go on with the next bits
+            if (tid == 0) S.stack[sp - 1] = make_uint4(e.x, e.y, next1, e.w), S.sp = sp;
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t t0 = 0; t0 < e.y; t0 += kCap) {
+            const uint32_t tn = e.y - t0 < kCap ? e.y - t0 : kCap;
+            uint64_t k[IPT];
+            uint32_t v[IPT], rank[IPT];
+#pragma unroll
+            for (int r = 0; r < IPT; r++) {
+                const uint32_t p = base + (uint32_t)r * 64 + lane;
+                k[r] = p < tn ? ks[e.x + t0 + p] : 0ull;
+                v[r] = p < tn ? vs[e.x + t0 + p] : 0u;
+            }
+            seg_rank(S, k, tn, shift, dbits, rank);
+#pragma unroll
+            for (int r = 0; r < IPT; r++) {
+                const uint32_t p = base + (uint32_t)r * 64 + lane;
+                if (p < tn) {
+                    const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
+                    const uint32_t pos = S.cnt[w][d] + rank[r];
+                    S.keys[pos] = k[r];
+                    S.vals[pos] = v[r];
+                }
+            }
+            __syncthreads();
+            for (uint32_t j = tid; j < tn; j += W * 64) {
+                const uint64_t key = S.keys[j];
+                const uint32_t d = (uint32_t)(key >> shift) & 255u;
+                const uint32_t dst = S.off[d] + (j - S.start[d]);
+                kt[dst] = key;
+                vt[dst] = S.vals[j];
+            }
+            __syncthreads();
+            if (tid < 256) S.off[tid] += S.tot[tid];
+            __syncthreads();
+        }
+        // pieces: digit d holds [off[d] - hist[d], off[d]) of buffer t; big ones go on the stack, runs
+        // of small ones become LDS sorts
+        if (w == 0) {
+            uint32_t c4[4], s4[4];
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++) c4[sl] = S.hist[4 * lane + sl], s4[sl] = S.off[4 * lane + sl] - c4[sl];
+            uint32_t nc = 0, top = sp - 1;
+            pack_digit_ranges(
+                c4, s4, kCap,
+                [&](uint32_t a, uint32_t l) {
+                    if (top < (uint32_t)kSegStack) {
+                        if (lane == 0) S.stack[top] = make_uint4(a, l, next1, e.w ^ 1u);
+                        top++;
+                    } else if (lane == 0) {
+                        atomicOr(err, 2u);
+                    }
+                },
+                [&](uint32_t a, uint32_t l) {
+                    if (lane == 0) S.chunks[nc] = SegJob{a, l};
+                    nc++;
+                });
+            if (lane == 0) S.nchunks = nc, S.sp = top;
+        }
+        __syncthreads();
+        const uint32_t nc = S.nchunks;
+        for (uint32_t ci = 0; ci < nc; ci++) {
+            const SegJob cj = S.chunks[ci];
+            seg_local_sort(S, kt, vt, kb, vb, cj.start, cj.len, xyz, p64);
+            __syncthreads();
+        }
+    }
+}
+
+// Persistent workgroups over the jobs of k_msd_jobs (the big ones first): the second partition
+// pass's output (ka, va) in, the sorted keys and permutation (kb, vb) out, and with xyz (n x 3 input
+// points) the sorted cloud p64 (the gather, overlapped with the LDS sorts of other workgroups).
+template <int W, int IPT>
+__global__ __launch_bounds__(W * 64, 4) void k_seg_sort(uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb,
+                                                     const SegJob *__restrict__ big, const SegJob *__restrict__ small,
+                                                     const uint32_t *__restrict__ njobs, uint32_t *__restrict__ next,
+                                                     uint32_t *__restrict__ err, const double *__restrict__ xyz,
+                                                     double4 *__restrict__ p64) {
+    __shared__ SegShared<W, IPT> S;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nbig = njobs[0], nsmall = njobs[1];
+    for (;;) {
+        if (tid == 0) {
+            uint32_t j = atomicAdd(&next[0], 1u);
+            if (j < nbig) {
+                S.job = big[j], S.kind = 0u;
+            } else {
+                j = atomicAdd(&next[1], 1u);
+                if (j < nsmall)
+                    S.job = small[j], S.kind = 1u;
+                else
+                    S.kind = 2u;
+            }
+        }
+        __syncthreads();
+        const uint32_t kind = S.kind;
+        const SegJob job = S.job;
+        __syncthreads();
+        if (kind == 2u) break;
+        if (job.len > SegShared<W, IPT>::kCap)
+            seg_split_sort(S, ka, va, kb, vb, job.start, job.len, err, xyz, p64);
+        else
+            seg_local_sort(S, ka, va, kb, vb, job.start, job.len, xyz, p64);
+        __syncthreads();
+    }
+}
+
+// Histogram of (key >> shift) & 255 per segment of a tile table: hist[segment * 256 + digit].
+__global__ __launch_bounds__(256) void k_msd_hist(const uint64_t *__restrict__ keys, const SortTile *__restrict__ tiles,
+                                                  uint32_t shift, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[256];
+    const uint32_t tid = threadIdx.x;
+    h[tid] = 0u;
+    __syncthreads();
+    const SortTile T = tiles[blockIdx.x];
+    constexpr int kPer = kSortTile / 256;
+    uint32_t dg[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const uint32_t i = (uint32_t)r * 256 + tid;
+        dg[r] = i < T.len ? (uint32_t)(keys[T.start + i] >> shift) & 255u : 256u;
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; r++)
+        if (dg[r] < 256u) atomicAdd(&h[dg[r]], 1u);
+    __syncthreads();
+    if (h[tid]) atomicAdd(&hist[(size_t)T.cloud * 256 + tid], h[tid]);
+}
+
+// One workgroup per first-level bin: the start of each of its second-level segments (the segmented
+// pass's bin bases, bases2[bin * 256 + digit]) and its jobs - a segment above cap alone (big), runs of
+// consecutive smaller segments up to cap together (small). njobs: [0] big, [1] small, [2] longest
+// segment.
+__global__ __launch_bounds__(256) void k_msd_jobs(const uint32_t *__restrict__ hist2, const uint32_t *__restrict__ base1,
+                                                  uint32_t cap, uint32_t *__restrict__ bases2, SegJob *__restrict__ big,
+                                                  SegJob *__restrict__ small, uint32_t *__restrict__ njobs) {
+    __shared__ uint32_t s_c[256], s_st[256], s_wsum[4], s_nb, s_ns, s_ob, s_os;
+    __shared__ SegJob s_big[256], s_small[256];
+    const uint32_t bin = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const uint32_t c = hist2[bin * 256 + tid];
+    const uint32_t incl = wave_incl_scan(c);
+    if (lane == 63) s_wsum[w] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t ww = 0; ww < w; ww++) pre += s_wsum[ww];
+    const uint32_t st = base1[bin] + pre + incl - c;
+    bases2[bin * 256 + tid] = st;
+    s_c[tid] = c, s_st[tid] = st;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t c4[4], s4[4];
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++) c4[sl] = s_c[4 * lane + sl], s4[sl] = s_st[4 * lane + sl];
+        uint32_t nb = 0, ns = 0, mx = 0;  // segments up to cap never exceed the longest big one's limit
+        pack_digit_ranges(
+            c4, s4, cap,
+            [&](uint32_t a, uint32_t l) {
+                if (lane == 0) s_big[nb] = SegJob{a, l};
+                nb++;
+                mx = l > mx ? l : mx;
+            },
+            [&](uint32_t a, uint32_t l) {
+                if (lane == 0) s_small[ns] = SegJob{a, l};
+                ns++;
+                mx = l > mx ? l : mx;
+            });
+        if (lane == 0) {
+            s_nb = nb, s_ns = ns;
+            s_ob = nb ? atomicAdd(&njobs[0], nb) : 0u;
+            s_os = ns ? atomicAdd(&njobs[1], ns) : 0u;
+            if (mx) atomicMax(&njobs[2], mx);
+        }
+    }
+    __syncthreads();
+    if (tid < s_nb) big[s_ob + tid] = s_big[tid];
+    if (tid < s_ns) small[s_os + tid] = s_small[tid];
+}
+
+namespace {
+
+// LDS sort shapes: OCT_TUNE_SORT 1 = 8 waves x 8 keys (4096), 2 = 16 waves x 8 keys (8192)
+constexpr uint32_t kSegMaxLen = 1u << 15;  // longest 16-bit segment the workgroup sort takes
+constexpr size_t kSegJobsMax = 256 * 256;  // jobs per kind: at most one per 16-bit segment
+
+// resident workgroups of k_seg_sort<W, IPT> over the device
+template <int W, int IPT>
+unsigned seg_grid() {
+    static unsigned cached = 0;
+    if (cached) return cached;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_seg_sort<W, IPT>, W * 64, 0);
+    cached = (unsigned)((cus > 0 ? cus : 256) * (per > 0 ? per : 1));
+    return cached;
+}
+
+// The partition sort (above) of the context's n encoded keys, keys[0] in, keys[1] / vals[1] out.
+// h9: the encode kernel's histograms. Returns with *done false, and keys[0] untouched, when a 16-bit
+// segment is longer than kSegMaxLen (one workgroup would sort it alone): the LSD passes run instead.
+template <int W, int IPT>
+hipError_t sort_partition(oct_ctx *ctx, const uint32_t *h9, uint32_t *counters, uint32_t *errflag, bool *done) {
+    constexpr uint32_t cap = SegShared<W, IPT>::kCap;
+    const size_t n = ctx->n;
+    const uint32_t *h1 = h9 + kRadixPassCount * 256;
+    uint32_t base1[256];
+    std::vector<SortTile> tl;
+    tl.reserve(n / kSortTile + 257);
+    uint32_t run = 0;
+    for (uint32_t d = 0; d < 256; d++) {
+        base1[d] = run;
+        const uint32_t first = (uint32_t)tl.size();
+        for (uint32_t p = 0; p < h1[d]; p += kSortTile)
+            tl.push_back(SortTile{run + p, std::min<uint32_t>(kSortTile, h1[d] - p), d, first});
+        run += h1[d];
+    }
+    const size_t tiles1 = (n + kSortTile - 1) / kSortTile, tiles2 = tl.size();
+    // msd layout (u32 words): base1 [256], hist2 [65536], bases2 [65536], njobs [4], big and small jobs
+    const size_t o_base1 = 0, o_hist2 = 256, o_bases2 = o_hist2 + 65536, o_njobs = o_bases2 + 65536, o_big = o_njobs + 4,
+                 o_small = o_big + 2 * kSegJobsMax, words = o_small + 2 * kSegJobsMax;
+    hipError_t e;
+    if ((e = ctx->msd.reserve(words)) != hipSuccess) return e;
+    if ((e = ctx->p64.reserve(n)) != hipSuccess) return e;
+    if ((e = ctx->msd_tiles.reserve(tiles2)) != hipSuccess) return e;
+    if ((e = ctx->lookback.reserve((tiles1 + tiles2) * 256 + 1)) != hipSuccess) return e;
+    uint32_t *M = ctx->msd.p;
+    SegJob *big = reinterpret_cast<SegJob *>(M + o_big), *small = reinterpret_cast<SegJob *>(M + o_small);
+    hipStream_t s = ctx->stream;
+    if ((e = hipMemcpyAsync(M + o_base1, base1, sizeof(base1), hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(ctx->msd_tiles.p, tl.data(), tiles2 * sizeof(SortTile), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return e;
+    if ((e = hipMemsetAsync(M + o_hist2, 0, 65536 * sizeof(uint32_t), s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(M + o_njobs, 0, 4 * sizeof(uint32_t), s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(ctx->lookback.p, 0, (tiles1 + tiles2) * 256 * sizeof(uint32_t), s)) != hipSuccess) return e;
+    // pass 1 on bits 55..62: keys[0] (values: the positions) -> keys[1] / vals[1]
+    k_onesweep<false><<<(unsigned)tiles1, kSortBlock, 0, s>>>(ctx->keys[0].p, nullptr, ctx->keys[1].p, ctx->vals[1].p, n,
+                                                              kMsdShift1, M + o_base1, ctx->lookback.p, counters + 0,
+                                                              errflag, nullptr, 0u);
+    k_msd_hist<<<(unsigned)tiles2, 256, 0, s>>>(ctx->keys[1].p, ctx->msd_tiles.p, kMsdShift2, M + o_hist2);
+    k_msd_jobs<<<256, 256, 0, s>>>(M + o_hist2, M + o_base1, cap, M + o_bases2, big, small, M + o_njobs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    uint32_t nj[4] = {0, 0, 0, 0};
+    if ((e = hipMemcpyAsync(nj, M + o_njobs, sizeof(nj), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    if (nj[2] > kSegMaxLen) {
+        *done = false;
+        return hipSuccess;
+    }
+    // pass 2 on bits 47..54 inside each first-level bin: keys[1] / vals[1] -> keys[0] / vals[0]
+    k_onesweep<true><<<(unsigned)tiles2, kSortBlock, 0, s>>>(ctx->keys[1].p, ctx->vals[1].p, ctx->keys[0].p, ctx->vals[0].p,
+                                                             n, kMsdShift2, M + o_bases2, ctx->lookback.p + tiles1 * 256,
+                                                             counters + 1, errflag, ctx->msd_tiles.p, 256u);
+    const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>(seg_grid<W, IPT>(), (size_t)nj[0] + nj[1]));
+    k_seg_sort<W, IPT><<<grid, W * 64, 0, s>>>(ctx->keys[0].p, ctx->vals[0].p, ctx->keys[1].p, ctx->vals[1].p, big, small,
+                                               M + o_njobs, counters + 2, errflag, ctx->xyz.p, ctx->p64.p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    ctx->p64_sorted = true;
+    ctx->sorted_keys = ctx->keys[1].p;
+    ctx->perm = ctx->vals[1].p;
+    *done = true;
+    return hipSuccess;
+}
+
+}  // namespace
+
 hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     const size_t n = ctx->n;
     if (n >= (size_t)kValMask) {
@@ -193,11 +727,12 @@ hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     if ((e = ctx->vals[0].reserve(n)) != hipSuccess) return e;
     if ((e = ctx->vals[1].reserve(n)) != hipSuccess) return e;
     // histograms of all passes (from the encode kernel)
-    uint32_t h[8 * 256];
+    uint32_t h[kEncHists * 256];
     if ((e = hipMemcpyAsync(h, ctx->hist.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
     int passes[8], np = 0;
     uint32_t base[8 * 256];
+    bool below63 = true;  // every key below 2^63
     for (int p = 0; p < 8; p++) {
         uint32_t run = 0;
         bool trivial = false;
@@ -205,6 +740,7 @@ hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
             base[p * 256 + d] = run;
             run += h[p * 256 + d];
             if (h[p * 256 + d] == n) trivial = true;
+            if (p == 7 && d >= 128 && h[p * 256 + d]) below63 = false;
         }
         if (!trivial && n > 1) passes[np++] = p;
     }
@@ -213,35 +749,48 @@ hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     // misc layout: [0..16) tile counters, [16] error flag, [32..32+2048) bin bases
     if ((e = ctx->misc.reserve(12 * 1024 + 64)) != hipSuccess) return e;
     uint32_t *counters = ctx->misc.p, *errflag = ctx->misc.p + 16, *bases = ctx->misc.p + 32;
+    ctx->p64_sorted = false;
     stage_begin(ctx, ST_SORT);
-    if ((e = hipMemcpyAsync(bases, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(counters, 0, 32 * sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
-    if (np && (e = hipMemsetAsync(ctx->lookback.p, 0, (size_t)np * tiles * 256 * sizeof(uint32_t), ctx->stream)) != hipSuccess)
-        return e;
-    int src = 0;
-    for (int i = 0; i < np; i++) {
-        int p = passes[i];
-        const uint32_t *vin = (i == 0) ? nullptr : ctx->vals[src].p;
-        k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, ctx->stream>>>(
-            ctx->keys[src].p, vin, ctx->keys[src ^ 1].p, ctx->vals[src ^ 1].p, n, 8u * p, bases + p * 256,
-            ctx->lookback.p + (size_t)i * tiles * 256, counters + i, errflag, nullptr);
-        src ^= 1;
+    // partition sort by default for more keys than one LDS sort holds (OCT_TUNE_SORT: 0 = LSD passes,
+    // 1 / 2 = partition sort with 4096- / 8192-key LDS sorts)
+    const int mode = ctx->sort_override < 0 ? 1 : ctx->sort_override;
+    bool done = false;
+    if (mode > 0 && below63 && n > (mode == 2 ? 8192u : 4096u)) {
+        e = mode == 2 ? sort_partition<16, 8>(ctx, h, counters, errflag, &done)
+                      : sort_partition<8, 8>(ctx, h, counters, errflag, &done);
+        if (e != hipSuccess) return e;
     }
-    if (np == 0) {
-        int nb = (int)((n + 255) / 256);
-        nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
-        k_iota_copy<<<nb, 256, 0, ctx->stream>>>(ctx->keys[0].p, ctx->keys[1].p, ctx->vals[1].p, n);
-        src = 1;
+    if (!done) {
+        if ((e = hipMemcpyAsync(bases, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(counters, 0, 32 * sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
+        if (np && (e = hipMemsetAsync(ctx->lookback.p, 0, (size_t)np * tiles * 256 * sizeof(uint32_t), ctx->stream)) != hipSuccess)
+            return e;
+        int src = 0;
+        for (int i = 0; i < np; i++) {
+            int p = passes[i];
+            const uint32_t *vin = (i == 0) ? nullptr : ctx->vals[src].p;
+            k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, ctx->stream>>>(
+                ctx->keys[src].p, vin, ctx->keys[src ^ 1].p, ctx->vals[src ^ 1].p, n, 8u * p, bases + p * 256,
+                ctx->lookback.p + (size_t)i * tiles * 256, counters + i, errflag, nullptr, 0u);
+            src ^= 1;
+        }
+        if (np == 0) {
+            int nb = (int)((n + 255) / 256);
+            nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
+            k_iota_copy<<<nb, 256, 0, ctx->stream>>>(ctx->keys[0].p, ctx->keys[1].p, ctx->vals[1].p, n);
+            src = 1;
+        }
+        ctx->sorted_keys = ctx->keys[src].p;
+        ctx->perm = ctx->vals[src].p;
     }
     stage_end(ctx, ST_SORT);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    ctx->sorted_keys = ctx->keys[src].p;
-    ctx->perm = ctx->vals[src].p;
     uint32_t flag = 0;
     if ((e = hipMemcpyAsync(&flag, errflag, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
     if (flag) {
-        err = "sort: look-back wait exceeded its bound";
+        err = (flag & 2u) ? "sort: a segment split overflowed its stack" : "sort: look-back wait exceeded its bound";
         return hipErrorUnknown;
     }
     return hipSuccess;
@@ -257,7 +806,7 @@ hipError_t launch_sort_segmented(hipStream_t s, uint64_t *keys0, uint64_t *keys1
         const int src = p & 1;
         k_onesweep<true><<<(unsigned)ntiles, kSortBlock, 0, s>>>(kb[src], p == 0 ? nullptr : vb[src], kb[src ^ 1], vb[src ^ 1], n,
                                                                 8u * p, bases + p * 256, lookback + (size_t)p * ntiles * 256,
-                                                                counters + p, err, tiles);
+                                                                counters + p, err, tiles, kSegBaseStride);
     }
     return hipGetLastError();
 }
@@ -272,7 +821,7 @@ hipError_t launch_sort_passes(hipStream_t s, uint64_t *keys0, uint64_t *keys1, u
         const int src = p & 1;
         k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, s>>>(kb[src], p == 0 ? nullptr : vb[src], kb[src ^ 1], vb[src ^ 1], n,
                                                                  8u * p, bases + p * 256, lookback + (size_t)p * tiles * 256,
-                                                                 counters + p, err, nullptr);
+                                                                 counters + p, err, nullptr, 0u);
     }
     return hipGetLastError();
 }
@@ -325,7 +874,7 @@ hipError_t launch_gather(oct_ctx *ctx) {
     hipError_t e;
     if ((e = ctx->p64.reserve(ctx->n)) != hipSuccess) return e;
     stage_begin(ctx, ST_GATHER);
-    if (ctx->n) {
+    if (ctx->n && !ctx->p64_sorted) {  // the partition sort writes the sorted cloud itself
         const unsigned nb = (unsigned)((ctx->n + 255) / 256);  // one point per thread (launch_gather_raw)
         k_gather<<<nb, 256, 0, ctx->stream>>>(ctx->xyz.p, ctx->perm, ctx->p64.p, ctx->n);
     }
