@@ -608,15 +608,20 @@ def main():
                 "work": f"9 flops x {T} point tests of the reference traversal"}
 
     es_ms = med(bbox_ms) + med(enc_ms) + med(sort_ms) + med(gather_ms)
-    # algorithmic bytes of bbox + encode + sort + gather: 24 + 32 + 8 (histogram) + P * 24 (key + index read
-    # and written per pass) + 52 (gather) B/pt with 8-bit digits, P = 8 passes
+    # algorithmic bytes of bbox + encode + sort + gather as the reference's algorithm moves them: 24 + 32 +
+    # 8 (histogram) + P * 24 (key + index read and written per pass) + 52 (gather) B/pt with 8-bit digits,
+    # P = 8 passes. The partition sort moves fewer: 24 + 32 + 20 + 24 (two passes) + 24 (LDS sorts) + 56
+    # (gather, fused into them) B/pt, plus 32 B for keys in 16-bit segments that are split.
     es_bytes = (116 + 24 * 8) * n
     es_gbs = es_bytes / (es_ms * 1e-3) / 1e9
     copy_gbs, _ = ctx.copy_bandwidth(1 << 30)  # the practical streaming ceiling, measured in this run
     log(f"[bench] streaming copy ceiling {copy_gbs:.0f} GB/s")
     roofline_es = {"bound": "hbm", "achieved": es_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": es_gbs / PEAK_HBM_GBS,
-                   "traffic": None, "kernel": "bbox + encode + 8 radix passes + gather",
-                   "work": f"{116 + 24 * 8} B/pt x {n} points", "copy_ceiling_gbs": copy_gbs,
+                   "traffic": None,
+                   "kernel": "bbox + encode + partition sort (2 device-wide passes on the top 16 key bits, "
+                             "per-segment LDS sorts that also gather the sorted cloud)",
+                   "work": f"{116 + 24 * 8} B/pt x {n} points (8-pass LSD sort + gather, the reference's algorithm)",
+                   "path_work": "180 B/pt + 32 B per key in a split 16-bit segment", "copy_ceiling_gbs": copy_gbs,
                    "frac_of_copy_ceiling": es_gbs / copy_gbs}
 
     out = {

@@ -119,16 +119,15 @@ __global__ void k_knn_checksum(const int32_t *__restrict__ idx, const float *__r
     }
 }
 
-// Streaming copy of 16-byte words, four in flight per thread: the device's practical read + write
-// ceiling that the HBM-bound stages are measured against.
-__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t words) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < words; i += 4 * stride) {
-        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-        dst[i] = a, dst[i + stride] = b, dst[i + 2 * stride] = c, dst[i + 3 * stride] = d;
-    }
-    for (; i < words; i += stride) dst[i] = src[i];
+// Streaming copy, one 16-byte word per thread over the whole grid, non-temporal loads and stores: the
+// fastest plain copy measured on this device (6.2-6.5 TB/s for 1 GiB against 4.3-4.8 TB/s for
+// grid-stride loops of 4-8 words per thread), so the practical read + write ceiling that the
+// HBM-bound stages are measured against.
+typedef unsigned int copy_word_t __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy16(const copy_word_t *__restrict__ src, copy_word_t *__restrict__ dst,
+                                                size_t words) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
 }
 
 // ---- scratch of the allocation-free entry point (oct_radius_lists_noalloc)
@@ -1083,10 +1082,9 @@ int oct_copy_bandwidth(oct_ctx *ctx, size_t bytes, double *ms_out) {
     HCHK(ctx->rec_in.reserve(bytes), "alloc copy source");
     HCHK(ctx->rec_out.reserve(bytes), "alloc copy target");
     HCHK(hipMemsetAsync(ctx->rec_in.p, 1, bytes, ctx->stream), "fill copy source");
-    int cus = 0;
-    HCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "device attribute");
     const size_t words = bytes / 16;
-    const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((words + 255) / 256, (size_t)cus * 8));
+    if ((words + 255) / 256 > 0x7fffffffu) return fail(ctx, OCT_ERR_ARG, "copy size above 2^31 workgroups of 4 KiB");
+    const unsigned nb = (unsigned)((words + 255) / 256);
     hipEvent_t e0, e1;
     HCHK(hipEventCreate(&e0), "event");
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -1098,7 +1096,7 @@ int oct_copy_bandwidth(oct_ctx *ctx, size_t bytes, double *ms_out) {
     for (int rep = 0; rep < 6 && err == hipSuccess; rep++) {  // the first repetition warms up
         err = hipEventRecord(e0, ctx->stream);
         if (err == hipSuccess) {
-            k_copy16<<<nb, 256, 0, ctx->stream>>>((const uint4 *)ctx->rec_in.p, (uint4 *)ctx->rec_out.p, words);
+            k_copy16<<<nb, 256, 0, ctx->stream>>>((const copy_word_t *)ctx->rec_in.p, (copy_word_t *)ctx->rec_out.p, words);
             err = hipGetLastError();
         }
         if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);

@@ -2,8 +2,8 @@
 //
 // k_bbox:   FP64 min/max per axis (box.hpp:94-137); 24 B/pt streamed once.
 // k_encode: per point FP64 quantisation (point_encoder.hpp:37-49) + Morton / Hilbert key, and the
-//           digit histograms of all radix passes, and of bits 55..62 for the partition sort, in the
-//           same read (so the sort needs no separate histogram pass). 24 B/pt read + 8 B/pt written.
+//           digit histograms of all radix passes, and of bits 55..62 and 47..54 for the partition
+//           sort, in the same read (so the sort needs no histogram pass). 24 B/pt read + 8 B/pt written.
 #include <cfloat>
 
 #include "device.hpp"
@@ -97,7 +97,9 @@ struct EncodeBox {
     double c[3], r[3];
 };
 
-template <int ENC>
+// LSDH: the 8 LSD digit histograms and the partition passes' (bits 55..62, 47..54); else only the
+// latter (the partition sort needs no other, and two LDS atomics per point are cheaper than ten).
+template <int ENC, bool LSDH>
 __global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__ xyz, size_t n, EncodeBox box,
                                                       uint64_t *__restrict__ keys, uint32_t *__restrict__ hist) {
     __shared__ uint32_t sh[kEncHists * 256];
@@ -117,9 +119,12 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__
         uint32_t z = quantize_axis(xyz[3 * i + 2], box.c[2], box.r[2]);
         uint64_t k = (ENC == OCT_ENC_HILBERT3D) ? hilbert_encode_tab2(stab, stab2, x, y, z) : morton_encode(x, y, z);
         keys[i] = k;
+        if (LSDH) {
 #pragma unroll
-        for (int p = 0; p < kRadixPasses; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+            for (int p = 0; p < kRadixPasses; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+        }
         atomicAdd(&sh[kRadixPasses * 256 + ((k >> kMsdShift1) & 255u)], 1u);
+        atomicAdd(&sh[kRadixPasses * 256 + 256 + ((k >> kMsdShift2) & 255u)], 1u);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kEncHists * 256; i += blockDim.x)
@@ -174,10 +179,19 @@ hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], cons
     if (n) {
         int nb = (int)((n + kEncBlock * 16 - 1) / (kEncBlock * 16));
         nb = nb < 1 ? 1 : (nb > 2048 ? 2048 : nb);
-        if (encoder == OCT_ENC_HILBERT3D)
-            k_encode<OCT_ENC_HILBERT3D><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
-        else
-            k_encode<OCT_ENC_MORTON3D><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+        const bool lsdh = !sort_partition_planned(ctx);
+        if (encoder == OCT_ENC_HILBERT3D) {
+            if (lsdh)
+                k_encode<OCT_ENC_HILBERT3D, true><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+            else
+                k_encode<OCT_ENC_HILBERT3D, false><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+        } else {
+            if (lsdh)
+                k_encode<OCT_ENC_MORTON3D, true><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+            else
+                k_encode<OCT_ENC_MORTON3D, false><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+        }
+        ctx->hist_lsd = lsdh;
     }
     stage_end(ctx, ST_ENCODE);
     return hipGetLastError();

@@ -68,11 +68,11 @@ struct SortTile {
 };
 // Per-cloud bin bases of a segmented sort: bases[cloud * kSegBaseStride + pass * 256 + digit].
 constexpr uint32_t kSegBaseStride = kRadixPassCount * 256;
-// Partition sort of one cloud (keys below 2^63): device-wide stable passes on bits 55..62 and 47..54,
-// then block sorts of the 16-bit segments, splitting big ones from bits 39..46 down. The encode kernel
-// writes the histogram of bits 55..62 after the 8 LSD digit histograms.
+// Partition sort of one cloud (keys below 2^63): device-wide stable passes on bits 47..54 and 55..62,
+// then workgroup sorts of the 16-bit segments, splitting big ones from bits 39..46 down. The encode
+// kernel writes the histograms of bits 55..62 and 47..54 after the 8 LSD digit histograms.
 constexpr uint32_t kMsdShift1 = 55, kMsdShift2 = 47, kMsdShift3 = 39;
-constexpr int kEncHists = kRadixPassCount + 1;
+constexpr int kEncHists = kRadixPassCount + 2;
 
 // Bits of the context's device error word (oct_ctx::derr): kernels set them with atomicOr, the host
 // reads and clears the word at its next synchronisation and fails the call.
@@ -103,13 +103,15 @@ struct oct_ctx {
     double center[3] = {0, 0, 0}, radii[3] = {0, 0, 0};
     oct::DevBuf<uint64_t> keys[2];
     oct::DevBuf<uint32_t> vals[2];
-    oct::DevBuf<uint32_t> hist;      // 8 x 256 digit histograms, then bits 55..62
+    oct::DevBuf<uint32_t> hist;      // 8 x 256 digit histograms, then bits 55..62 and 47..54
     oct::DevBuf<uint32_t> lookback;  // per pass, per tile, per bin
     oct::DevBuf<uint32_t> misc;      // tile counters, flags, scan partials
-    oct::DevBuf<oct::SortTile> msd_tiles;  // partition sort: tiles of the second pass
-    oct::DevBuf<uint32_t> msd;             // partition sort: histograms, bases, segment jobs, counters
+    oct::DevBuf<uint32_t> msd;             // partition sort: bases, prefix counts, segment jobs, counters
     int sort_override = -1;                // OCT_TUNE_SORT
     bool p64_sorted = false;               // the last sort also wrote the sorted cloud (p64)
+    bool hist_lsd = false;                 // the encode kernel wrote the 8 LSD digit histograms too
+    oct::DevBuf<uint64_t> keys_p;          // partition sort: keys ordered by their top 16 bits
+    oct::DevBuf<uint32_t> vals_p;
     uint64_t *sorted_keys = nullptr; // points into keys[0|1]
     uint32_t *perm = nullptr;        // points into vals[0|1]
     bool has_keys = false, has_sorted = false;
@@ -268,6 +270,9 @@ hipError_t launch_sort_passes(hipStream_t s, uint64_t *keys0, uint64_t *keys1, u
 hipError_t launch_gather_records(hipStream_t s, const void *in, const uint32_t *perm, void *out, size_t n, size_t rb);
 // out[i] = (xyz[3 perm[i]], xyz[3 perm[i] + 1], xyz[3 perm[i] + 2], 0)
 hipError_t launch_gather_raw(hipStream_t s, const double *xyz, const uint32_t *perm, double4 *out, size_t n);
+// whether oct_sort will take the partition sort for the context's current cloud and tuning (the encode
+// kernel then writes only the partition histogram)
+bool sort_partition_planned(const oct_ctx *ctx);
 hipError_t exclusive_scan_u32_to_u64(oct_ctx *ctx, const uint32_t *in, uint64_t *out, size_t n);
 hipError_t exclusive_scan_u32(oct_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev);
 

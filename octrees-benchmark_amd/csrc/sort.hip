@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
                                                          uint32_t shift, const uint32_t *__restrict__ bin_base,
                                                          uint32_t *__restrict__ lookback, uint32_t *__restrict__ tile_counter,
                                                          uint32_t *__restrict__ err, const SortTile *__restrict__ tiles,
-                                                         uint32_t base_stride) {
+                                                         uint32_t base_stride, uint32_t *__restrict__ hist16) {
     __shared__ uint64_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_wave[kSortBlock / 64][256];
@@ -174,6 +174,14 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t *__restr
         uint32_t dst = s_global[d] + (j - s_tile_start[d]);
         kout[dst] = key;
         vout[dst] = s_vals[j];
+        if (hist16) {
+            // the keys' top 16 bits (47..62) per prefix: when this pass orders bits 55..62 after a pass
+            // on 47..54, the staged tile is ordered by them, so each run of one prefix adds its length
+            // (its first key subtracts its index, its last adds index + 1)
+            const uint32_t pre = (uint32_t)(key >> kMsdShift2);
+            if (j == 0 || (uint32_t)(s_keys[j - 1] >> kMsdShift2) != pre) atomicAdd(&hist16[pre], 0u - j);
+            if (j + 1 == tile_n || (uint32_t)(s_keys[j + 1] >> kMsdShift2) != pre) atomicAdd(&hist16[pre], j + 1u);
+        }
     }
 }
 
@@ -189,24 +197,29 @@ __global__ void k_iota_copy(const uint64_t *__restrict__ kin, uint64_t *__restri
 //
 // For more keys than one workgroup sorts in LDS, all below 2^63 (both 3-D encoders' keys), the same
 // stable sort runs as two device-wide passes and one workgroup pass instead of eight device-wide passes:
-//  1. the onesweep pass above on bits 55..62 (its histogram comes from the encode kernel);
-//  2. a histogram of bits 47..54 inside each of those bins (k_msd_hist), their bin offsets and the
-//     workgroup jobs (k_msd_jobs), then a segmented onesweep pass on bits 47..54 inside each bin: the
-//     keys are now ordered by their top 16 bits;
-//  3. k_seg_sort: one workgroup per job sorts its range completely. A job is one 16-bit segment larger
-//     than the workgroup's LDS capacity, or a run of consecutive smaller segments of one bin that fits.
-//     A range that fits is sorted in LDS by LSD passes over only the digits, and inside a digit only
-//     the bits, that vary in it, each ranked stably with wave ballots. A larger segment is first split
-//     stably by its next 8 bits (39..46, then 31..38, ...) in LDS-sized tiles with running bin offsets
-//     (one workgroup owns the segment, so no look-back), until every piece fits.
+//  1. the onesweep pass above on bits 47..54, then on bits 55..62 (both histograms come from the encode
+//     kernel): the keys are now ordered by their top 16 bits. The second pass stages each tile in that
+//     order, so it also counts the keys of every 16-bit prefix (two atomics per run of one prefix);
+//  2. k_msd_jobs turns those counts into workgroup jobs: one 16-bit segment larger than the
+//     workgroup's LDS capacity, or a run of consecutive smaller segments of one top-8-bit bin that fits;
+//  3. k_seg_sort: one workgroup per job sorts its range completely. A range that fits is sorted in LDS
+//     by LSD passes over only the digits, and inside a digit only the bits, that vary in it, each
+//     ranked stably with wave ballots. A larger segment is first split stably by its next 8 bits
+//     (39..46, then 31..38, ...) in LDS-sized tiles with running bin offsets (one workgroup owns the
+//     segment, so no look-back), until every piece fits. The workgroup then reads the points of its
+//     sorted range through the permutation and writes the sorted cloud (the gather), so the random
+//     reads overlap other workgroups' LDS work.
 // Every step keeps equal keys in input order, so the permutation is the stable sort by key, as with
-// the 8 LSD passes. Traffic per key: 20 B (pass 1) + 8 B (histogram) + 24 B (pass 2) + 24 B (LDS
-// sort), plus 32 B for keys in segments that are split, against 188 B for eight passes.
+// the 8 LSD passes. Traffic per key: 20 B + 24 B (passes) + 24 B (LDS sort) + 56 B (gather), plus
+// 32 B for keys in segments that are split, against 188 B + 60 B for eight passes and a gather.
 
 struct SegJob {
     uint32_t start, len;
 };
-constexpr int kSegStack = 64;  // pieces awaiting a split: at most 7 per level x 6 levels for 2^15 keys
+constexpr uint32_t kSegMaxLen = 1u << 15;  // longest 16-bit segment the workgroup sort takes
+// pieces awaiting a split: a split of up to 2^15 keys pushes at most 15 pieces above 2048 keys and
+// pops one, over at most 6 levels (bits 39..46 down to 0..7)
+constexpr int kSegStack = 96;
 
 template <int W, int IPT>
 struct SegShared {
@@ -216,15 +229,14 @@ struct SegShared {
     uint32_t cnt[W][256];  // per wave: digit counts, then the wave's first position inside each digit
     uint32_t start[256];   // first position of each digit in the ranked range
     uint32_t tot[256];     // keys per digit in the ranked range
-    uint32_t hist[256];    // split: keys per digit of the whole piece
-    uint32_t off[256];     // split: next output position of each digit
+    uint32_t hist[256];    // split: keys per digit of the whole piece, then the start of each run to sort
+    uint32_t off[256];     // split: next output position of each digit, then the length of each run
     uint32_t wsum[W];
     uint32_t mask_lo, mask_hi;  // OR of (key ^ first key) over a range
     uint32_t or_d, and_d;       // split: OR / AND of the digit values present
     uint32_t sp, nchunks, kind;
     SegJob job;
     uint4 stack[kSegStack];  // start, len, shift + 1 (0: no bits left, keys equal), buffer (0: a, 1: b)
-    SegJob chunks[256];      // split: runs of small pieces, each one LDS sort
 };
 
 // Greedy packing, by one wave, of 256 consecutive digit ranges (digit d = 4 * lane + slot: count
@@ -504,17 +516,16 @@ __device__ void seg_split_sort(SegShared<W, IPT> &S, uint64_t *ka, uint32_t *va,
                         atomicOr(err, 2u);
                     }
                 },
-                [&](uint32_t a, uint32_t l) {
-                    if (lane == 0) S.chunks[nc] = SegJob{a, l};
+                [&](uint32_t a, uint32_t l) {  // wave 0 has read hist / off already
+                    if (lane == 0) S.hist[nc] = a, S.off[nc] = l;
                     nc++;
                 });
             if (lane == 0) S.nchunks = nc, S.sp = top;
         }
         __syncthreads();
         const uint32_t nc = S.nchunks;
-        for (uint32_t ci = 0; ci < nc; ci++) {
-            const SegJob cj = S.chunks[ci];
-            seg_local_sort(S, kt, vt, kb, vb, cj.start, cj.len, xyz, p64);
+        for (uint32_t ci = 0; ci < nc; ci++) {  // the LDS sorts leave hist / off alone
+            seg_local_sort(S, kt, vt, kb, vb, S.hist[ci], S.off[ci], xyz, p64);
             __syncthreads();
         }
     }
@@ -524,13 +535,14 @@ __device__ void seg_split_sort(SegShared<W, IPT> &S, uint64_t *ka, uint32_t *va,
 // pass's output (ka, va) in, the sorted keys and permutation (kb, vb) out, and with xyz (n x 3 input
 // points) the sorted cloud p64 (the gather, overlapped with the LDS sorts of other workgroups).
 template <int W, int IPT>
-__global__ __launch_bounds__(W * 64, 4) void k_seg_sort(uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb,
+__global__ __attribute__((amdgpu_flat_work_group_size(W * 64, W * 64), amdgpu_waves_per_eu(IPT <= 4 ? 8 : 4))) void k_seg_sort(uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb,
                                                      const SegJob *__restrict__ big, const SegJob *__restrict__ small,
                                                      const uint32_t *__restrict__ njobs, uint32_t *__restrict__ next,
                                                      uint32_t *__restrict__ err, const double *__restrict__ xyz,
                                                      double4 *__restrict__ p64) {
     __shared__ SegShared<W, IPT> S;
     const uint32_t tid = threadIdx.x;
+    if (njobs[2] > kSegMaxLen) return;  // a segment too long for one workgroup: the LSD passes run instead
     const uint32_t nbig = njobs[0], nsmall = njobs[1];
     for (;;) {
         if (tid == 0) {
@@ -558,35 +570,13 @@ __global__ __launch_bounds__(W * 64, 4) void k_seg_sort(uint64_t *ka, uint32_t *
     }
 }
 
-// Histogram of (key >> shift) & 255 per segment of a tile table: hist[segment * 256 + digit].
-__global__ __launch_bounds__(256) void k_msd_hist(const uint64_t *__restrict__ keys, const SortTile *__restrict__ tiles,
-                                                  uint32_t shift, uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[256];
-    const uint32_t tid = threadIdx.x;
-    h[tid] = 0u;
-    __syncthreads();
-    const SortTile T = tiles[blockIdx.x];
-    constexpr int kPer = kSortTile / 256;
-    uint32_t dg[kPer];
-#pragma unroll
-    for (int r = 0; r < kPer; r++) {
-        const uint32_t i = (uint32_t)r * 256 + tid;
-        dg[r] = i < T.len ? (uint32_t)(keys[T.start + i] >> shift) & 255u : 256u;
-    }
-#pragma unroll
-    for (int r = 0; r < kPer; r++)
-        if (dg[r] < 256u) atomicAdd(&h[dg[r]], 1u);
-    __syncthreads();
-    if (h[tid]) atomicAdd(&hist[(size_t)T.cloud * 256 + tid], h[tid]);
-}
-
-// One workgroup per first-level bin: the start of each of its second-level segments (the segmented
-// pass's bin bases, bases2[bin * 256 + digit]) and its jobs - a segment above cap alone (big), runs of
+// One workgroup per top-8-bit bin: the start of each of its 16-bit segments (hist2[bin * 256 + digit]
+// keys each; the bin starts at base1[bin]) and its jobs - a segment above cap alone (big), runs of
 // consecutive smaller segments up to cap together (small). njobs: [0] big, [1] small, [2] longest
 // segment.
 __global__ __launch_bounds__(256) void k_msd_jobs(const uint32_t *__restrict__ hist2, const uint32_t *__restrict__ base1,
-                                                  uint32_t cap, uint32_t *__restrict__ bases2, SegJob *__restrict__ big,
-                                                  SegJob *__restrict__ small, uint32_t *__restrict__ njobs) {
+                                                  uint32_t cap, SegJob *__restrict__ big, SegJob *__restrict__ small,
+                                                  uint32_t *__restrict__ njobs) {
     __shared__ uint32_t s_c[256], s_st[256], s_wsum[4], s_nb, s_ns, s_ob, s_os;
     __shared__ SegJob s_big[256], s_small[256];
     const uint32_t bin = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -597,7 +587,6 @@ __global__ __launch_bounds__(256) void k_msd_jobs(const uint32_t *__restrict__ h
     uint32_t pre = 0;
     for (uint32_t ww = 0; ww < w; ww++) pre += s_wsum[ww];
     const uint32_t st = base1[bin] + pre + incl - c;
-    bases2[bin * 256 + tid] = st;
     s_c[tid] = c, s_st[tid] = st;
     __syncthreads();
     if (w == 0) {
@@ -629,10 +618,43 @@ __global__ __launch_bounds__(256) void k_msd_jobs(const uint32_t *__restrict__ h
     if (tid < s_ns) small[s_os + tid] = s_small[tid];
 }
 
+// Everything the partition passes start from, in one launch (no host copies or fills in between):
+// zero look-back words, prefix counts, job and tile counters and the error flag; the bin starts of
+// both passes.
+struct PartInit {
+    uint32_t base[2][256];  // [0]: bits 47..54, [1]: bits 55..62
+};
+__global__ __launch_bounds__(256) void k_part_init(PartInit P, size_t nlook, uint32_t *__restrict__ lookback,
+                                                   uint32_t *__restrict__ bases, uint32_t *__restrict__ hist2,
+                                                   uint32_t *__restrict__ njobs, uint32_t *__restrict__ counters) {
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+    for (size_t i = g; i < nlook; i += stride) lookback[i] = 0u;
+    for (size_t i = g; i < 65536; i += stride) hist2[i] = 0u;
+    if (g < 256) bases[g] = P.base[0][g], bases[256 + g] = P.base[1][g];
+    if (g < 4) njobs[g] = 0u;
+    if (g < 32) counters[g] = 0u;
+}
+
+// The 8 LSD digit histograms of n keys (hist must be zero), for an LSD sort after an encode that
+// skipped them.
+__global__ __launch_bounds__(256) void k_digit_hist(const uint64_t *__restrict__ keys, size_t n, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t sh[kRadixPassCount * 256];
+    for (uint32_t i = threadIdx.x; i < kRadixPassCount * 256; i += 256) sh[i] = 0u;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const uint64_t k = keys[i];
+#pragma unroll
+        for (int p = 0; p < kRadixPassCount; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kRadixPassCount * 256; i += 256)
+        if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
 namespace {
 
-// LDS sort shapes: OCT_TUNE_SORT 1 = 8 waves x 8 keys (4096), 2 = 16 waves x 8 keys (8192)
-constexpr uint32_t kSegMaxLen = 1u << 15;  // longest 16-bit segment the workgroup sort takes
+// LDS sort shapes: OCT_TUNE_SORT 1 = 8 waves x 4 keys (2048; four workgroups per CU, so more gather
+// reads in flight), 2 = 8 waves x 8 keys (4096)
 constexpr size_t kSegJobsMax = 256 * 256;  // jobs per kind: at most one per 16-bit segment
 
 // resident workgroups of k_seg_sort<W, IPT> over the device
@@ -648,73 +670,104 @@ unsigned seg_grid() {
     return cached;
 }
 
-// The partition sort (above) of the context's n encoded keys, keys[0] in, keys[1] / vals[1] out.
-// h9: the encode kernel's histograms. Returns with *done false, and keys[0] untouched, when a 16-bit
-// segment is longer than kSegMaxLen (one workgroup would sort it alone): the LSD passes run instead.
+// The partition sort (above) of the context's n encoded keys, keys[0] in (left untouched, for the LSD
+// passes should a 16-bit segment turn out longer than kSegMaxLen: k_seg_sort then does nothing and
+// njobs[2] tells the host), keys[1] / vals[1] and the sorted cloud out. h47, h55: the histograms of
+// bits 47..54 and 55..62. Launches only; returns where the njobs words are.
 template <int W, int IPT>
-hipError_t sort_partition(oct_ctx *ctx, const uint32_t *h9, uint32_t *counters, uint32_t *errflag, bool *done) {
+hipError_t sort_partition(oct_ctx *ctx, const uint32_t *h47, const uint32_t *h55, uint32_t *counters, uint32_t *errflag,
+                          uint32_t **njobs_dev) {
     constexpr uint32_t cap = SegShared<W, IPT>::kCap;
     const size_t n = ctx->n;
-    const uint32_t *h1 = h9 + kRadixPassCount * 256;
-    uint32_t base1[256];
-    std::vector<SortTile> tl;
-    tl.reserve(n / kSortTile + 257);
-    uint32_t run = 0;
-    for (uint32_t d = 0; d < 256; d++) {
-        base1[d] = run;
-        const uint32_t first = (uint32_t)tl.size();
-        for (uint32_t p = 0; p < h1[d]; p += kSortTile)
-            tl.push_back(SortTile{run + p, std::min<uint32_t>(kSortTile, h1[d] - p), d, first});
-        run += h1[d];
-    }
-    const size_t tiles1 = (n + kSortTile - 1) / kSortTile, tiles2 = tl.size();
-    // msd layout (u32 words): base1 [256], hist2 [65536], bases2 [65536], njobs [4], big and small jobs
-    const size_t o_base1 = 0, o_hist2 = 256, o_bases2 = o_hist2 + 65536, o_njobs = o_bases2 + 65536, o_big = o_njobs + 4,
-                 o_small = o_big + 2 * kSegJobsMax, words = o_small + 2 * kSegJobsMax;
+    PartInit P;
+    uint32_t r0 = 0, r1 = 0;
+    for (uint32_t d = 0; d < 256; d++) P.base[0][d] = r0, P.base[1][d] = r1, r0 += h47[d], r1 += h55[d];
+    const size_t tiles = (n + kSortTile - 1) / kSortTile;
+    // msd layout (u32 words): bases [2 x 256], hist2 [65536], njobs [4], big and small jobs
+    const size_t o_bases = 0, o_hist2 = 512, o_njobs = o_hist2 + 65536, o_big = o_njobs + 4, o_small = o_big + 2 * kSegJobsMax,
+                 words = o_small + 2 * kSegJobsMax;
     hipError_t e;
     if ((e = ctx->msd.reserve(words)) != hipSuccess) return e;
+    if ((e = ctx->lookback.reserve(2 * tiles * 256 + 1)) != hipSuccess) return e;
     if ((e = ctx->p64.reserve(n)) != hipSuccess) return e;
-    if ((e = ctx->msd_tiles.reserve(tiles2)) != hipSuccess) return e;
-    if ((e = ctx->lookback.reserve((tiles1 + tiles2) * 256 + 1)) != hipSuccess) return e;
+    if ((e = ctx->keys_p.reserve(n)) != hipSuccess) return e;
+    if ((e = ctx->vals_p.reserve(n)) != hipSuccess) return e;
     uint32_t *M = ctx->msd.p;
     SegJob *big = reinterpret_cast<SegJob *>(M + o_big), *small = reinterpret_cast<SegJob *>(M + o_small);
     hipStream_t s = ctx->stream;
-    if ((e = hipMemcpyAsync(M + o_base1, base1, sizeof(base1), hipMemcpyHostToDevice, s)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(ctx->msd_tiles.p, tl.data(), tiles2 * sizeof(SortTile), hipMemcpyHostToDevice, s)) != hipSuccess)
-        return e;
-    if ((e = hipMemsetAsync(M + o_hist2, 0, 65536 * sizeof(uint32_t), s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(M + o_njobs, 0, 4 * sizeof(uint32_t), s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(ctx->lookback.p, 0, (tiles1 + tiles2) * 256 * sizeof(uint32_t), s)) != hipSuccess) return e;
-    // pass 1 on bits 55..62: keys[0] (values: the positions) -> keys[1] / vals[1]
-    k_onesweep<false><<<(unsigned)tiles1, kSortBlock, 0, s>>>(ctx->keys[0].p, nullptr, ctx->keys[1].p, ctx->vals[1].p, n,
-                                                              kMsdShift1, M + o_base1, ctx->lookback.p, counters + 0,
-                                                              errflag, nullptr, 0u);
-    k_msd_hist<<<(unsigned)tiles2, 256, 0, s>>>(ctx->keys[1].p, ctx->msd_tiles.p, kMsdShift2, M + o_hist2);
-    k_msd_jobs<<<256, 256, 0, s>>>(M + o_hist2, M + o_base1, cap, M + o_bases2, big, small, M + o_njobs);
+    const size_t nlook = 2 * tiles * 256;
+    const unsigned ig = (unsigned)std::min<size_t>(2048, std::max<size_t>((nlook + 255) / 256, 256));
+    k_part_init<<<ig, 256, 0, s>>>(P, nlook, ctx->lookback.p, M + o_bases, M + o_hist2, M + o_njobs, counters);
+    // bits 47..54: keys[0] (values: the positions) -> keys[1] / vals[1]
+    k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, s>>>(ctx->keys[0].p, nullptr, ctx->keys[1].p, ctx->vals[1].p, n,
+                                                             kMsdShift2, M + o_bases, ctx->lookback.p, counters + 0, errflag,
+                                                             nullptr, 0u, nullptr);
+    // bits 55..62: keys[1] / vals[1] -> keys_p / vals_p, counting the keys of every 16-bit prefix
+    k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, s>>>(ctx->keys[1].p, ctx->vals[1].p, ctx->keys_p.p, ctx->vals_p.p, n,
+                                                             kMsdShift1, M + o_bases + 256, ctx->lookback.p + tiles * 256,
+                                                             counters + 1, errflag, nullptr, 0u, M + o_hist2);
+    k_msd_jobs<<<256, 256, 0, s>>>(M + o_hist2, M + o_bases + 256, cap, big, small, M + o_njobs);
+    // every 16-bit segment sorted by one workgroup: keys_p / vals_p -> keys[1] / vals[1], sorted cloud
+    k_seg_sort<W, IPT><<<seg_grid<W, IPT>(), W * 64, 0, s>>>(ctx->keys_p.p, ctx->vals_p.p, ctx->keys[1].p, ctx->vals[1].p,
+                                                             big, small, M + o_njobs, counters + 2, errflag, ctx->xyz.p,
+                                                             ctx->p64.p);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    uint32_t nj[4] = {0, 0, 0, 0};
-    if ((e = hipMemcpyAsync(nj, M + o_njobs, sizeof(nj), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-    if (nj[2] > kSegMaxLen) {
-        *done = false;
-        return hipSuccess;
-    }
-    // pass 2 on bits 47..54 inside each first-level bin: keys[1] / vals[1] -> keys[0] / vals[0]
-    k_onesweep<true><<<(unsigned)tiles2, kSortBlock, 0, s>>>(ctx->keys[1].p, ctx->vals[1].p, ctx->keys[0].p, ctx->vals[0].p,
-                                                             n, kMsdShift2, M + o_bases2, ctx->lookback.p + tiles1 * 256,
-                                                             counters + 1, errflag, ctx->msd_tiles.p, 256u);
-    const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>(seg_grid<W, IPT>(), (size_t)nj[0] + nj[1]));
-    k_seg_sort<W, IPT><<<grid, W * 64, 0, s>>>(ctx->keys[0].p, ctx->vals[0].p, ctx->keys[1].p, ctx->vals[1].p, big, small,
-                                               M + o_njobs, counters + 2, errflag, ctx->xyz.p, ctx->p64.p);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    ctx->p64_sorted = true;
-    ctx->sorted_keys = ctx->keys[1].p;
-    ctx->perm = ctx->vals[1].p;
-    *done = true;
+    *njobs_dev = M + o_njobs;
     return hipSuccess;
 }
 
+// LSD passes (the 8-bit digits that vary) over keys[0]; h: their 8 histograms.
+hipError_t sort_lsd(oct_ctx *ctx, const uint32_t *h, uint32_t *counters, uint32_t *errflag, uint32_t *bases) {
+    const size_t n = ctx->n;
+    int passes[8], np = 0;
+    uint32_t base[8 * 256];
+    for (int p = 0; p < 8; p++) {
+        uint32_t run = 0;
+        bool trivial = false;
+        for (int d = 0; d < 256; d++) {
+            base[p * 256 + d] = run;
+            run += h[p * 256 + d];
+            if (h[p * 256 + d] == n) trivial = true;
+        }
+        if (!trivial && n > 1) passes[np++] = p;
+    }
+    const size_t tiles = (n + kSortTile - 1) / kSortTile;
+    hipError_t e;
+    if ((e = ctx->lookback.reserve((size_t)(np ? np : 1) * tiles * 256 + 1)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(bases, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(counters, 0, 16 * sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
+    if (np && (e = hipMemsetAsync(ctx->lookback.p, 0, (size_t)np * tiles * 256 * sizeof(uint32_t), ctx->stream)) != hipSuccess)
+        return e;
+    int src = 0;
+    for (int i = 0; i < np; i++) {
+        int p = passes[i];
+        const uint32_t *vin = (i == 0) ? nullptr : ctx->vals[src].p;
+        k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, ctx->stream>>>(
+            ctx->keys[src].p, vin, ctx->keys[src ^ 1].p, ctx->vals[src ^ 1].p, n, 8u * p, bases + p * 256,
+            ctx->lookback.p + (size_t)i * tiles * 256, counters + i, errflag, nullptr, 0u, nullptr);
+        src ^= 1;
+    }
+    if (np == 0) {
+        int nb = (int)((n + 255) / 256);
+        nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
+        k_iota_copy<<<nb, 256, 0, ctx->stream>>>(ctx->keys[0].p, ctx->keys[1].p, ctx->vals[1].p, n);
+        src = 1;
+    }
+    ctx->sorted_keys = ctx->keys[src].p;
+    ctx->perm = ctx->vals[src].p;
+    ctx->p64_sorted = false;
+    return hipGetLastError();
+}
+
+// OCT_TUNE_SORT: 0 = LSD passes, 1 / 2 = partition sort with 2048- / 4096-key LDS sorts (1 by default)
+int sort_mode(const oct_ctx *ctx) { return ctx->sort_override < 0 ? 1 : ctx->sort_override; }
+
 }  // namespace
+
+bool sort_partition_planned(const oct_ctx *ctx) {
+    const int mode = sort_mode(ctx);
+    return mode > 0 && ctx->n > (mode == 2 ? 4096u : 2048u);
+}
 
 hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     const size_t n = ctx->n;
@@ -726,71 +779,68 @@ hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     if ((e = ctx->keys[1].reserve(n)) != hipSuccess) return e;
     if ((e = ctx->vals[0].reserve(n)) != hipSuccess) return e;
     if ((e = ctx->vals[1].reserve(n)) != hipSuccess) return e;
-    // histograms of all passes (from the encode kernel)
-    uint32_t h[kEncHists * 256];
-    if ((e = hipMemcpyAsync(h, ctx->hist.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
-    int passes[8], np = 0;
-    uint32_t base[8 * 256];
-    bool below63 = true;  // every key below 2^63
-    for (int p = 0; p < 8; p++) {
-        uint32_t run = 0;
-        bool trivial = false;
-        for (int d = 0; d < 256; d++) {
-            base[p * 256 + d] = run;
-            run += h[p * 256 + d];
-            if (h[p * 256 + d] == n) trivial = true;
-            if (p == 7 && d >= 128 && h[p * 256 + d]) below63 = false;
-        }
-        if (!trivial && n > 1) passes[np++] = p;
-    }
-    const size_t tiles = (n + kSortTile - 1) / kSortTile;
-    if ((e = ctx->lookback.reserve((size_t)(np ? np : 1) * tiles * 256 + 1)) != hipSuccess) return e;
     // misc layout: [0..16) tile counters, [16] error flag, [32..32+2048) bin bases
     if ((e = ctx->misc.reserve(12 * 1024 + 64)) != hipSuccess) return e;
     uint32_t *counters = ctx->misc.p, *errflag = ctx->misc.p + 16, *bases = ctx->misc.p + 32;
+    // histograms from the encode kernel: the partition passes' always, the LSD digits' if hist_lsd
+    uint32_t h[kEncHists * 256];
+    if ((e = hipMemcpyAsync(h, ctx->hist.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
     ctx->p64_sorted = false;
+    // keys come from the 3-D encoders (63 bits), so bits 47..62 are the top 16 the partition sort uses
+    const int mode = sort_mode(ctx);
+    const bool partition = sort_partition_planned(ctx);
     stage_begin(ctx, ST_SORT);
-    if ((e = hipMemsetAsync(counters, 0, 32 * sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
-    // partition sort by default for more keys than one LDS sort holds (OCT_TUNE_SORT: 0 = LSD passes,
-    // 1 / 2 = partition sort with 4096- / 8192-key LDS sorts)
-    const int mode = ctx->sort_override < 0 ? 1 : ctx->sort_override;
-    bool done = false;
-    if (mode > 0 && below63 && n > (mode == 2 ? 8192u : 4096u)) {
-        e = mode == 2 ? sort_partition<16, 8>(ctx, h, counters, errflag, &done)
-                      : sort_partition<8, 8>(ctx, h, counters, errflag, &done);
+    uint32_t *njobs_dev = nullptr;
+    if (partition) {
+        const uint32_t *h55 = h + kRadixPassCount * 256, *h47 = h55 + 256;
+        e = mode == 2 ? sort_partition<8, 8>(ctx, h47, h55, counters, errflag, &njobs_dev)
+                      : sort_partition<8, 4>(ctx, h47, h55, counters, errflag, &njobs_dev);
         if (e != hipSuccess) return e;
-    }
-    if (!done) {
-        if ((e = hipMemcpyAsync(bases, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return e;
+        ctx->p64_sorted = true;
+        ctx->sorted_keys = ctx->keys[1].p;
+        ctx->perm = ctx->vals[1].p;
+    } else {
         if ((e = hipMemsetAsync(counters, 0, 32 * sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
-        if (np && (e = hipMemsetAsync(ctx->lookback.p, 0, (size_t)np * tiles * 256 * sizeof(uint32_t), ctx->stream)) != hipSuccess)
-            return e;
-        int src = 0;
-        for (int i = 0; i < np; i++) {
-            int p = passes[i];
-            const uint32_t *vin = (i == 0) ? nullptr : ctx->vals[src].p;
-            k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, ctx->stream>>>(
-                ctx->keys[src].p, vin, ctx->keys[src ^ 1].p, ctx->vals[src ^ 1].p, n, 8u * p, bases + p * 256,
-                ctx->lookback.p + (size_t)i * tiles * 256, counters + i, errflag, nullptr, 0u);
-            src ^= 1;
+        if (!ctx->hist_lsd) {  // the encode kernel wrote only the partition histogram
+            if ((e = hipMemsetAsync(ctx->hist.p, 0, kRadixPassCount * 256 * sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
+            const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 1024));
+            k_digit_hist<<<nb, 256, 0, ctx->stream>>>(ctx->keys[0].p, n, ctx->hist.p);
+            if ((e = hipMemcpyAsync(h, ctx->hist.p, kRadixPassCount * 256 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                    ctx->stream)) != hipSuccess)
+                return e;
+            if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+            ctx->hist_lsd = true;
         }
-        if (np == 0) {
-            int nb = (int)((n + 255) / 256);
-            nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
-            k_iota_copy<<<nb, 256, 0, ctx->stream>>>(ctx->keys[0].p, ctx->keys[1].p, ctx->vals[1].p, n);
-            src = 1;
-        }
-        ctx->sorted_keys = ctx->keys[src].p;
-        ctx->perm = ctx->vals[src].p;
+        if ((e = sort_lsd(ctx, h, counters, errflag, bases)) != hipSuccess) return e;
     }
     stage_end(ctx, ST_SORT);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    uint32_t flag = 0;
-    if ((e = hipMemcpyAsync(&flag, errflag, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+    uint32_t fb[2] = {0, 0};  // error flag, longest 16-bit segment
+    if ((e = hipMemcpyAsync(&fb[0], errflag, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+    if (njobs_dev && (e = hipMemcpyAsync(&fb[1], njobs_dev + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess)
+        return e;
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
-    if (flag) {
-        err = (flag & 2u) ? "sort: a segment split overflowed its stack" : "sort: look-back wait exceeded its bound";
+    if (!fb[0] && partition && fb[1] > kSegMaxLen) {
+        // a segment too long for one workgroup: the LSD passes over the untouched encoded keys, timed
+        // as part of the same stage
+        if (!ctx->hist_lsd) {
+            if ((e = hipMemsetAsync(ctx->hist.p, 0, kRadixPassCount * 256 * sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
+            const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 1024));
+            k_digit_hist<<<nb, 256, 0, ctx->stream>>>(ctx->keys[0].p, n, ctx->hist.p);
+            if ((e = hipMemcpyAsync(h, ctx->hist.p, kRadixPassCount * 256 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                    ctx->stream)) != hipSuccess)
+                return e;
+            if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+            ctx->hist_lsd = true;
+        }
+        if ((e = sort_lsd(ctx, h, counters, errflag, bases)) != hipSuccess) return e;
+        stage_end(ctx, ST_SORT);
+        if ((e = hipMemcpyAsync(&fb[0], errflag, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+    }
+    if (fb[0]) {
+        err = (fb[0] & 2u) ? "sort: a segment split overflowed its stack" : "sort: look-back wait exceeded its bound";
         return hipErrorUnknown;
     }
     return hipSuccess;
@@ -806,7 +856,7 @@ hipError_t launch_sort_segmented(hipStream_t s, uint64_t *keys0, uint64_t *keys1
         const int src = p & 1;
         k_onesweep<true><<<(unsigned)ntiles, kSortBlock, 0, s>>>(kb[src], p == 0 ? nullptr : vb[src], kb[src ^ 1], vb[src ^ 1], n,
                                                                 8u * p, bases + p * 256, lookback + (size_t)p * ntiles * 256,
-                                                                counters + p, err, tiles, kSegBaseStride);
+                                                                counters + p, err, tiles, kSegBaseStride, nullptr);
     }
     return hipGetLastError();
 }
@@ -821,7 +871,7 @@ hipError_t launch_sort_passes(hipStream_t s, uint64_t *keys0, uint64_t *keys1, u
         const int src = p & 1;
         k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, s>>>(kb[src], p == 0 ? nullptr : vb[src], kb[src ^ 1], vb[src ^ 1], n,
                                                                  8u * p, bases + p * 256, lookback + (size_t)p * tiles * 256,
-                                                                 counters + p, err, nullptr, 0u);
+                                                                 counters + p, err, nullptr, 0u, nullptr);
     }
     return hipGetLastError();
 }

@@ -692,8 +692,8 @@ class Octree:
         self._chk(self.L.oct_set_tuning(self.h, 4, -1 if on is None else (1 if on else 0)), "oct_set_tuning")
 
     def set_sort(self, mode):
-        """How oct_sort sorts: 1 partition + 4096-key LDS sorts (default, also None), 2 the same with
-        8192-key LDS sorts, 0 the device-wide LSD pass per 8-bit digit."""
+        """How oct_sort sorts: 1 partition + 2048-key LDS sorts (default, also None), 2 the same with
+        4096-key LDS sorts, 0 the device-wide LSD pass per 8-bit digit."""
         self._chk(self.L.oct_set_tuning(self.h, 5, -1 if mode is None else int(mode)), "oct_set_tuning")
 
     def set_knn_lanes(self, lanes):

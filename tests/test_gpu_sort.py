@@ -1,7 +1,7 @@
 """GPU sort parity: every sort path of oct_sort against the oracle's stable sort, on clouds built to
 reach each branch of the partition sort.
 
-The partition sort (OCT_TUNE_SORT 1 and 2, the default for more than 4096 / 8192 keys) orders the keys
+The partition sort (OCT_TUNE_SORT 1 and 2, the default for more than 2048 / 4096 keys) orders the keys
 by their top 16 bits in two device-wide passes, then sorts each 16-bit segment inside one workgroup,
 splitting segments above the workgroup's LDS capacity by the next 8 bits until the pieces fit; a
 segment above 32768 keys sends the whole sort to the LSD passes (OCT_TUNE_SORT 0). Bar: permutation
@@ -70,7 +70,7 @@ CLOUDS = {
     # every key equal (zero-extent box)
     "same_30k_morton": lambda: (np.tile([[3.25, -7.5, 1.0]], (30_000, 1)), A.ENC_MORTON),
 }
-SIZES = [1, 2, 100, 4095, 4096, 4097, 8192, 8193, 12289, 70_001]
+SIZES = [1, 2, 100, 2047, 2048, 2049, 4095, 4096, 4097, 8193, 12289, 70_001]
 
 
 @pytest.fixture(scope="module")
