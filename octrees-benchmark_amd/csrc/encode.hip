@@ -113,11 +113,11 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__
         hilbert_tab2_fill(stab2, stab, threadIdx.x, blockDim.x);
         __syncthreads();
     }
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        uint32_t x = quantize_axis(xyz[3 * i + 0], box.c[0], box.r[0]);
-        uint32_t y = quantize_axis(xyz[3 * i + 1], box.c[1], box.r[1]);
-        uint32_t z = quantize_axis(xyz[3 * i + 2], box.c[2], box.r[2]);
-        uint64_t k = (ENC == OCT_ENC_HILBERT3D) ? hilbert_encode_tab2(stab, stab2, x, y, z) : morton_encode(x, y, z);
+    auto point = [&](size_t i, double px, double py, double pz) {
+        const uint32_t x = quantize_axis(px, box.c[0], box.r[0]);
+        const uint32_t y = quantize_axis(py, box.c[1], box.r[1]);
+        const uint32_t z = quantize_axis(pz, box.c[2], box.r[2]);
+        const uint64_t k = (ENC == OCT_ENC_HILBERT3D) ? hilbert_encode_tab2(stab, stab2, x, y, z) : morton_encode(x, y, z);
         keys[i] = k;
         if (LSDH) {
 #pragma unroll
@@ -125,7 +125,19 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__
         }
         atomicAdd(&sh[kRadixPasses * 256 + ((k >> kMsdShift1) & 255u)], 1u);
         atomicAdd(&sh[kRadixPasses * 256 + 256 + ((k >> kMsdShift2) & 255u)], 1u);
+    };
+    // two points per thread and iteration: both points' loads are in flight together and their
+    // table walks interleave
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + stride < n; i += 2 * stride) {
+        const size_t i2 = i + stride;
+        const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2];
+        const double x1 = xyz[3 * i2], y1 = xyz[3 * i2 + 1], z1 = xyz[3 * i2 + 2];
+        point(i, x0, y0, z0);
+        point(i2, x1, y1, z1);
     }
+    if (i < n) point(i, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
     __syncthreads();
     for (int i = threadIdx.x; i < kEncHists * 256; i += blockDim.x)
         if (sh[i]) atomicAdd(&hist[i], sh[i]);
