@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kBboxBlock) void k_bbox(const double *__restrict__ 
 
 __global__ __launch_bounds__(kBboxBlock) void k_bbox_final(const double *__restrict__ partial, int nb,
                                                            double *__restrict__ out) {
-    __shared__ double s[kBboxBlock][6];
+    __shared__ double s[kBboxBlock / 64][6];
     double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int b = threadIdx.x; b < nb; b += blockDim.x)
         for (int a = 0; a < 3; a++) {
@@ -56,13 +56,23 @@ __global__ __launch_bounds__(kBboxBlock) void k_bbox_final(const double *__restr
             v[a] = (lo < v[a]) ? lo : v[a];
             v[3 + a] = (v[3 + a] < hi) ? hi : v[3 + a];
         }
-    for (int k = 0; k < 6; k++) s[threadIdx.x][k] = v[k];
+    // the partials hold no NaN (a NaN never replaces a running value), so the order of the
+    // combination does not change the result: lanes by exchange, then the block's waves
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; a++) {
+            const double lo = __shfl_xor(v[a], off), hi = __shfl_xor(v[3 + a], off);
+            v[a] = (lo < v[a]) ? lo : v[a];
+            v[3 + a] = (v[3 + a] < hi) ? hi : v[3 + a];
+        }
+    const int w = threadIdx.x / 64;
+    if (lane_id() == 0)
+        for (int k = 0; k < 6; k++) s[w][k] = v[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int t = 1; t < (int)blockDim.x; t++)
+        for (int j = 1; j < kBboxBlock / 64; j++)
             for (int a = 0; a < 3; a++) {
-                v[a] = (s[t][a] < v[a]) ? s[t][a] : v[a];
-                v[3 + a] = (v[3 + a] < s[t][3 + a]) ? s[t][3 + a] : v[3 + a];
+                v[a] = (s[j][a] < v[a]) ? s[j][a] : v[a];
+                v[3 + a] = (v[3 + a] < s[j][3 + a]) ? s[j][3 + a] : v[3 + a];
             }
         for (int k = 0; k < 6; k++) out[k] = v[k];
     }
