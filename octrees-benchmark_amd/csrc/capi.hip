@@ -511,13 +511,34 @@ int oct_gather_records(oct_ctx *ctx, const void *in, void *out, size_t record_by
     return OCT_OK;
 }
 
+// oct_bbox + oct_encode + oct_sort with nothing read back in between: the box stays on the device
+// (k_bbox_final computes the centre and radii with the host's expressions), the encode reads it there,
+// and the host reads it back after the sort.
 int oct_sort_points(oct_ctx *ctx, int encoder, double center_out[3], double radii_out[3], uint32_t *perm_out,
                     uint64_t *keys_out) {
+    if (!ctx) return OCT_ERR_ARG;
+    if (!valid_encoder(encoder)) return fail(ctx, OCT_ERR_UNSUPPORTED, "only MortonEncoder3D (1) and HilbertEncoder3D (5) build octrees");
+    if (!ctx->has_points) return fail(ctx, OCT_ERR_STATE, "oct_sort_points before oct_set_points");
     double c[3], r[3];
-    int s = oct_bbox(ctx, c, r);
-    if (s) return s;
-    if ((s = oct_encode(ctx, encoder, c, r, nullptr))) return s;
-    if ((s = oct_sort(ctx, perm_out, keys_out))) return s;
+    int s;
+    if (ctx->n == 0) {  // the empty box of oct_bbox, through the separate calls
+        if ((s = oct_bbox(ctx, c, r))) return s;
+        if ((s = oct_encode(ctx, encoder, c, r, nullptr))) return s;
+        if ((s = oct_sort(ctx, perm_out, keys_out))) return s;
+    } else {
+        HCHK(hipSetDevice(ctx->device), "set device");
+        HCHK(launch_bbox_dev(ctx), "bbox");
+        HCHK(launch_encode(ctx, encoder, nullptr, nullptr), "encode");
+        ctx->encoder = encoder;
+        ctx->has_keys = true;
+        ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
+        if ((s = oct_sort(ctx, perm_out, keys_out))) return s;  // ends with a stream synchronisation
+        double b[12];
+        HCHK(hipMemcpyAsync(b, ctx->box.p, sizeof(b), hipMemcpyDeviceToHost, ctx->stream), "copy box");
+        HCHK(hipStreamSynchronize(ctx->stream), "copy box");
+        for (int a = 0; a < 3; a++) c[a] = b[6 + a], r[a] = b[9 + a];
+        for (int a = 0; a < 3; a++) ctx->center[a] = c[a], ctx->radii[a] = r[a];
+    }
     if (center_out)
         for (int a = 0; a < 3; a++) center_out[a] = c[a];
     if (radii_out)

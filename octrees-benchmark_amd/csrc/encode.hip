@@ -46,8 +46,10 @@ __global__ __launch_bounds__(kBboxBlock) void k_bbox(const double *__restrict__ 
     }
 }
 
+// out: min and max per axis; with box, also [min, max, centre, radii] per axis there, the centre and
+// radii in the reference's mbb expressions (util.hpp:139-143), so they equal the host's values.
 __global__ __launch_bounds__(kBboxBlock) void k_bbox_final(const double *__restrict__ partial, int nb,
-                                                           double *__restrict__ out) {
+                                                           double *__restrict__ out, double *__restrict__ box) {
     __shared__ double s[kBboxBlock / 64][6];
     double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int b = threadIdx.x; b < nb; b += blockDim.x)
@@ -75,20 +77,40 @@ __global__ __launch_bounds__(kBboxBlock) void k_bbox_final(const double *__restr
                 v[3 + a] = (v[3 + a] < s[j][3 + a]) ? s[j][3 + a] : v[3 + a];
             }
         for (int k = 0; k < 6; k++) out[k] = v[k];
+        if (box)
+            for (int a = 0; a < 3; a++) {
+                box[a] = v[a], box[3 + a] = v[3 + a];
+                box[6 + a] = v[a] + (v[3 + a] - v[a]) / 2;
+                box[9 + a] = (v[3 + a] - v[a]) / 2;
+            }
     }
 }
 
-hipError_t launch_bbox(oct_ctx *ctx, double mn[3], double mx[3]) {
+static hipError_t enqueue_bbox(oct_ctx *ctx, double **out, double *box) {
     hipError_t e = ctx->misc.reserve(kBboxBlocks * 12 + 64);
     if (e != hipSuccess) return e;
     double *partial = reinterpret_cast<double *>(ctx->misc.p);
-    double *out = partial + (size_t)kBboxBlocks * 6;
+    *out = partial + (size_t)kBboxBlocks * 6;
     int nb = (int)((ctx->n + kBboxBlock - 1) / kBboxBlock);
     nb = nb < 1 ? 1 : (nb > kBboxBlocks ? kBboxBlocks : nb);
     stage_begin(ctx, ST_BBOX);
     k_bbox<<<nb, kBboxBlock, 0, ctx->stream>>>(ctx->xyz.p, ctx->n, partial);
-    k_bbox_final<<<1, kBboxBlock, 0, ctx->stream>>>(partial, nb, out);
+    k_bbox_final<<<1, kBboxBlock, 0, ctx->stream>>>(partial, nb, *out, box);
     stage_end(ctx, ST_BBOX);
+    return hipGetLastError();
+}
+
+hipError_t launch_bbox_dev(oct_ctx *ctx) {
+    hipError_t e = ctx->box.reserve(12);
+    if (e != hipSuccess) return e;
+    double *out = nullptr;
+    return enqueue_bbox(ctx, &out, ctx->box.p);
+}
+
+hipError_t launch_bbox(oct_ctx *ctx, double mn[3], double mx[3]) {
+    double *out = nullptr;
+    hipError_t e = enqueue_bbox(ctx, &out, nullptr);
+    if (e != hipSuccess) return e;
     double h[6];
     e = hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) return e;
@@ -109,9 +131,15 @@ struct EncodeBox {
 
 // LSDH: the 8 LSD digit histograms and the partition passes' (bits 55..62, 47..54); else only the
 // latter (the partition sort needs no other, and two LDS atomics per point are cheaper than ten).
+// The box: box_arg, or with boxd (min, max, centre, radii per axis, as k_bbox_final leaves them) the
+// centre and radii read from there.
 template <int ENC, bool LSDH>
-__global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__ xyz, size_t n, EncodeBox box,
-                                                      uint64_t *__restrict__ keys, uint32_t *__restrict__ hist) {
+__global__ __launch_bounds__(kEncBlock) void k_encode(const double *__restrict__ xyz, size_t n, EncodeBox box_arg,
+                                                      const double *__restrict__ boxd, uint64_t *__restrict__ keys,
+                                                      uint32_t *__restrict__ hist) {
+    EncodeBox box = box_arg;
+    if (boxd)
+        for (int a = 0; a < 3; a++) box.c[a] = boxd[6 + a], box.r[a] = boxd[9 + a];
     __shared__ uint32_t sh[kEncHists * 256];
     __shared__ uint16_t stab[48 * 8];
     __shared__ uint16_t stab2[kHilbertTab2];
@@ -193,8 +221,10 @@ hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], cons
     hipError_t e;
     if ((e = ctx->keys[0].reserve(n)) != hipSuccess) return e;
     if ((e = ctx->hist.reserve(kEncHists * 256)) != hipSuccess) return e;
-    EncodeBox box;
-    for (int a = 0; a < 3; a++) box.c[a] = center[a], box.r[a] = radii[a];
+    EncodeBox box = {};
+    const double *boxd = center ? nullptr : ctx->box.p;
+    if (center)
+        for (int a = 0; a < 3; a++) box.c[a] = center[a], box.r[a] = radii[a];
     stage_begin(ctx, ST_ENCODE);
     e = hipMemsetAsync(ctx->hist.p, 0, kEncHists * 256 * sizeof(uint32_t), ctx->stream);
     if (e != hipSuccess) return e;
@@ -204,14 +234,14 @@ hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], cons
         const bool lsdh = !sort_partition_planned(ctx);
         if (encoder == OCT_ENC_HILBERT3D) {
             if (lsdh)
-                k_encode<OCT_ENC_HILBERT3D, true><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+                k_encode<OCT_ENC_HILBERT3D, true><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, boxd, ctx->keys[0].p, ctx->hist.p);
             else
-                k_encode<OCT_ENC_HILBERT3D, false><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+                k_encode<OCT_ENC_HILBERT3D, false><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, boxd, ctx->keys[0].p, ctx->hist.p);
         } else {
             if (lsdh)
-                k_encode<OCT_ENC_MORTON3D, true><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+                k_encode<OCT_ENC_MORTON3D, true><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, boxd, ctx->keys[0].p, ctx->hist.p);
             else
-                k_encode<OCT_ENC_MORTON3D, false><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, ctx->keys[0].p, ctx->hist.p);
+                k_encode<OCT_ENC_MORTON3D, false><<<nb, kEncBlock, 0, ctx->stream>>>(ctx->xyz.p, n, box, boxd, ctx->keys[0].p, ctx->hist.p);
         }
         ctx->hist_lsd = lsdh;
     }

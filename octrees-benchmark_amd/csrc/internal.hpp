@@ -104,6 +104,7 @@ struct oct_ctx {
     oct::DevBuf<uint64_t> keys[2];
     oct::DevBuf<uint32_t> vals[2];
     oct::DevBuf<uint32_t> hist;      // 8 x 256 digit histograms, then bits 55..62 and 47..54
+    oct::DevBuf<double> box;         // oct_sort_points: min, max, centre and radii per axis, on the device
     oct::DevBuf<uint32_t> lookback;  // per pass, per tile, per bin
     oct::DevBuf<uint32_t> misc;      // tile counters, flags, scan partials
     oct::DevBuf<uint32_t> msd;             // partition sort: bases, prefix counts, segment jobs, counters
@@ -179,6 +180,10 @@ namespace oct {
 
 // ---- launchers implemented in the .hip files (all enqueue on ctx->stream) ----
 hipError_t launch_bbox(oct_ctx *ctx, double mn[3], double mx[3]);
+// The same reduction left on the device: ctx->box receives min, max, centre and radii (12 doubles),
+// nothing is read back (n >= 1).
+hipError_t launch_bbox_dev(oct_ctx *ctx);
+// Encode with the box given on the host, or (center == nullptr) the one launch_bbox_dev left in ctx->box.
 hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], const double radii[3]);
 hipError_t launch_sort(oct_ctx *ctx, std::string &err);
 hipError_t launch_gather(oct_ctx *ctx);

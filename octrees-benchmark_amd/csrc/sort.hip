@@ -620,19 +620,27 @@ __global__ __launch_bounds__(256) void k_msd_jobs(const uint32_t *__restrict__ h
 
 // Everything the partition passes start from, in one launch (no host copies or fills in between):
 // zero look-back words, prefix counts, job and tile counters and the error flag; the bin starts of
-// both passes.
-struct PartInit {
-    uint32_t base[2][256];  // [0]: bits 47..54, [1]: bits 55..62
-};
-__global__ __launch_bounds__(256) void k_part_init(PartInit P, size_t nlook, uint32_t *__restrict__ lookback,
+// both passes, by block 0 from the encode kernel's histograms of bits 47..54 (h47) and 55..62 (h55).
+__global__ __launch_bounds__(256) void k_part_init(const uint32_t *__restrict__ h47, const uint32_t *__restrict__ h55,
+                                                   size_t nlook, uint32_t *__restrict__ lookback,
                                                    uint32_t *__restrict__ bases, uint32_t *__restrict__ hist2,
                                                    uint32_t *__restrict__ njobs, uint32_t *__restrict__ counters) {
     const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
     for (size_t i = g; i < nlook; i += stride) lookback[i] = 0u;
     for (size_t i = g; i < 65536; i += stride) hist2[i] = 0u;
-    if (g < 256) bases[g] = P.base[0][g], bases[256 + g] = P.base[1][g];
     if (g < 4) njobs[g] = 0u;
     if (g < 32) counters[g] = 0u;
+    if (blockIdx.x == 0) {
+        __shared__ uint32_t s_w[2][4];
+        const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+        const uint32_t c0 = h47[tid], c1 = h55[tid];
+        const uint32_t i0 = wave_incl_scan(c0), i1 = wave_incl_scan(c1);
+        if (lane == 63) s_w[0][w] = i0, s_w[1][w] = i1;
+        __syncthreads();
+        uint32_t p0 = 0, p1 = 0;
+        for (uint32_t ww = 0; ww < w; ww++) p0 += s_w[0][ww], p1 += s_w[1][ww];
+        bases[tid] = p0 + i0 - c0, bases[256 + tid] = p1 + i1 - c1;
+    }
 }
 
 // The 8 LSD digit histograms of n keys (hist must be zero), for an LSD sort after an encode that
@@ -675,13 +683,10 @@ unsigned seg_grid() {
 // njobs[2] tells the host), keys[1] / vals[1] and the sorted cloud out. h47, h55: the histograms of
 // bits 47..54 and 55..62. Launches only; returns where the njobs words are.
 template <int W, int IPT>
-hipError_t sort_partition(oct_ctx *ctx, const uint32_t *h47, const uint32_t *h55, uint32_t *counters, uint32_t *errflag,
-                          uint32_t **njobs_dev) {
+hipError_t sort_partition(oct_ctx *ctx, uint32_t *counters, uint32_t *errflag, uint32_t **njobs_dev) {
     constexpr uint32_t cap = SegShared<W, IPT>::kCap;
     const size_t n = ctx->n;
-    PartInit P;
-    uint32_t r0 = 0, r1 = 0;
-    for (uint32_t d = 0; d < 256; d++) P.base[0][d] = r0, P.base[1][d] = r1, r0 += h47[d], r1 += h55[d];
+    const uint32_t *h55 = ctx->hist.p + kRadixPassCount * 256, *h47 = h55 + 256;  // the encode kernel's
     const size_t tiles = (n + kSortTile - 1) / kSortTile;
     // msd layout (u32 words): bases [2 x 256], hist2 [65536], njobs [4], big and small jobs
     const size_t o_bases = 0, o_hist2 = 512, o_njobs = o_hist2 + 65536, o_big = o_njobs + 4, o_small = o_big + 2 * kSegJobsMax,
@@ -697,7 +702,7 @@ hipError_t sort_partition(oct_ctx *ctx, const uint32_t *h47, const uint32_t *h55
     hipStream_t s = ctx->stream;
     const size_t nlook = 2 * tiles * 256;
     const unsigned ig = (unsigned)std::min<size_t>(2048, std::max<size_t>((nlook + 255) / 256, 256));
-    k_part_init<<<ig, 256, 0, s>>>(P, nlook, ctx->lookback.p, M + o_bases, M + o_hist2, M + o_njobs, counters);
+    k_part_init<<<ig, 256, 0, s>>>(h47, h55, nlook, ctx->lookback.p, M + o_bases, M + o_hist2, M + o_njobs, counters);
     // bits 47..54: keys[0] (values: the positions) -> keys[1] / vals[1]
     k_onesweep<false><<<(unsigned)tiles, kSortBlock, 0, s>>>(ctx->keys[0].p, nullptr, ctx->keys[1].p, ctx->vals[1].p, n,
                                                              kMsdShift2, M + o_bases, ctx->lookback.p, counters + 0, errflag,
@@ -782,20 +787,22 @@ hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     // misc layout: [0..16) tile counters, [16] error flag, [32..32+2048) bin bases
     if ((e = ctx->misc.reserve(12 * 1024 + 64)) != hipSuccess) return e;
     uint32_t *counters = ctx->misc.p, *errflag = ctx->misc.p + 16, *bases = ctx->misc.p + 32;
-    // histograms from the encode kernel: the partition passes' always, the LSD digits' if hist_lsd
-    uint32_t h[kEncHists * 256];
-    if ((e = hipMemcpyAsync(h, ctx->hist.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
     ctx->p64_sorted = false;
     // keys come from the 3-D encoders (63 bits), so bits 47..62 are the top 16 the partition sort uses
     const int mode = sort_mode(ctx);
     const bool partition = sort_partition_planned(ctx);
+    // the LSD passes are planned on the host from the encode kernel's histograms; the partition sort
+    // takes its bin starts from them on the device, so nothing is read back before it
+    uint32_t h[kEncHists * 256];
+    if (!partition) {
+        if ((e = hipMemcpyAsync(h, ctx->hist.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+    }
     stage_begin(ctx, ST_SORT);
     uint32_t *njobs_dev = nullptr;
     if (partition) {
-        const uint32_t *h55 = h + kRadixPassCount * 256, *h47 = h55 + 256;
-        e = mode == 2 ? sort_partition<8, 8>(ctx, h47, h55, counters, errflag, &njobs_dev)
-                      : sort_partition<8, 4>(ctx, h47, h55, counters, errflag, &njobs_dev);
+        e = mode == 2 ? sort_partition<8, 8>(ctx, counters, errflag, &njobs_dev)
+                      : sort_partition<8, 4>(ctx, counters, errflag, &njobs_dev);
         if (e != hipSuccess) return e;
         ctx->p64_sorted = true;
         ctx->sorted_keys = ctx->keys[1].p;
