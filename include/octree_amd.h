@@ -114,7 +114,9 @@ int oct_sort(oct_ctx *ctx, uint32_t *perm_out, uint64_t *keys_out);
  * back). in and out must not overlap. */
 int oct_gather_records(oct_ctx *ctx, const void *in, void *out, size_t record_bytes, int device_pointers);
 
-/* One call for sortPoints(points, meta) (point_encoder.hpp:484-503): bbox + encode + sort. */
+/* One call for sortPoints(points, meta) (point_encoder.hpp:484-503): bbox + encode + sort, with the
+ * same results as oct_bbox + oct_encode + oct_sort; the box stays on the device between the three
+ * steps and is read back once at the end, so the host waits only there. */
 int oct_sort_points(oct_ctx *ctx, int encoder, double center_out[3], double radii_out[3], uint32_t *perm_out,
                     uint64_t *keys_out);
 
