@@ -112,3 +112,37 @@ def test_sort_knob_rejects_bad_values(ctx):
     with pytest.raises(A.OctError):
         ctx.set_sort(3)
     ctx.set_sort(None)
+
+
+def plane_cloud(n, seed):
+    """Points on the plane z = 0 (a zero-extent axis), negative and positive x and y."""
+    rng = np.random.default_rng(seed)
+    xyz = np.zeros((n, 3))
+    xyz[:, :2] = 200.0 * rng.random((n, 2)) - 100.0
+    return xyz
+
+
+@pytest.mark.parametrize("name,make", [
+    ("uniform_3", lambda: A.uniform_box(3, seed=5)),
+    ("uniform_4097", lambda: A.uniform_box(4097, seed=6)),
+    ("tile_300k", lambda: A.lidar_tile(300_000, seed=7, extent=120.0)),
+    ("plane_50k", lambda: plane_cloud(50_000, 8)),
+])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_sort_points_equals_separate_calls(ctx, restore_sort, name, make, mode):
+    """oct_sort_points keeps the box on the device between bbox, encode and sort; the separate calls
+    hand it through the host. Same box (bit for bit), keys and permutation."""
+    xyz = make()
+    say(f"[sort] sort_points vs separate calls {name} mode={mode}")
+    ctx.set_sort(mode)
+    ctx.set_points(xyz)
+    ctx.current = None
+    c1, r1, p1, k1 = ctx.sort_points(A.ENC_HILBERT)
+    ctx.set_points(xyz)
+    c2, r2 = ctx.bbox()
+    ctx.encode(A.ENC_HILBERT, c2, r2, fetch=False)
+    p2, k2 = ctx.sort()
+    assert np.array_equal(c1.view(np.uint64), c2.view(np.uint64))
+    assert np.array_equal(r1.view(np.uint64), r2.view(np.uint64))
+    np.testing.assert_array_equal(k1, k2)
+    np.testing.assert_array_equal(p1, p2)
