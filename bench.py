@@ -411,7 +411,31 @@ def main():
         log(f"[bench] {name} step {wall * 1e3:.2f} ms kernel {np.mean(kms):.2f} ms stats {stats}")
 
     fp32, fp64, fp32v = results["fp32"], results["fp64"], results["fp32_alt"]
-    ctx.set_mfma(None)
+    # the alternative's autotune left its own unit as the context's FP32 choice: pin the tuned one
+    ctx.set_mfma(bool(tuned["fp32"]["mfma"]))
+
+    # One GPU's share of the all-points search on 8 GPUs: the queries split into 8 contiguous SFC
+    # ranges, each searched against the whole (replicated) cloud and tree, FP32 + recheck as tuned
+    shard = None
+    if n >= 8:
+        bounds = np.linspace(0, n, 9).astype(np.int64)
+        sidx = A.DeviceArray(np.arange(n, dtype=np.uint32))  # the sorted-cloud indices, in SFC order
+        try:
+            sms, snb = [], 0
+            for s in range(8):
+                q0, nq_s = int(bounds[s]), int(bounds[s + 1] - bounds[s])
+                ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, args.radius, nq=nq_s, idx_ptr=sidx.ptr(q0))
+                ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, args.radius, nq=nq_s, idx_ptr=sidx.ptr(q0))
+                sms.append(float(ctx.stage_times()["search"]))
+                st_s = ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, args.radius, nq=nq_s, stats=True,
+                                      idx_ptr=sidx.ptr(q0))
+                snb += int(st_s["total_neighbors"])
+        finally:
+            sidx.close()
+        shard = {"gpus": 8, "queries_per_shard": int(bounds[1] - bounds[0]), "kernel_ms": sms,
+                 "max_ms": max(sms), "sum_ms": sum(sms), "balance": float(np.mean(sms) / max(sms)),
+                 "neighbors_eq_all_points": snb == int(fp32["stats"]["total_neighbors"])}
+        log(f"[bench] 8 query shards of {shard['queries_per_shard']}: kernel ms {[round(v, 2) for v in sms]}")
 
     # kNN (KNNV2) on the same tile: every point as a query (the first nq SFC positions for large k, the
     # per-query work being additive), int32 indices + FP32 squared distances kept on the device; each
@@ -444,12 +468,13 @@ def main():
     qps = n / fp32["wall_s"]
     verify = {}
     if not args.no_verify:
+        ctx.set_mfma(bool(tuned["fp32"]["mfma"]))
         c32, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, radius=args.radius, nq=n)
         c64, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP64, radius=args.radius, nq=n)
         verify["fp32_eq_fp64_all_queries"] = bool(np.array_equal(c32, c64))
         ctx.set_mfma(not tuned["fp32"]["mfma"])
         c32v, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, radius=args.radius, nq=n)
-        ctx.set_mfma(None)
+        ctx.set_mfma(bool(tuned["fp32"]["mfma"]))
         verify["fp32_other_unit_eq_fp64_all_queries"] = bool(np.array_equal(c32v, c64))
         del c32v
         verify["avg_result_size"] = int(c64.astype(np.int64).sum() // max(n, 1))
@@ -665,6 +690,7 @@ def main():
                          for kk, v in ent.items()} for k, ent in knn.items()},
         "roofline": roofline,
         "roofline_encode_sort": roofline_es,
+        "shard_8": shard,
         "cpu_baseline": cpu,
         "competitor_cpu": competitor,
     }

@@ -76,7 +76,37 @@ class SearchStats(C.Structure):
 
 _LIB = None
 _SYNTH = None
+_HIP = None
 _vp = C.c_void_p
+
+
+class DeviceArray:
+    """A device copy of a numpy array for the device-pointer entry points, allocated with the HIP
+    runtime the library loaded; ptr(offset) is the address of element `offset`; close() frees it."""
+
+    def __init__(self, arr):
+        global _HIP
+        lib()
+        if _HIP is None:
+            _HIP = C.CDLL("libamdhip64.so.7")
+            _HIP.hipMalloc.argtypes = [C.POINTER(_vp), C.c_size_t]
+            _HIP.hipMemcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]
+            _HIP.hipFree.argtypes = [_vp]
+        self.arr = np.ascontiguousarray(arr)
+        self.p = _vp()
+        if _HIP.hipMalloc(C.byref(self.p), max(self.arr.nbytes, 16)) != 0:
+            raise OctError("hipMalloc failed")
+        if self.arr.nbytes and _HIP.hipMemcpy(self.p, self.arr.ctypes.data, self.arr.nbytes, 1) != 0:
+            self.close()
+            raise OctError("hipMemcpy to the device failed")
+
+    def ptr(self, offset=0):
+        return _vp(self.p.value + int(offset) * self.arr.itemsize)
+
+    def close(self):
+        if self.p:
+            _HIP.hipFree(self.p)
+            self.p = _vp()
 
 
 def lib():
@@ -619,11 +649,12 @@ class Octree:
         self._chk(self.L.oct_radius_fetch(self.h, _ptr(offs), _ptr(ent), ent.shape[0]), "oct_radius_fetch")
         return counts, offs, ent[:words], st.as_dict()
 
-    def radius_dev(self, kernel, algo, precision, radius, nq=None, stats=False):
-        """All-points search on the device-resident sorted cloud; counts stay on the device."""
+    def radius_dev(self, kernel, algo, precision, radius, nq=None, stats=False, idx_ptr=None):
+        """All-points search on the device-resident sorted cloud (or, with idx_ptr, the nq sorted-cloud
+        indices in that device buffer as centres); counts stay on the device."""
         nq = self.n if nq is None else nq
         st = SearchStats() if stats else None
-        self._chk(self.L.oct_radius_dev(self.h, kernel, algo, precision, None, float(radius), nq, None,
+        self._chk(self.L.oct_radius_dev(self.h, kernel, algo, precision, idx_ptr, float(radius), nq, None,
                                         C.byref(st) if stats else None), "oct_radius_dev")
         return st.as_dict() if stats else None
 
