@@ -532,12 +532,14 @@ int oct_sort_points(oct_ctx *ctx, int encoder, double center_out[3], double radi
         ctx->encoder = encoder;
         ctx->has_keys = true;
         ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
-        if ((s = oct_sort(ctx, perm_out, keys_out))) return s;  // ends with a stream synchronisation
+        s = oct_sort(ctx, perm_out, keys_out);  // ends with a stream synchronisation
+        // the box belongs to the encoded keys, which stay valid for another oct_sort if this one failed
         double b[12];
         HCHK(hipMemcpyAsync(b, ctx->box.p, sizeof(b), hipMemcpyDeviceToHost, ctx->stream), "copy box");
         HCHK(hipStreamSynchronize(ctx->stream), "copy box");
         for (int a = 0; a < 3; a++) c[a] = b[6 + a], r[a] = b[9 + a];
         for (int a = 0; a < 3; a++) ctx->center[a] = c[a], ctx->radii[a] = r[a];
+        if (s) return s;
     }
     if (center_out)
         for (int a = 0; a < 3; a++) center_out[a] = c[a];
