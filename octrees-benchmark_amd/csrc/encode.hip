@@ -223,6 +223,7 @@ hipError_t launch_encode(oct_ctx *ctx, int encoder, const double center[3], cons
     if ((e = ctx->hist.reserve(kEncHists * 256)) != hipSuccess) return e;
     EncodeBox box = {};
     const double *boxd = center ? nullptr : ctx->box.p;
+    if (!center && !boxd) return hipErrorInvalidValue;  // no box on the host, none left on the device
     if (center)
         for (int a = 0; a < 3; a++) box.c[a] = center[a], box.r[a] = radii[a];
     stage_begin(ctx, ST_ENCODE);
