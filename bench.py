@@ -365,9 +365,10 @@ def main():
         xyz = A.lidar_tile(n, seed=2, extent=extent)
         log(f"[bench] generated {n} points over {extent:.0f} m in {time.time() - t0:.1f} s")
         ctx.set_points(xyz)
-    # encode + sort + build (untimed warm-up, then timed with device events)
+    # encode + sort + build (an untimed warm-up that also fetches the keys and permutation, then 5
+    # repetitions timed with device events; each stage's median is reported)
     sort_ms, enc_ms, bbox_ms, gather_ms, build_ms = [], [], [], [], []
-    for rep in range(3):
+    for rep in range(6):
         c, r, perm, keys = ctx.sort_points(A.ENC_HILBERT, fetch=(rep == 0))
         if rep == 0:
             keys0, perm0 = keys, perm
