@@ -113,7 +113,6 @@ def profile(args):
     ctx.build(args.max_leaf)
     for v in spec.split(","):
         prec = A.PREC_FP64 if v[0] == "d" else A.PREC_FP32
-        ctx.set_mfma(v[0] == "m")
         ctx.set_group(int(v[1:]))
         ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, prec, args.radius)
         ctx.synchronize()
@@ -325,7 +324,7 @@ def main():
                                                  "instead of the synthetic tile")
     ap.add_argument("--profile", default=None,
                     help="profiling driver instead of the benchmark: 'search:v64,d64' (one all-points search per "
-                         "configuration: v = FP32 vector ALUs, m = FP32 matrix cores, d = FP64, then queries per "
+                         "configuration: v = FP32 + recheck, d = FP64, then queries per "
                          "wave), 'knn:10' (one all-points kNN per precision), 'sort' (bbox + encode + sort + gather "
                          "+ build, 3 times, then the 1 GiB copy kernel) or 'copy' (the 1 GiB copy kernel alone, the "
                          "counter calibration)")
@@ -384,19 +383,13 @@ def main():
     K, W = args.steps, args.warmup
     results = {}
     tuned = {}
-    # Each precision gets its own best configuration, tuned on the full workload: FP32 + recheck
-    # (tile shape and point-test unit: vector ALUs or matrix cores), plain FP64 (tile shape). The
-    # FP32 form with the other point-test unit is timed too, for comparison.
-    variants = [("fp32", A.PREC_FP32, None), ("fp64", A.PREC_FP64, None), ("fp32_alt", A.PREC_FP32, "alt")]
-    for name, prec, mx in variants:
-        if mx == "alt":
-            ctx.set_mfma(not tuned["fp32"]["mfma"])
-        else:
-            ctx.set_mfma(None)
+    # Each precision gets its own best tile shape, tuned on the full workload: FP32 + recheck and
+    # plain FP64.
+    for name, prec in (("fp32", A.PREC_FP32), ("fp64", A.PREC_FP64)):
         best, cand = ctx.autotune_radius(A.K_SPHERE, A.A_STRUCT, prec, args.radius, sample_nq=n)
-        g, use_mx = ctx.tuned_radius(A.K_SPHERE, A.A_STRUCT, prec)
-        tuned[name] = {"queries_per_wave": g, "mfma": use_mx, "candidate_ms": cand}
-        log(f"[bench] autotune {name}: G={g} mfma={use_mx} candidates(ms)={cand}")
+        g = ctx.tuned_radius(A.K_SPHERE, A.A_STRUCT, prec)
+        tuned[name] = {"queries_per_wave": g, "candidate_ms": cand}
+        log(f"[bench] autotune {name}: G={g} candidates(ms)={cand}")
         for _ in range(W):
             ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, prec, args.radius)
         sync_all(ctx)
@@ -411,9 +404,7 @@ def main():
         results[name] = {"wall_s": wall, "kernel_ms": float(np.mean(kms)), "stats": stats}
         log(f"[bench] {name} step {wall * 1e3:.2f} ms kernel {np.mean(kms):.2f} ms stats {stats}")
 
-    fp32, fp64, fp32v = results["fp32"], results["fp64"], results["fp32_alt"]
-    # the alternative's autotune left its own unit as the context's FP32 choice: pin the tuned one
-    ctx.set_mfma(bool(tuned["fp32"]["mfma"]))
+    fp32, fp64 = results["fp32"], results["fp64"]
 
     # One GPU's share of the all-points search on 8 GPUs: the queries split into 8 contiguous SFC
     # ranges, each searched against the whole (replicated) cloud and tree, FP32 + recheck as tuned
@@ -469,15 +460,9 @@ def main():
     qps = n / fp32["wall_s"]
     verify = {}
     if not args.no_verify:
-        ctx.set_mfma(bool(tuned["fp32"]["mfma"]))
         c32, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, radius=args.radius, nq=n)
         c64, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP64, radius=args.radius, nq=n)
         verify["fp32_eq_fp64_all_queries"] = bool(np.array_equal(c32, c64))
-        ctx.set_mfma(not tuned["fp32"]["mfma"])
-        c32v, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, radius=args.radius, nq=n)
-        ctx.set_mfma(bool(tuned["fp32"]["mfma"]))
-        verify["fp32_other_unit_eq_fp64_all_queries"] = bool(np.array_equal(c32v, c64))
-        del c32v
         verify["avg_result_size"] = int(c64.astype(np.int64).sum() // max(n, 1))
 
     cpu = None
@@ -616,7 +601,7 @@ def main():
     digest = source_digest()
     try:
         tr = json.load(open(os.path.join(ROOT, "profiles", "search_c2_50M_traffic.json")))
-        cfg = f"k_count<1, 3, 1, {tuned['fp32']['queries_per_wave']}, false, {str(tuned['fp32']['mfma']).lower()}>"
+        cfg = f"k_count<1, 3, 1, {tuned['fp32']['queries_per_wave']}, false>"
         if n != 50_000_000 or las is not None:
             traffic_note = "counter summary is of the 50 M-point tile"
         elif not tr["kernel"].startswith(cfg):
@@ -630,7 +615,7 @@ def main():
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "traffic_source": traffic_note,
                 "kernel": "radius count (FP32 + FP64 re-check)",
-                "compute_unit": "matrix cores" if tuned["fp32"]["mfma"] else "vector ALUs",
+                "compute_unit": "vector ALUs",
                 "work": f"9 flops x {T} point tests of the reference traversal"}
 
     es_ms = med(bbox_ms) + med(enc_ms) + med(sort_ms) + med(gather_ms)
@@ -671,7 +656,6 @@ def main():
                    "encoder": "hilbert"},
         "fp64_queries_per_s": n / fp64["wall_s"],
         "fp32_vs_fp64_speedup": fp64["wall_s"] / fp32["wall_s"],
-        "fp32_other_unit_queries_per_s": n / fp32v["wall_s"],
         "recheck_rate": fp32["stats"]["rechecked"] / max(1, fp32["stats"]["points_tested"]),
         "neighbors_per_s": fp32["stats"]["total_neighbors"] / fp32["wall_s"],
         "build_ms": med(build_ms),
@@ -680,7 +664,7 @@ def main():
         "points_encoded_per_s": n / (med(enc_ms) * 1e-3),
         "tree": {k: info[k] for k in ("n_leaf", "n_internal", "n_total", "max_depth_seen")},
         "stages_ms": {"bbox": med(bbox_ms), "encode": med(enc_ms), "sort": med(sort_ms), "gather": med(gather_ms),
-                      "build": med(build_ms), "search_fp32": fp32["kernel_ms"], "search_fp32_other_unit": fp32v["kernel_ms"],
+                      "build": med(build_ms), "search_fp32": fp32["kernel_ms"],
                       "search_fp64": fp64["kernel_ms"]},
         "verify": verify,
         "las": las,

@@ -222,14 +222,14 @@ int oct_knn_checksum(oct_ctx *ctx, const int32_t *idx_dev, const float *d2_dev, 
 /* ---- tuning (no counterpart in the reference: its per-query loop has no tile shape to choose).
  * OCT_TUNE_RADIUS_GROUP: queries per wave of the count-mode radius kernel (16, 32 or 64 queries
  * share one wave's traversal, 64 / G lanes serve each query), or 0 for the one-lane-per-query
- * kernel; -1 restores "autotuned, else 32". OCT_TUNE_RADIUS_MFMA: 1 runs the FP32 sphere count-mode
- * point tests on the matrix cores (16 points x 16 queries per v_mfma_f32_16x16x4_f32), 0 on the
- * vector ALUs, -1 restores "autotuned, else vector ALUs"; every setting gives the same results.
- * oct_autotune_radius: times each candidate (G = 0, 16, 32, 64, and for FP32 sphere both point-test
- * units unless fixed by oct_set_tuning) on sample_nq queries taken from the sorted cloud (16 equal
- * strided chunks of consecutive SFC points; sample_nq >= n uses every point), stores the fastest for
- * (kernel, algo, precision), returns its G in *best and the best milliseconds per G in ms_out[0..3].
- * oct_tuned_radius: the G and point-test unit a count-mode search would use now.
+ * kernel; -1 restores "autotuned, else 32". OCT_TUNE_RADIUS_MFMA: the point tests run on the vector
+ * ALUs (a matrix-core form measured slower on every workload and is not built): 0 and -1 are
+ * accepted, 1 fails with OCT_ERR_UNSUPPORTED.
+ * oct_autotune_radius: times each candidate (G = 0, 16, 32, 64) on sample_nq queries taken from the
+ * sorted cloud (16 equal strided chunks of consecutive SFC points; sample_nq >= n uses every point),
+ * stores the fastest for (kernel, algo, precision), returns its G in *best and the milliseconds per G
+ * in ms_out[0..3].
+ * oct_tuned_radius: the G a count-mode search would use now (*mfma, if given, is always 0).
  * OCT_TUNE_RADIUS_REMAP: 1 (default, also -1) lets the count-mode vector point tests move the
  * queries of a leaf that at most half of a wave's queries want onto all 64 lanes; 0 keeps every
  * query on its own lanes.

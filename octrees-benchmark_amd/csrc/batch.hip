@@ -468,7 +468,7 @@ static hipError_t enqueue_search(oct_batch *b, int kernel, int algo, int precisi
     if (b->rpass.n && only_class < 0) {
         if ((e = enqueue_query_groups(s, b->rpass.Q, C, b->cloud_off.p)) != hipSuccess) return e;
         RadiusLaunch L;
-        L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = 64, L.mx = false, L.remap = true;
+        L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = 64, L.remap = true;
         L.order = b->rpass.Q.order[0], L.qidx = b->rpass.Q.gidx, L.qrad = b->p_rad.p, L.nq = b->rpass.Q.slots[0];
         L.counts = b->p_counts.p, L.tile_info = b->rpass.Q.tile_info[0], L.roots = b->roots.p, L.radtab = b->radtab.p;
         L.queue_tiles = b->rpass.Q.ctrl + kQCtrlTiles, L.queue_next = b->rpass.Q.ctrl + kQCtrlQueue;
@@ -906,7 +906,7 @@ int oct_batch_radius(oct_batch *b, int kernel, int algo, int precision, const ui
     BCHK(hipMemcpyAsync(b->qidx.p, gidx.data(), nq * 4, hipMemcpyHostToDevice, f->stream), "copy");
     BCHK(hipMemcpyAsync(b->qrad.p, radii, nq * 8, hipMemcpyHostToDevice, f->stream), "copy");
     RadiusLaunch L;
-    L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = 64, L.mx = false, L.remap = true;
+    L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = 64, L.remap = true;
     L.order = b->order.p, L.qidx = b->qidx.p, L.qrad = b->qrad.p, L.nq = order.size(), L.counts = b->counts.p;
     L.tile_info = b->tile_info.p, L.roots = b->roots.p, L.radtab = b->radtab.p;
     if (stats) {

@@ -346,9 +346,6 @@ int oct_create(oct_ctx **out, int device) {
     for (auto &a : ctx->group_tuned)
         for (auto &b : a)
             for (auto &c : b) c = -1;
-    for (auto &a : ctx->mx_tuned)
-        for (auto &b : a)
-            for (auto &c : b) c = -1;
     for (auto &a : ctx->knn_lanes_tuned)
         for (auto &b : a) b = -1;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -622,11 +619,6 @@ static int radius_group(const oct_ctx *ctx, int kernel, int algo, int precision)
     int t = ctx->group_tuned[kernel & 3][algo & 3][precision & 1];
     return t >= 0 ? t : 32;
 }
-// FP32 sphere point tests on the matrix cores: explicit override, else autotuned, else vector ALUs.
-static bool radius_mx(const oct_ctx *ctx, int kernel, int algo, int precision) {
-    if (ctx->mx_override >= 0) return ctx->mx_override != 0;
-    return ctx->mx_tuned[kernel & 3][algo & 3][precision & 1] == 1;
-}
 
 static int read_stats(oct_ctx *ctx, oct_search_stats *stats, double ms) {
     unsigned long long h[6];
@@ -678,7 +670,6 @@ static int radius_impl(oct_ctx *ctx, int kernel, int algo, int precision, const 
     RadiusLaunch L;
     L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = want_lists ? 1 : 0;
     L.group = radius_group(ctx, kernel, algo, precision);
-    L.mx = radius_mx(ctx, kernel, algo, precision);
     L.remap = ctx->remap_override != 0;
     L.approx_level = approx_level, L.approx_upper = approx_upper;
     L.order = d_order;
@@ -873,7 +864,6 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
     RadiusLaunch L;
     L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0;
     L.group = radius_group(ctx, kernel, algo, precision);
-    L.mx = radius_mx(ctx, kernel, algo, precision);
     L.remap = ctx->remap_override != 0;
     L.qidx = center_idx_dev;
     L.radius = radius;
@@ -896,9 +886,10 @@ int oct_radius_dev(oct_ctx *ctx, int kernel, int algo, int precision, const uint
 
 int oct_set_tuning(oct_ctx *ctx, int knob, int value) {
     if (!ctx) return OCT_ERR_ARG;
-    if (knob == OCT_TUNE_RADIUS_MFMA) {
-        if (value != -1 && value != 0 && value != 1) return fail(ctx, OCT_ERR_ARG, "radius mfma switch must be -1, 0 or 1");
-        ctx->mx_override = value;
+    if (knob == OCT_TUNE_RADIUS_MFMA) {  // the point tests run on the vector ALUs only
+        if (value == 1)
+            return fail(ctx, OCT_ERR_UNSUPPORTED, "no matrix-core point tests: the vector ALUs measured faster on every workload");
+        if (value != -1 && value != 0) return fail(ctx, OCT_ERR_ARG, "radius mfma switch must be -1 or 0");
         return OCT_OK;
     }
     if (knob == OCT_TUNE_RADIUS_REMAP) {
@@ -944,36 +935,26 @@ int oct_autotune_radius(oct_ctx *ctx, int kernel, int algo, int precision, doubl
     HCHK(ctx->counts.reserve(nq + 1), "alloc counts");
     HCHK(hipMemcpyAsync(ctx->qidx.p, idx.data(), nq * 4, hipMemcpyHostToDevice, ctx->stream), "copy sample");
     const int cand[4] = {0, 16, 32, 64};
-    // the matrix-core point tests exist for FP32 sphere searches; an explicit switch is respected
-    const bool mx_possible = kernel == OCT_KERNEL_SPHERE && precision == OCT_PREC_FP32_RECHECK;
-    int mx_lo = 0, mx_hi = mx_possible ? 1 : 0;
-    if (ctx->mx_override >= 0) mx_lo = mx_hi = mx_possible ? ctx->mx_override : 0;
     double bestms = 1e300;
-    int bestg = 32, bestmx = 0;
+    int bestg = 32;
     for (int ci = 0; ci < 4; ci++) {
-        if (ms_out) ms_out[ci] = 1e300;
-        for (int mx = mx_lo; mx <= mx_hi; mx++) {
-            if (cand[ci] == 0 && mx) continue;  // the one-lane kernel has no matrix-core variant
-            RadiusLaunch L;
-            L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = cand[ci];
-            L.mx = mx != 0;
-            L.remap = ctx->remap_override != 0;
-            L.qidx = ctx->qidx.p;
-            L.radius = radius;
-            L.nq = nq;
-            L.counts = ctx->counts.p;
-            double t = 1e300;
-            for (int rep = 0; rep < 3; rep++) {  // first repetition warms caches and code objects
-                HCHK(launch_radius(ctx, L), "autotune");
-                HCHK(hipStreamSynchronize(ctx->stream), "autotune");
-                if (rep) t = std::min(t, stage_ms(ctx, ST_SEARCH));
-            }
-            if (ms_out) ms_out[ci] = std::min(ms_out[ci], t);
-            if (t < bestms) bestms = t, bestg = cand[ci], bestmx = mx;
+        RadiusLaunch L;
+        L.kernel = kernel, L.algo = algo, L.precision = precision, L.mode = 0, L.group = cand[ci];
+        L.remap = ctx->remap_override != 0;
+        L.qidx = ctx->qidx.p;
+        L.radius = radius;
+        L.nq = nq;
+        L.counts = ctx->counts.p;
+        double t = 1e300;
+        for (int rep = 0; rep < 3; rep++) {  // first repetition warms caches and code objects
+            HCHK(launch_radius(ctx, L), "autotune");
+            HCHK(hipStreamSynchronize(ctx->stream), "autotune");
+            if (rep) t = std::min(t, stage_ms(ctx, ST_SEARCH));
         }
+        if (ms_out) ms_out[ci] = t;
+        if (t < bestms) bestms = t, bestg = cand[ci];
     }
     ctx->group_tuned[kernel & 3][algo & 3][precision & 1] = bestg;
-    ctx->mx_tuned[kernel & 3][algo & 3][precision & 1] = bestmx;
     ctx->has_lists = false;
     if (best) *best = bestg;
     return OCT_OK;
@@ -984,7 +965,7 @@ int oct_tuned_radius(oct_ctx *ctx, int kernel, int algo, int precision, int *gro
     if (kernel < 0 || kernel > 3 || algo < 1 || algo > 3 || (precision != 0 && precision != 1))
         return fail(ctx, OCT_ERR_ARG, "unknown kernel, algorithm or precision");
     if (group) *group = radius_group(ctx, kernel, algo, precision);
-    if (mfma) *mfma = radius_mx(ctx, kernel, algo, precision) ? 1 : 0;
+    if (mfma) *mfma = 0;  // point tests on the vector ALUs
     return OCT_OK;
 }
 

@@ -154,10 +154,8 @@ struct oct_ctx {
     size_t last_nq = 0;
     int last_algo = 0;
     int group_override = -1;        // OCT_TUNE_RADIUS_GROUP (-1: use tuned / default)
-    int mx_override = -1;           // OCT_TUNE_RADIUS_MFMA (-1: use tuned / default)
     int remap_override = -1;        // OCT_TUNE_RADIUS_REMAP (-1: default, on)
     int group_tuned[4][4][2];       // autotuned group per (kernel, algo, precision), -1 = not tuned
-    int mx_tuned[4][4][2];          // autotuned point-test unit (1 = matrix cores), -1 = not tuned
     bool has_lists = false;
 
     // kNN
@@ -194,7 +192,6 @@ hipError_t launch_build(oct_ctx *ctx, uint32_t max_points_leaf, std::string &err
 struct RadiusLaunch {
     int kernel = 1, algo = 3, precision = 0, mode = 0;
     int group = 16;  // count mode: queries per wave (16, 32, 64), or 0 for the one-lane-per-query kernel
-    bool mx = false;  // count mode, FP32 sphere: point tests on the matrix cores
     bool remap = true;  // count mode, vector point tests: compact the queries of sparsely wanted leaves
     uint32_t approx_level = ~0u;  // OCT_ALGO_APPROX: neighborsApprox depth cutoff and bound mode
     bool approx_upper = false;

@@ -23,9 +23,8 @@
 //    re-evaluated with the FP64 expression. In count mode a lane re-counts a whole leaf in FP64 if
 //    any point fell in the band.
 //  - FP32_RECHECK, count kernel k_count: the same u with the band folded in, t = u + hw, from an
-//    FP32 set-up and a wider band hw = 2^-18 (amax + |b|^2 + R) that also covers any summation
-//    order, so the matrix-core tiles (16 points x 16 queries per v_mfma_f32_16x16x4_f32, sphere)
-//    and the vector path share one bound (see leaf_mx in k_count); counts come from sign bits.
+//    FP32 set-up and a wider band hw = 2^-18 (amax + |b|^2 + R) that covers the FP32 set-up's
+//    roundings as well (its bound: see the leaf's tests in k_count); counts come from sign bits.
 //  - A leaf/query pair whose FP32 terms could overflow, or a leaf holding a non-finite FP32 point
 //    (then amax = inf), is decided in FP64 throughout.
 //  - Cube/square tests are comparisons only and run on the FP64 coordinates in both modes.
@@ -335,18 +334,16 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
 // Struct / Prune algorithms: 8 for the FP32 vector path (64 VGPRs, a 20-byte spill, 1.5 % faster
 // than 7), 6 for FP64 (80 VGPRs). The other variants keep the compiler's choice (forcing them spills
 // registers).
-template <int KERNEL, int ALGO, int PREC, bool MX>
+template <int KERNEL, int ALGO, int PREC>
 constexpr int count_waves() {
-    if (MX || ALGO == OCT_ALGO_NEIGHBORS || !(KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE)) return 2;
+    if (ALGO == OCT_ALGO_NEIGHBORS || !(KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE)) return 2;
     return PREC == OCT_PREC_FP32_RECHECK ? 8 : 6;
 }
 // One wave tile (G queries) of a count-mode search, run by one wave.
-template <int KERNEL, int ALGO, int PREC, int G, bool STATS, bool MX>
+template <int KERNEL, int ALGO, int PREC, int G, bool STATS>
 __device__ __attribute__((always_inline)) inline void count_tile(const SearchArgs &a, const uint32_t tile) {
     constexpr int S = 64 / G;
     constexpr bool kFp32 = PREC == OCT_PREC_FP32_RECHECK && (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE);
-    static_assert(!MX || (kFp32 && KERNEL == OCT_KERNEL_SPHERE), "matrix-core leaf tests are FP32 sphere only");
-    constexpr int NH = MX ? G / 16 : 1;  // 16-query column blocks per wave
     // real points per staging chunk: 64 FP32 or 32 FP64 points (leaves average ~30 points; a small
     // chunk keeps the LDS per block low enough for 7 (FP32) or 6 (FP64) waves per SIMD)
     constexpr int CAP = kFp32 ? kStageBytes / 32 : kStageBytes / 64;
@@ -402,14 +399,7 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
     uint32_t cnt = 0;
     uint64_t n_class = 0, n_test = 0, n_recheck = 0, n_cslots = 0, n_tslots = 0;
 
-    // Matrix-core layout (MX): lane l owns column c16 = l % 16 and row group kk = l / 16 of each
-    // 16 x 16 tile D = A B + C of v_mfma_f32_16x16x4_f32; column block h holds queries 16h .. 16h+15
-    // of this wave, whose per-leaf terms are computed by the query's own lane (lane 16h + c16) and
-    // fetched with ds_bpermute. mcnt[h] is this lane's share of query 16h + c16's count.
-    const uint32_t c16 = lane & 15, kk = lane >> 4;
-    uint32_t mcnt[NH];
-#pragma unroll
-    for (int h = 0; h < NH; h++) mcnt[h] = 0;
+    // a value of another lane (ds_bpermute)
     auto bperm_u32 = [](uint32_t src_lane, uint32_t v) -> uint32_t {
         return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
     };
@@ -478,113 +468,6 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
         return iters;
     };
 
-    // Matrix-core leaf (FP32 sphere): 16 points x 16 queries per v_mfma_f32_16x16x4_f32,
-    // D = A B + C with A = the points' [ax, ay, az, w] rows, B = the queries' [-2b; 1] columns and
-    // C = fl32(fl32(hw - fl32(R)) + fl32(|b|^2)) per column, so D approximates u + hw with
-    // u = |p - q|^2 - R. Only b = fl32(q - c) is formed in FP64; the rest of the set-up is FP32.
-    // The tile's summation order is not specified, so its error is bounded for any association
-    // with every product and sum rounded to nearest or toward zero (u' = 2^-23, 5 terms); with the
-    // representation errors of a, b, w and the FP32 set-up (unit u = 2^-24):
-    //   |D - (u + hw)| <= u (25 amax + 28 |b|^2 + 13 R + 12 hw) < 32 u (amax + |b|^2 + R).
-    // hw = 2^-18 fl32(fl32(amax + fl32|b|^2) + fl32 R) >= 2 * 32 u (amax + |b|^2 + R)(1 - 5u) covers
-    // it with a factor-2 margin. Hence D <= 0 (sign bit set, -0 included) is inside, D > 2 hw is
-    // outside, and 0 <= D <= 2 hw - an unsigned compare of the bit patterns, which also sends +0
-    // there - is re-tested in FP64. Rows past the leaf are sentinels (0, 0, 0, 1e30): D ~ 1e30,
-    // positive and above 2 hw. A leaf/query pair with amax + |b|^2 + R >= 1e30 (or not finite: a
-    // non-finite point makes amax = inf) is re-tested in FP64 outright, so no term can overflow.
-    // The four blocks of a group are multiplied back to back and then counted.
-    auto leaf_mx = [&](uint64_t m, float amf, const NodeRec &nr, uint32_t first, uint32_t last) {
-        typedef float f4 __attribute__((ext_vector_type(4)));
-        const float *pf = reinterpret_cast<const float *>(a.p32);
-        const uint32_t nblk = (last - first + 15) >> 4;
-        // per-query terms in the query's own lane; a forced FP64 re-test is lim = ~0 (every D <= lim)
-        const float bx = (float)(q.x - nr.cx), by = (float)(q.y - nr.cy), bz = (float)(q.z - nr.cz);
-        const float b2 = __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz));
-        const float sum = (amf + b2) + qRf;
-        const float hw = 3.814697265625e-06f * sum;  // 2^-18
-        const float ciq = (hw - qRf) + b2;
-        const uint32_t limq = sum < 1e30f ? __float_as_uint(2.0f * hw) : 0xffffffffu;
-        const float nbx = -2.0f * bx, nby = -2.0f * by, nbz = -2.0f * bz;
-        // rows of 16-row blocks b0 .. b0+3: clamped row, then the sentinel (0, 0, 0, 1e30) past the leaf
-        auto load_rows = [&](uint32_t b0, float (&av)[4]) {
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const uint32_t row = first + 16 * (b0 + t) + c16;
-                const float v = pf[4 * (size_t)min(row, last - 1) + kk];
-                av[t] = row < last ? v : (kk == 3 ? 1e30f : 0.0f);
-            }
-        };
-        float av0[4];
-        load_rows(0, av0);
-        // one 16-query column block at a time: N = 1..4 blocks of 16 rows multiplied back to back,
-        // then counted (sign bits) and bounded (unsigned minimum of the bit patterns)
-#pragma unroll
-        for (int h = 0; h < NH; h++) {
-            if (((m >> (16 * h)) & 0xffffull) == 0) continue;  // no query of this block wants the leaf
-            const uint32_t src = 16 * h + c16;
-            const float ci = bperm_f32(src, ciq);
-            const uint32_t lim = bperm_u32(src, limq);
-            const float sx = bperm_f32(src, nbx), sy = bperm_f32(src, nby), sz = bperm_f32(src, nbz);
-            const float bop = kk == 0 ? sx : kk == 1 ? sy : kk == 2 ? sz : 1.0f;
-            uint32_t c8 = 0, mnb = 0xffffffffu;
-            auto tiles = [&](auto NB, const float (&av)[4]) {
-                constexpr int N = decltype(NB)::value;
-                const f4 cv = {ci, ci, ci, ci};
-                f4 d[N];
-#pragma unroll
-                for (int t = 0; t < N; t++) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bop, cv, 0, 0, 0);
-#pragma unroll
-                for (int t = 0; t < N; t++) {
-                    const uint32_t d0 = __float_as_uint(d[t][0]), d1 = __float_as_uint(d[t][1]);
-                    const uint32_t d2 = __float_as_uint(d[t][2]), d3 = __float_as_uint(d[t][3]);
-                    // v_perm selector 9 / 11: a byte of 8 copies of bit 31 of src1 / src0
-                    c8 = __builtin_popcount(__builtin_amdgcn_perm(d0, d1, 0x0c0c0b09u)) + c8;
-                    c8 = __builtin_popcount(__builtin_amdgcn_perm(d2, d3, 0x0c0c0b09u)) + c8;
-                    mnb = min(min(mnb, d0), d1);
-                    mnb = min(min(mnb, d2), d3);
-                }
-            };
-            for (uint32_t b0 = 0; b0 < nblk; b0 += 4) {
-                const uint32_t nb = (nblk - b0) < 4u ? (nblk - b0) : 4u;  // wave-uniform
-                float av[4];
-                if (b0 == 0) {
-#pragma unroll
-                    for (int t = 0; t < 4; t++) av[t] = av0[t];
-                } else {
-                    load_rows(b0, av);
-                }
-                if (nb == 4)
-                    tiles(std::integral_constant<int, 4>{}, av);
-                else if (nb == 3)
-                    tiles(std::integral_constant<int, 3>{}, av);
-                else if (nb == 2)
-                    tiles(std::integral_constant<int, 2>{}, av);
-                else
-                    tiles(std::integral_constant<int, 1>{}, av);
-            }
-            const bool act = ((m >> (16 * h + c16)) & 1ull) != 0;
-            uint32_t lc = c8 >> 3;
-            const uint64_t bm = ballot64(act && mnb <= lim);
-            const uint32_t cols = (uint32_t)((bm | (bm >> 16) | (bm >> 32) | (bm >> 48)) & 0xffffull);
-            if (cols) {  // rare: exact FP64 re-count of this column's rows for queries with a band hit
-                Query qh;
-                qh.x = bperm_f64(src, q.x), qh.y = bperm_f64(src, q.y), qh.z = bperm_f64(src, q.z);
-                qh.R = bperm_f64(src, q.R);
-                if ((cols >> c16) & 1u) {
-                    lc = 0;
-                    for (uint32_t i = first + 4 * kk; i < last; i += 16)
-                        for (uint32_t r = 0; r < 4 && i + r < last; r++) {
-                            const double4 pd = a.p64[i + r];
-                            lc += inside64<KERNEL>(qh, pd.x, pd.y, pd.z);
-                            if (STATS) n_recheck++;
-                        }
-                }
-            }
-            if (act) mcnt[h] += lc;
-            if (STATS) n_tslots += 256ull * nblk;  // (point, query) pairs evaluated
-        }
-    };
-
     // leaf: the queries in m count the leaf's points. Normally lane group g of each query takes
     // points g, g + S, ... . A leaf wanted by at most half of the wave's queries is remapped: its mc
     // queries are compacted onto groups of Sp = 64 / Gp lanes (Gp = max(4, mc rounded up to a power
@@ -594,10 +477,6 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
     auto leaf = [&](uint64_t m, float am, const NodeRec &nr) {
         const uint32_t first = nr.first, last = nr.last;
         if (STATS) n_test += (uint64_t)__popcll(m) * (last - first);
-        if constexpr (MX) {
-            leaf_mx(m, am, nr, first, last);
-            return;
-        }
         const uint32_t mc = (uint32_t)__popcll(m);
         const bool rm = a.remap && 2 * mc <= (uint32_t)G;  // wave-uniform
         constexpr uint32_t LS = S == 1 ? 0 : S == 2 ? 1 : S == 4 ? 2 : S == 8 ? 3 : S == 16 ? 4 : S == 32 ? 5 : 6;
@@ -634,9 +513,20 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
         };
         uint32_t lc = 0;
         if (kFp32) {
-            // t = w - 2 a.b + C approximates u + hw (the band of leaf_mx, same set-up): t <= 0 (sign
-            // bit set) is inside, 0 <= t <= 2 hw (unsigned compare of the bit patterns) is re-tested;
-            // a forced FP64 re-test is lim = ~0
+            // t = fma(ax, -2bx, fma(ay, -2by, fma(az, -2bz, w + C))) (circle: fma(ax, ax - 2bx,
+            // fma(ay, ay - 2by, C))) with C = fl32(fl32(hw - fl32(R)) + fl32(|b|^2)) approximates
+            // u + hw, u = |p - q|^2 - R. Only b = fl32(q - c) is formed in FP64; the rest of the set-up
+            // is FP32. Bounding the error for any association of the five terms, with every product
+            // and sum rounded to nearest or toward zero (u' = 2^-23), and with the representation
+            // errors of a, b, w and the FP32 set-up (unit u = 2^-24):
+            //   |t - (u + hw)| <= u (25 amax + 28 |b|^2 + 13 R + 12 hw) < 32 u (amax + |b|^2 + R);
+            // the circle form's fl32(ax - 2bx) adds at most u (2 amax + |b|^2). hw = 2^-18 fl32(fl32(amax +
+            // fl32|b|^2) + fl32 R) >= 2 * 32 u (amax + |b|^2 + R)(1 - 5u) covers both with a factor-2
+            // margin. Hence t <= 0 (sign bit set, -0 included) is inside, t > 2 hw is outside, and
+            // 0 <= t <= 2 hw - an unsigned compare of the bit patterns, which also sends +0 there - is
+            // re-tested in FP64. Slots past the leaf are sentinels (t ~ 1e30 or +inf, above 2 hw). A
+            // leaf/query pair with amax + |b|^2 + R >= 1e30 (or not finite: a non-finite point makes
+            // amax = inf) is re-tested in FP64 outright (lim = ~0), so no term can overflow.
             const float bx = (float)(q.x - nr.cx), by = (float)(q.y - nr.cy), bz = (float)(q.z - nr.cz);
             const float b2 = KERNEL == OCT_KERNEL_SPHERE ? __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz))
                                                          : __builtin_fmaf(bx, bx, by * by);
@@ -877,19 +767,6 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
     // the S lanes of a query hold parts of its count: lanes j, j + G, ... (G >= 16)
     if constexpr (G <= 16) cnt += lane_xor<16>(cnt);
     if constexpr (G <= 32) cnt += lane_xor32(cnt);
-    if (MX) {
-        // column totals: lanes c16, c16 + 16, c16 + 32, c16 + 48 hold query 16h + c16's row groups;
-        // lane j (< G) has c16 = j % 16 and takes block h = j / 16
-        uint32_t tot = 0;
-#pragma unroll
-        for (int h = 0; h < NH; h++) {
-            uint32_t t = mcnt[h];
-            t += __shfl_xor(t, 16);
-            t += __shfl_xor(t, 32);
-            if ((int)(j >> 4) == h) tot = t;
-        }
-        cnt += tot;
-    }
     if (overflow && lane == 0) atomicOr(a.err, kErrStack);
     if (valid && g == 0 && a.counts) a.counts[qid] = cnt;
     if (STATS) {
@@ -908,9 +785,9 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
     }
 }
 
-template <int KERNEL, int ALGO, int PREC, int G, bool STATS, bool MX>
-__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(count_waves<KERNEL, ALGO, PREC, MX>()))) void k_count(const SearchArgs a) {
-    count_tile<KERNEL, ALGO, PREC, G, STATS, MX>(a, blockIdx.x * kWaves + (threadIdx.x >> 6));
+template <int KERNEL, int ALGO, int PREC, int G, bool STATS>
+__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(count_waves<KERNEL, ALGO, PREC>()))) void k_count(const SearchArgs a) {
+    count_tile<KERNEL, ALGO, PREC, G, STATS>(a, blockIdx.x * kWaves + (threadIdx.x >> 6));
 }
 
 // Work-queue form (batches): a grid of resident waves; each wave takes the next wave tile from a
@@ -918,27 +795,27 @@ __global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(co
 // launch shape does not depend on how many tiles the queries made. Every wave leaves once the
 // counter has passed the count.
 template <int KERNEL, int ALGO, int PREC>
-__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(count_waves<KERNEL, ALGO, PREC, false>()))) void k_count_queue(const SearchArgs a) {
+__global__ __launch_bounds__(kSearchBlock) __attribute__((amdgpu_waves_per_eu(count_waves<KERNEL, ALGO, PREC>()))) void k_count_queue(const SearchArgs a) {
     const uint32_t ntiles = uniform_u32(*a.queue_tiles);
     for (;;) {
         uint32_t t = 0;
         if (lane_id() == 0) t = atomicAdd(a.queue_next, 1u);
         t = readlane_u32(t, 0);
         if (t >= ntiles) return;
-        count_tile<KERNEL, ALGO, PREC, 64, false, false>(a, t);
+        count_tile<KERNEL, ALGO, PREC, 64, false>(a, t);
     }
 }
 
-template <int KERNEL, int ALGO, int PREC, bool STATS, bool MX>
+template <int KERNEL, int ALGO, int PREC, bool STATS>
 static void launch_count_s(const SearchArgs &a, int G, hipStream_t s) {
     const size_t tiles = ((size_t)a.nq + G - 1) / G;
     const unsigned nb = (unsigned)((tiles + kWaves - 1) / kWaves);
     if (G == 16)
-        k_count<KERNEL, ALGO, PREC, 16, STATS, MX><<<nb, kSearchBlock, 0, s>>>(a);
+        k_count<KERNEL, ALGO, PREC, 16, STATS><<<nb, kSearchBlock, 0, s>>>(a);
     else if (G == 32)
-        k_count<KERNEL, ALGO, PREC, 32, STATS, MX><<<nb, kSearchBlock, 0, s>>>(a);
+        k_count<KERNEL, ALGO, PREC, 32, STATS><<<nb, kSearchBlock, 0, s>>>(a);
     else
-        k_count<KERNEL, ALGO, PREC, 64, STATS, MX><<<nb, kSearchBlock, 0, s>>>(a);
+        k_count<KERNEL, ALGO, PREC, 64, STATS><<<nb, kSearchBlock, 0, s>>>(a);
 }
 
 // resident waves of the work-queue kernel: blocks per CU at its occupancy, times the CUs
@@ -955,7 +832,7 @@ static unsigned queue_grid() {
 }
 
 template <int KERNEL, int ALGO, int PREC>
-static void launch_count(const SearchArgs &a, int G, bool mx, hipStream_t s) {
+static void launch_count(const SearchArgs &a, int G, hipStream_t s) {
     if constexpr (ALGO != OCT_ALGO_APPROX) {
         if (a.queue_next) {  // work-queue form: 64 queries per wave tile, vector ALUs, no counters
             k_count_queue<KERNEL, ALGO, PREC><<<queue_grid<KERNEL, ALGO, PREC>(), kSearchBlock, 0, s>>>(a);
@@ -965,30 +842,21 @@ static void launch_count(const SearchArgs &a, int G, bool mx, hipStream_t s) {
     if constexpr (ALGO == OCT_ALGO_APPROX) {  // one shape: 64 queries per wave, vector ALUs
         const unsigned nb = (unsigned)(((size_t)a.nq + 64 * kWaves - 1) / (64 * kWaves));
         if (a.stats)
-            k_count<KERNEL, ALGO, PREC, 64, true, false><<<nb, kSearchBlock, 0, s>>>(a);
+            k_count<KERNEL, ALGO, PREC, 64, true><<<nb, kSearchBlock, 0, s>>>(a);
         else
-            k_count<KERNEL, ALGO, PREC, 64, false, false><<<nb, kSearchBlock, 0, s>>>(a);
+            k_count<KERNEL, ALGO, PREC, 64, false><<<nb, kSearchBlock, 0, s>>>(a);
         return;
     }
-    if constexpr (KERNEL == OCT_KERNEL_SPHERE && PREC == OCT_PREC_FP32_RECHECK) {
-        if (mx) {
-            if (a.stats)
-                launch_count_s<KERNEL, ALGO, PREC, true, true>(a, G, s);
-            else
-                launch_count_s<KERNEL, ALGO, PREC, false, true>(a, G, s);
-            return;
-        }
-    }
     if (a.stats)
-        launch_count_s<KERNEL, ALGO, PREC, true, false>(a, G, s);
+        launch_count_s<KERNEL, ALGO, PREC, true>(a, G, s);
     else
-        launch_count_s<KERNEL, ALGO, PREC, false, false>(a, G, s);
+        launch_count_s<KERNEL, ALGO, PREC, false>(a, G, s);
 }
 
 template <int KERNEL, int ALGO, int PREC>
-static void launch_k(const SearchArgs &a, int mode, int G, bool mx, unsigned nb, hipStream_t s) {
+static void launch_k(const SearchArgs &a, int mode, int G, unsigned nb, hipStream_t s) {
     if (mode == 0 && G > 0)
-        launch_count<KERNEL, ALGO, PREC>(a, G, mx, s);
+        launch_count<KERNEL, ALGO, PREC>(a, G, s);
     else if (mode == 0)
         k_radius<KERNEL, ALGO, PREC, 0><<<nb, kSearchBlock, 0, s>>>(a);
     else if (mode == 1)
@@ -998,23 +866,23 @@ static void launch_k(const SearchArgs &a, int mode, int G, bool mx, unsigned nb,
 }
 
 template <int KERNEL, int ALGO>
-static void launch_p(const SearchArgs &a, int prec, int mode, int G, bool mx, unsigned nb, hipStream_t s) {
+static void launch_p(const SearchArgs &a, int prec, int mode, int G, unsigned nb, hipStream_t s) {
     if (prec == OCT_PREC_FP32_RECHECK)
-        launch_k<KERNEL, ALGO, OCT_PREC_FP32_RECHECK>(a, mode, G, mx, nb, s);
+        launch_k<KERNEL, ALGO, OCT_PREC_FP32_RECHECK>(a, mode, G, nb, s);
     else
-        launch_k<KERNEL, ALGO, OCT_PREC_FP64>(a, mode, G, mx, nb, s);
+        launch_k<KERNEL, ALGO, OCT_PREC_FP64>(a, mode, G, nb, s);
 }
 
 template <int KERNEL>
-static void launch_a(const SearchArgs &a, int algo, int prec, int mode, int G, bool mx, unsigned nb, hipStream_t s) {
+static void launch_a(const SearchArgs &a, int algo, int prec, int mode, int G, unsigned nb, hipStream_t s) {
     if (algo == OCT_ALGO_NEIGHBORS)
-        launch_p<KERNEL, OCT_ALGO_NEIGHBORS>(a, prec, mode, G, mx, nb, s);
+        launch_p<KERNEL, OCT_ALGO_NEIGHBORS>(a, prec, mode, G, nb, s);
     else if (algo == OCT_ALGO_PRUNE)
-        launch_p<KERNEL, OCT_ALGO_PRUNE>(a, prec, mode, G, mx, nb, s);
+        launch_p<KERNEL, OCT_ALGO_PRUNE>(a, prec, mode, G, nb, s);
     else if (algo == OCT_ALGO_STRUCT)
-        launch_p<KERNEL, OCT_ALGO_STRUCT>(a, prec, mode, G, mx, nb, s);
+        launch_p<KERNEL, OCT_ALGO_STRUCT>(a, prec, mode, G, nb, s);
     else
-        launch_p<KERNEL, OCT_ALGO_APPROX>(a, prec, mode, G, mx, nb, s);
+        launch_p<KERNEL, OCT_ALGO_APPROX>(a, prec, mode, G, nb, s);
 }
 
 hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
@@ -1057,10 +925,10 @@ hipError_t launch_radius(oct_ctx *ctx, const RadiusLaunch &L) {
     stage_begin(ctx, ST_SEARCH);
     const int G = L.group;
     switch (L.kernel) {
-    case OCT_KERNEL_SPHERE: launch_a<OCT_KERNEL_SPHERE>(a, L.algo, L.precision, L.mode, G, L.mx, nb, ctx->stream); break;
-    case OCT_KERNEL_CIRCLE: launch_a<OCT_KERNEL_CIRCLE>(a, L.algo, L.precision, L.mode, G, L.mx, nb, ctx->stream); break;
-    case OCT_KERNEL_CUBE: launch_a<OCT_KERNEL_CUBE>(a, L.algo, L.precision, L.mode, G, L.mx, nb, ctx->stream); break;
-    default: launch_a<OCT_KERNEL_SQUARE>(a, L.algo, L.precision, L.mode, G, L.mx, nb, ctx->stream); break;
+    case OCT_KERNEL_SPHERE: launch_a<OCT_KERNEL_SPHERE>(a, L.algo, L.precision, L.mode, G, nb, ctx->stream); break;
+    case OCT_KERNEL_CIRCLE: launch_a<OCT_KERNEL_CIRCLE>(a, L.algo, L.precision, L.mode, G, nb, ctx->stream); break;
+    case OCT_KERNEL_CUBE: launch_a<OCT_KERNEL_CUBE>(a, L.algo, L.precision, L.mode, G, nb, ctx->stream); break;
+    default: launch_a<OCT_KERNEL_SQUARE>(a, L.algo, L.precision, L.mode, G, nb, ctx->stream); break;
     }
     stage_end(ctx, ST_SEARCH);
     return hipGetLastError();

@@ -747,15 +747,15 @@ class Octree:
         self._chk(self.L.oct_set_tuning(self.h, 1, g), "oct_set_tuning")
 
     def set_mfma(self, on):
-        """FP32 sphere count-mode point tests on the matrix cores (True), the vector ALUs (False), or
-        autotuned / vector ALUs (None, the default)."""
+        """The OCT_TUNE_RADIUS_MFMA knob: the point tests run on the vector ALUs, so None and False are
+        accepted and True raises OctError (no matrix-core form is built)."""
         self._chk(self.L.oct_set_tuning(self.h, 2, -1 if on is None else (1 if on else 0)), "oct_set_tuning")
 
     def tuned_radius(self, kernel, algo, precision):
-        """(queries per wave, matrix cores?) a count-mode search would use now."""
+        """Queries per wave a count-mode search would use now."""
         g, m = C.c_int(0), C.c_int(0)
         self._chk(self.L.oct_tuned_radius(self.h, kernel, algo, precision, C.byref(g), C.byref(m)), "oct_tuned_radius")
-        return g.value, bool(m.value)
+        return g.value
 
     def autotune_radius(self, kernel, algo, precision, radius, sample_nq=1 << 20):
         best = C.c_int(0)

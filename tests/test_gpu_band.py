@@ -53,20 +53,16 @@ def check_all(ctx, sx, tree, centers, radii, kernels=KERNELS, algos=ALGOS, tag="
                     if not np.array_equal(lists[q], ref[q]):
                         raise AssertionError(f"{tag} k={kernel} a={algo} p={prec} query {q}: lists differ")
             try:
-                for mx, remap in ((True, None), (False, None), (False, False)):
-                    ctx.set_mfma(mx)
+                for remap in (None, False):
                     ctx.set_remap(remap)
                     for g in (0, 16, 32, 64):
-                        if g == 0 and mx:
-                            continue  # the one-lane kernel has no matrix-core variant
                         ctx.set_group(g)
                         for prec in (A.PREC_FP64, A.PREC_FP32):
                             counts, _ = ctx.radius(kernel, algo, prec, centers=centers, radii=radii)
                             np.testing.assert_array_equal(
-                                counts, sizes, err_msg=f"{tag} counts k={kernel} a={algo} G={g} mx={mx} remap={remap} p={prec}")
+                                counts, sizes, err_msg=f"{tag} counts k={kernel} a={algo} G={g} remap={remap} p={prec}")
             finally:
                 ctx.set_group(-1)
-                ctx.set_mfma(None)
                 ctx.set_remap(None)
     return rechecked
 

@@ -33,19 +33,17 @@ def run_both(ctx, pts, enc, ml, kernel, algo, centers, radius):
 
 
 def check_counts(ctx, kernel, algo, centers, radius, ref_sizes):
-    """Every count-mode variant (tile shapes, matrix-core / vector FP32 point tests) on the same case."""
+    """Every count-mode variant (tile shapes, leaf remapping on and off, both precisions) on the same case."""
     try:
-        for mx, remap in ((True, None), (False, None), (False, False)):
-            ctx.set_mfma(mx)
+        for remap in (None, False):
             ctx.set_remap(remap)
             for g in (0, 16, 32, 64):
                 ctx.set_group(g)
                 for prec in (A.PREC_FP64, A.PREC_FP32):
                     counts, _ = ctx.radius(kernel, algo, prec, centers=np.asarray(centers, np.float64), radius=radius)
-                    np.testing.assert_array_equal(counts, ref_sizes, err_msg=f"mx={mx} remap={remap} G={g} prec={prec}")
+                    np.testing.assert_array_equal(counts, ref_sizes, err_msg=f"remap={remap} G={g} prec={prec}")
     finally:
         ctx.set_group(-1)
-        ctx.set_mfma(None)
         ctx.set_remap(None)
 
 

@@ -121,47 +121,43 @@ def test_radius_lists(case, ctx, kernel, algo):
 
 
 @pytest.mark.parametrize("group", [0, 16, 32, 64])
-@pytest.mark.parametrize("mx", [True, False], ids=["mfma", "valu"])
-def test_radius_counts_every_group(case, ctx, group, mx):
-    """Count-mode kernels (one lane per query, or G queries per wave; FP32 sphere point tests on the
-    matrix cores or the vector ALUs) against the oracle sizes."""
-    if group == 0 and mx:
-        pytest.skip("the one-lane-per-query kernel has no matrix-core variant (its vector-ALU run is the valu case)")
+def test_radius_counts_every_group(case, ctx, group):
+    """Count-mode kernels (one lane per query, or G queries per wave; leaf remapping on and off)
+    against the oracle sizes."""
     name, xyz, enc, ml, (sx, keys, perm, c, r, tree), _ = case
     ensure(ctx, case)
     idx = A.random_indices(3000, len(xyz), seed=17)
     ctx.set_group(group)
-    ctx.set_mfma(mx)
     try:
         for kernel in (A.K_SPHERE, A.K_CIRCLE, A.K_CUBE, A.K_SQUARE):
             for algo in (A.A_NEIGHBORS, A.A_PRUNE, A.A_STRUCT):
                 ref = O.radius_count(tree, sx, kernel, algo, sx[idx.astype(np.int64)], RADII[name][-1])
-                for remap in ((None,) if mx else (None, False)):  # leaf remapping on / off (vector path)
+                for remap in (None, False):
                     ctx.set_remap(remap)
                     for prec in (A.PREC_FP64, A.PREC_FP32):
                         counts, st = ctx.radius(kernel, algo, prec, idx=idx, radius=RADII[name][-1])
                         np.testing.assert_array_equal(counts, ref, err_msg=f"G={group} k={kernel} a={algo} p={prec} remap={remap}")
     finally:
         ctx.set_group(-1)
-        ctx.set_mfma(None)
         ctx.set_remap(None)
 
 
-@pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32, "fp32-mfma"])
+def test_matrix_core_knob_refused(ctx):
+    """The point tests run on the vector ALUs: the matrix-core switch accepts only off / default."""
+    ctx.set_mfma(False)
+    ctx.set_mfma(None)
+    with pytest.raises(A.OctError):
+        ctx.set_mfma(True)
+
+
+@pytest.mark.parametrize("prec", [A.PREC_FP64, A.PREC_FP32])
 def test_radius_all_points_counts(case, ctx, prec):
     """All-points counts (the harness' -s all mode) against the oracle on a strided sample."""
     name, xyz, enc, ml, (sx, keys, perm, c, r, tree), _ = case
     ensure(ctx, case)
     n = len(xyz)
     say(f"[gpu] all-points {name} prec={prec}")
-    if prec == "fp32-mfma":  # FP32 point tests on the matrix cores
-        ctx.set_mfma(True)
-        try:
-            counts, st = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, radius=2.5, nq=n)
-        finally:
-            ctx.set_mfma(None)
-    else:
-        counts, st = ctx.radius(A.K_SPHERE, A.A_STRUCT, prec, radius=2.5, nq=n)
+    counts, st = ctx.radius(A.K_SPHERE, A.A_STRUCT, prec, radius=2.5, nq=n)
     sample = np.arange(0, n, 97)
     ref = O.radius_count(tree, sx, O.K_SPHERE, O.A_STRUCT, sx[sample], 2.5)
     np.testing.assert_array_equal(counts[sample], ref)
