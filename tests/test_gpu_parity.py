@@ -164,6 +164,28 @@ def test_radius_all_points_counts(case, ctx, prec):
     assert st["total_neighbors"] == int(counts.astype(np.int64).sum())
 
 
+def test_radius_dev_index_buffer(case, ctx):
+    """oct_radius_dev with a device buffer of sorted-cloud indices as the centres (one SFC range of
+    the queries, as a GPU's share of a multi-GPU search): its neighbour total equals the host-index
+    entry point's counts over the same centres, which equal the oracle's."""
+    name, xyz, enc, ml, (sx, keys, perm, c, r, tree), _ = case
+    ensure(ctx, case)
+    n = len(xyz)
+    q0, nq = n // 3, min(20_000, n - n // 3)
+    idx = np.arange(q0, q0 + nq, dtype=np.uint32)
+    say(f"[gpu] device index buffer {name} [{q0}, {q0 + nq})")
+    counts, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, idx=idx, radius=2.5)
+    ref = O.radius_count(tree, sx, O.K_SPHERE, O.A_STRUCT, sx[idx.astype(np.int64)], 2.5)
+    np.testing.assert_array_equal(counts, ref)
+    buf = A.DeviceArray(np.arange(n, dtype=np.uint32))
+    try:
+        for prec in (A.PREC_FP32, A.PREC_FP64):
+            st = ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, prec, 2.5, nq=nq, stats=True, idx_ptr=buf.ptr(q0))
+            assert st["total_neighbors"] == int(ref.astype(np.int64).sum())
+    finally:
+        buf.close()
+
+
 _KNN_CASE = {}
 
 
