@@ -406,29 +406,6 @@ def main():
 
     fp32, fp64 = results["fp32"], results["fp64"]
 
-    # One GPU's share of the all-points search on 8 GPUs: the queries split into 8 contiguous SFC
-    # ranges, each searched against the whole (replicated) cloud and tree, FP32 + recheck as tuned
-    shard = None
-    if n >= 8:
-        bounds = np.linspace(0, n, 9).astype(np.int64)
-        sidx = A.DeviceArray(np.arange(n, dtype=np.uint32))  # the sorted-cloud indices, in SFC order
-        try:
-            sms, snb = [], 0
-            for s in range(8):
-                q0, nq_s = int(bounds[s]), int(bounds[s + 1] - bounds[s])
-                ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, args.radius, nq=nq_s, idx_ptr=sidx.ptr(q0))
-                ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, args.radius, nq=nq_s, idx_ptr=sidx.ptr(q0))
-                sms.append(float(ctx.stage_times()["search"]))
-                st_s = ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, args.radius, nq=nq_s, stats=True,
-                                      idx_ptr=sidx.ptr(q0))
-                snb += int(st_s["total_neighbors"])
-        finally:
-            sidx.close()
-        shard = {"gpus": 8, "queries_per_shard": int(bounds[1] - bounds[0]), "kernel_ms": sms,
-                 "max_ms": max(sms), "sum_ms": sum(sms), "balance": float(np.mean(sms) / max(sms)),
-                 "neighbors_eq_all_points": snb == int(fp32["stats"]["total_neighbors"])}
-        log(f"[bench] 8 query shards of {shard['queries_per_shard']}: kernel ms {[round(v, 2) for v in sms]}")
-
     # kNN (KNNV2) on the same tile: every point as a query (the first nq SFC positions for large k, the
     # per-query work being additive), int32 indices + FP32 squared distances kept on the device; each
     # precision with its autotuned lanes-per-query shape
@@ -464,6 +441,37 @@ def main():
         c64, _ = ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP64, radius=args.radius, nq=n)
         verify["fp32_eq_fp64_all_queries"] = bool(np.array_equal(c32, c64))
         verify["avg_result_size"] = int(c64.astype(np.int64).sum() // max(n, 1))
+
+    # One GPU's share of the all-points search on 8 GPUs: the queries split into 8 contiguous SFC
+    # ranges, each searched against the whole (replicated) cloud and tree, FP32 + recheck as tuned;
+    # once with equal query counts and once balanced by work (equal neighbour counts + 64 per query,
+    # from this run's all-points counts, as a previous pass would give them)
+    shard = None
+    if n >= 8:
+        counts_w = c32 if not args.no_verify else ctx.radius(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32,
+                                                             radius=args.radius, nq=n)[0]
+        cum = np.cumsum(counts_w.astype(np.int64) + 64)
+        splits = {"equal_queries": np.linspace(0, n, 9).astype(np.int64),
+                  "balanced": np.concatenate([[0], np.searchsorted(cum, cum[-1] * np.arange(1, 8) / 8), [n]])}
+        sidx = A.DeviceArray(np.arange(n, dtype=np.uint32))  # the sorted-cloud indices, in SFC order
+        shard = {"gpus": 8}
+        try:
+            for sname, bounds in splits.items():
+                sms, snb = [], 0
+                for s in range(8):
+                    q0, nq_s = int(bounds[s]), int(bounds[s + 1] - bounds[s])
+                    for _ in range(2):
+                        ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, args.radius, nq=nq_s, idx_ptr=sidx.ptr(q0))
+                    sms.append(float(ctx.stage_times()["search"]))
+                    st_s = ctx.radius_dev(A.K_SPHERE, A.A_STRUCT, A.PREC_FP32, args.radius, nq=nq_s, stats=True,
+                                          idx_ptr=sidx.ptr(q0))
+                    snb += int(st_s["total_neighbors"])
+                shard[sname] = {"queries": [int(bounds[s + 1] - bounds[s]) for s in range(8)], "kernel_ms": sms,
+                                "max_ms": max(sms), "sum_ms": sum(sms), "balance": float(np.mean(sms) / max(sms)),
+                                "neighbors_eq_all_points": snb == int(fp32["stats"]["total_neighbors"])}
+                log(f"[bench] 8 query shards, {sname}: kernel ms {[round(v, 2) for v in sms]}")
+        finally:
+            sidx.close()
 
     cpu = None
     competitor = None
