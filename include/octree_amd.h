@@ -241,11 +241,13 @@ int oct_knn_checksum(oct_ctx *ctx, const int32_t *idx_dev, const float *d2_dev, 
  * all-points queries (16 strided chunks, as above; 0 = 2^18), stores the fastest for (precision,
  * size class of k), returns it in *best and the best milliseconds per shape in ms_out[0..2].
  * oct_tuned_knn: the lanes per query a kNN call with this k would use now.
- * OCT_TUNE_SORT: how oct_sort sorts more than 2048 (4096) keys: 1 (default, also -1) orders them by
- * their top 16 bits in two device-wide passes, then sorts each 16-bit segment inside one workgroup
- * (2048 keys per LDS sort), which also writes the sorted cloud; 2 the same with 4096-key LDS sorts;
- * 0 runs the device-wide LSD pass per 8-bit digit and a separate gather. A 16-bit segment above
- * 32768 keys selects the LSD passes. Results never depend on the tuning. */
+ * OCT_TUNE_SORT: how oct_sort sorts more than 2048 (4096) keys: 1 orders them by their top 16 bits
+ * in two device-wide passes, then sorts each 16-bit segment inside one workgroup (2048 keys per LDS
+ * sort), which also writes the sorted cloud; 2 the same with 4096-key LDS sorts; 0 runs the
+ * device-wide LSD pass per 8-bit digit and a separate gather. A 16-bit segment above 32768 keys
+ * selects the LSD passes. -1 (the default) is 1, except that once a sort of the current points has
+ * fallen back to the LSD passes, later sorts of the same points run them directly (new points reset
+ * this). Results never depend on the tuning. */
 enum oct_tune_knob {
     OCT_TUNE_RADIUS_GROUP = 1,
     OCT_TUNE_RADIUS_MFMA = 2,

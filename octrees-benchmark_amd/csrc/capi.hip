@@ -324,6 +324,7 @@ static int set_sorted_impl(oct_ctx *ctx, size_t n, const uint64_t *codes, int en
     ctx->encoder = encoder;
     for (int a = 0; a < 3; a++) ctx->center[a] = center[a], ctx->radii[a] = radii[a];
     ctx->has_points = true;
+    ctx->sort_lsd_learned = false;
     ctx->has_keys = false;
     ctx->has_sorted = true;
     ctx->has_tree = ctx->has_lists = false;
@@ -411,6 +412,7 @@ int oct_set_points(oct_ctx *ctx, const double *xyz, size_t n, size_t stride_byte
     if (r) return r;
     HCHK(hipStreamSynchronize(ctx->stream), "copy points");
     ctx->has_points = true;
+    ctx->sort_lsd_learned = false;
     ctx->has_keys = ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
     return OCT_OK;
 }
@@ -422,6 +424,7 @@ int oct_set_points_soa(oct_ctx *ctx, const double *x, const double *y, const dou
     if (r) return r;
     HCHK(hipStreamSynchronize(ctx->stream), "copy points");
     ctx->has_points = true;
+    ctx->sort_lsd_learned = false;
     ctx->has_keys = ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
     return OCT_OK;
 }
@@ -432,6 +435,7 @@ int oct_set_points_dev(oct_ctx *ctx, const double *xyz_dev, size_t n, size_t str
     int r = upload_xyz(ctx, xyz_dev, n, stride_bytes, hipMemcpyDeviceToDevice);
     if (r) return r;
     ctx->has_points = true;
+    ctx->sort_lsd_learned = false;
     ctx->has_keys = ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
     return OCT_OK;
 }

@@ -109,6 +109,7 @@ struct oct_ctx {
     oct::DevBuf<uint32_t> misc;      // tile counters, flags, scan partials
     oct::DevBuf<uint32_t> msd;             // partition sort: bases, prefix counts, segment jobs, counters
     int sort_override = -1;                // OCT_TUNE_SORT
+    bool sort_lsd_learned = false;         // a default-form sort of these points fell back to the LSD passes
     bool p64_sorted = false;               // the last sort also wrote the sorted cloud (p64)
     bool hist_lsd = false;                 // the encode kernel wrote the 8 LSD digit histograms too
     oct::DevBuf<uint64_t> keys_p;          // partition sort: keys ordered by their top 16 bits

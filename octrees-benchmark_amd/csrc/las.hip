@@ -251,6 +251,7 @@ int oct_las_load(oct_ctx *ctx, const char *path, oct_las_header *h_out) {
     ctx->las_n = n;
     ctx->las_loaded = true;
     ctx->has_points = true;
+    ctx->sort_lsd_learned = false;
     ctx->has_keys = ctx->has_sorted = ctx->has_tree = ctx->has_lists = false;
     if (h_out) *h_out = h;
     return OCT_OK;

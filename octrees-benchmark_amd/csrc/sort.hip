@@ -765,7 +765,12 @@ hipError_t sort_lsd(oct_ctx *ctx, const uint32_t *h, uint32_t *counters, uint32_
 }
 
 // OCT_TUNE_SORT: 0 = LSD passes, 1 / 2 = partition sort with 2048- / 4096-key LDS sorts (1 by default)
-int sort_mode(const oct_ctx *ctx) { return ctx->sort_override < 0 ? 1 : ctx->sort_override; }
+// the default form is the partition sort, unless a sort of the current points already had to fall
+// back to the LSD passes (a 16-bit segment above the workgroup limit): then the LSD passes directly
+int sort_mode(const oct_ctx *ctx) {
+    if (ctx->sort_override >= 0) return ctx->sort_override;
+    return ctx->sort_lsd_learned ? 0 : 1;
+}
 
 }  // namespace
 
@@ -830,7 +835,8 @@ hipError_t launch_sort(oct_ctx *ctx, std::string &err) {
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
     if (!fb[0] && partition && fb[1] > kSegMaxLen) {
         // a segment too long for one workgroup: the LSD passes over the untouched encoded keys, timed
-        // as part of the same stage
+        // as part of the same stage; later default-form sorts of these points go to them directly
+        ctx->sort_lsd_learned = true;
         if (!ctx->hist_lsd) {
             if ((e = hipMemsetAsync(ctx->hist.p, 0, kRadixPassCount * 256 * sizeof(uint32_t), ctx->stream)) != hipSuccess) return e;
             const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 1024));

@@ -146,3 +146,20 @@ def test_sort_points_equals_separate_calls(ctx, restore_sort, name, make, mode):
     assert np.array_equal(r1.view(np.uint64), r2.view(np.uint64))
     np.testing.assert_array_equal(k1, k2)
     np.testing.assert_array_equal(p1, p2)
+
+
+def test_default_sort_remembers_fallback(ctx, restore_sort):
+    """Default form on a cloud whose dense cluster makes a 16-bit segment above the workgroup limit:
+    the first sort falls back to the LSD passes, the next sorts of the same points run them directly,
+    and new points start from the partition sort again - every result the oracle's."""
+    ctx.set_sort(None)
+    for name in ("cluster_60k_hilbert", "uniform_300k_morton", "cluster_150k_morton"):
+        xyz, enc = CLOUDS[name]()
+        _, keys, perm, c, r = O.sort_points(enc, xyz)
+        ctx.set_points(xyz)
+        ctx.current = None
+        for rep in range(2):
+            say(f"[sort] default form {name} rep {rep}")
+            gc, gr, gperm, gkeys = ctx.sort_points(enc)
+            np.testing.assert_array_equal(gkeys, keys)
+            np.testing.assert_array_equal(gperm, perm)
