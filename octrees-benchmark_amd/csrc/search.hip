@@ -331,13 +331,13 @@ __global__ __launch_bounds__(kSearchBlock) void k_radius(const SearchArgs a) {
 // depend on the order of the traversal, which is therefore free.
 
 // Occupancy targets (waves per SIMD), measured on the 50 M-point tile for the sphere kernel with the
-// Struct / Prune algorithms: 8 for the FP32 vector path (64 VGPRs, a 20-byte spill, 1.5 % faster
-// than 7), 6 for FP64 (80 VGPRs). The other variants keep the compiler's choice (forcing them spills
-// registers).
+// Struct / Prune algorithms: 7 for the FP32 vector path (71 VGPRs, no scratch; at 8 the 64-VGPR and
+// 72-SGPR budget spills the children's judgement masks and the search runs 3 % slower), 6 for FP64
+// (80 VGPRs). The other variants keep the compiler's choice (forcing them spills registers).
 template <int KERNEL, int ALGO, int PREC>
 constexpr int count_waves() {
     if (ALGO == OCT_ALGO_NEIGHBORS || !(KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE)) return 2;
-    return PREC == OCT_PREC_FP32_RECHECK ? 8 : 6;
+    return PREC == OCT_PREC_FP32_RECHECK ? 7 : 6;
 }
 // One wave tile (G queries) of a count-mode search, run by one wave.
 template <int KERNEL, int ALGO, int PREC, int G, bool STATS>
@@ -619,14 +619,16 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
     // per axis (NodeExt), so |q - c|, the kernel-box bounds and the squared extents are computed
     // once per axis value and combined per child, in the reference's operation order, e.g. for the
     // sphere (ex*ex + ey*ey) + ez*ez with ex = |q.x - cx| + rx (kernel_3d.hpp:181-223). Each lane
-    // judges all 8 children for its own query (the S lanes of a query agree); results are lane
-    // masks, so the per-child decisions are scalar-unit work.
-    auto judge8 = [&](ConstNodeExt &E, uint32_t d, bool act, bool (&in)[8], bool (&ov)[8]) {
+    // judges all 8 children for its own query (the S lanes of a query agree). Every comparison
+    // becomes a lane mask at once (ballot), so the per-child decisions are scalar-unit operations
+    // on masks, and the results are lane masks: inm[c] (INSIDE) and ovm[c] (OVERLAP), both within
+    // actm, the lanes whose query wants the popped node.
+    auto judge8 = [&](ConstNodeExt &E, uint32_t d, uint64_t actm, uint64_t (&inm)[8], uint64_t (&ovm)[8]) {
         if (ALGO == OCT_ALGO_APPROX && d > a.approx_level) {
             // neighborsApprox beyond its precision level: every child whole, or none
             // (linear_octree.hpp:987-994)
 #pragma unroll
-            for (int c = 0; c < 8; c++) in[c] = act && a.approx_upper, ov[c] = false;
+            for (int c = 0; c < 8; c++) inm[c] = a.approx_upper ? actm : 0ull, ovm[c] = 0ull;
             return;
         }
         const double rx = radp[3 * d], ry = radp[3 * d + 1], rz = radp[3 * d + 2];
@@ -634,58 +636,68 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
         if (ALGO == OCT_ALGO_NEIGHBORS || KERNEL == OCT_KERNEL_CUBE || KERNEL == OCT_KERNEL_SQUARE) {
             // boxOverlap (kernel_3d.hpp:100-115, kernel_2d.hpp:29-41) and the box kernels'
             // boxIntersect (kernel_3d.hpp:277-302, kernel_2d.hpp:195-218): per-axis comparisons
-            bool ox[2], oy[2], oz[2], ix[2], iy[2], iz[2];
+            uint64_t ox[2], oy[2], oz[2], ix[2], iy[2], iz[2];
 #pragma unroll
             for (int b = 0; b < 2; b++) {
                 const double hx = E.ax[b] + rx, lx = E.ax[b] - rx, hy = E.ax[2 + b] + ry, ly = E.ax[2 + b] - ry;
                 const double hz = E.ax[4 + b] + rz, lz = E.ax[4 + b] - rz;
-                ox[b] = hx < q.lox || lx > q.hix;
-                oy[b] = hy < q.loy || ly > q.hiy;
-                oz[b] = is3d && (hz < q.loz || lz > q.hiz);
-                ix[b] = hx <= q.hix && lx >= q.lox;
-                iy[b] = hy <= q.hiy && ly >= q.loy;
-                iz[b] = !is3d || (hz <= q.hiz && lz >= q.loz);
+                ox[b] = ballot64(hx < q.lox || lx > q.hix);
+                oy[b] = ballot64(hy < q.loy || ly > q.hiy);
+                oz[b] = is3d ? ballot64(hz < q.loz || lz > q.hiz) : 0ull;
+                ix[b] = ballot64(hx <= q.hix && lx >= q.lox);
+                iy[b] = ballot64(hy <= q.hiy && ly >= q.loy);
+                iz[b] = is3d ? ballot64(hz <= q.hiz && lz >= q.loz) : ~0ull;
             }
 #pragma unroll
             for (int c = 0; c < 8; c++) {
                 const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-                const bool out = ox[bx] || oy[by] || oz[bz];
-                const bool inside = ALGO != OCT_ALGO_NEIGHBORS && !out && ix[bx] && iy[by] && iz[bz];
-                in[c] = act && inside;
-                ov[c] = act && !out && !inside;
+                const uint64_t keep = actm & ~(ox[bx] | oy[by] | oz[bz]);
+                const uint64_t inside = ALGO != OCT_ALGO_NEIGHBORS ? keep & ix[bx] & iy[by] & iz[bz] : 0ull;
+                inm[c] = inside;
+                ovm[c] = keep & ~inside;
             }
         } else {
             // sphere / circle boxIntersect (kernel_3d.hpp:181-223, kernel_2d.hpp:107-144)
             double e2x[2], e2y[2], e2z[2], m2x[2], m2y[2], m2z[2];
-            bool ox[2], oy[2], oz[2], nx[2], ny[2], nz[2];
+            uint64_t ox[2], oy[2], oz[2], nx[2], ny[2], nz[2];
             const double mdx = q.r + rx, mdy = q.r + ry, mdz = q.r + rz;
 #pragma unroll
             for (int b = 0; b < 2; b++) {
-                const double dx = fabs(q.x - E.ax[b]), dy = fabs(q.y - E.ax[2 + b]), dz = fabs(q.z - E.ax[4 + b]);
-                ox[b] = dx > mdx, oy[b] = dy > mdy, oz[b] = is3d && dz > mdz;
-                const double ex = dx + rx, ey = dy + ry, ez = dz + rz;
-                e2x[b] = ex * ex, e2y[b] = ey * ey, e2z[b] = ez * ez;
-                nx[b] = dx < rx, ny[b] = dy < ry, nz[b] = is3d && dz < rz;
-                const double fx = dmax0(dx - rx), fy = dmax0(dy - ry), fz = dmax0(dz - rz);
-                m2x[b] = fx * fx, m2y[b] = fy * fy, m2z[b] = fz * fz;
+                const double dx = fabs(q.x - E.ax[b]), dy = fabs(q.y - E.ax[2 + b]);
+                ox[b] = ballot64(dx > mdx), oy[b] = ballot64(dy > mdy);
+                nx[b] = ballot64(dx < rx), ny[b] = ballot64(dy < ry);
+                const double ex = dx + rx, ey = dy + ry;
+                e2x[b] = ex * ex, e2y[b] = ey * ey;
+                const double fx = dmax0(dx - rx), fy = dmax0(dy - ry);
+                m2x[b] = fx * fx, m2y[b] = fy * fy;
+                if (is3d) {
+                    const double dz = fabs(q.z - E.ax[4 + b]);
+                    oz[b] = ballot64(dz > mdz), nz[b] = ballot64(dz < rz);
+                    const double ez = dz + rz, fz = dmax0(dz - rz);
+                    e2z[b] = ez * ez, m2z[b] = fz * fz;
+                } else {
+                    oz[b] = nz[b] = 0ull, e2z[b] = m2z[b] = 0.0;
+                }
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {  // (bx, by) pairs; each partial sum serves both z halves
                 const int bx = k & 1, by = k >> 1;
                 const double sxy = e2x[bx] + e2y[by], txy = m2x[bx] + m2y[by];
+                const uint64_t oxy = ox[bx] | oy[by], nxy = nx[bx] & ny[by], nxoy = nx[bx] | ny[by];
 #pragma unroll
                 for (int bz = 0; bz < (is3d ? 2 : 1); bz++) {
                     const int c = k | (bz << 2);
-                    const bool out = ox[bx] || oy[by] || oz[bz];
+                    const uint64_t keep = actm & ~(oxy | oz[bz]);
                     const double smax = is3d ? sxy + e2z[bz] : sxy;
                     const double smin = is3d ? txy + m2z[bz] : txy;
-                    const bool inside = !out && smax <= q.R;
-                    const bool two = is3d ? ((nx[bx] && ny[by]) || (nx[bx] && nz[bz]) || (ny[by] && nz[bz])) : (nx[bx] && ny[by]);
-                    in[c] = act && inside;
-                    ov[c] = act && !out && !inside && (two || smin <= q.R);
+                    const uint64_t inside = keep & ballot64(smax <= q.R);
+                    // two of the three axes inside the node's slab: (nx && ny) || (nz && (nx || ny))
+                    const uint64_t two = is3d ? (nxy | (nz[bz] & nxoy)) : nxy;
+                    inm[c] = inside;
+                    ovm[c] = keep & ~inside & (two | ballot64(smin <= q.R));
                     if (!is3d) {  // 2-D kernels: the z half of the children repeats the judgement
-                        in[c | 4] = in[c];
-                        ov[c | 4] = ov[c];
+                        inm[c | 4] = inm[c];
+                        ovm[c | 4] = ovm[c];
                     }
                 }
             }
@@ -722,16 +734,20 @@ __device__ __attribute__((always_inline)) inline void count_tile(const SearchArg
         const uint64_t m = uniform_u64(s_mask[w][sp]);
         const uint32_t d = uniform_u32(s_depth[w][sp]) + 1;  // the children's level
         ConstNodeExt &E = node_ext(a.next, (fc - 1) >> 3);
-        const bool inm = (m & qbit) != 0;
-        bool in[8], ovl[8];
-        judge8(E, d, inm, in, ovl);
+        // lanes whose query wants the node: bit j of m for lane g * G + j
+        const uint64_t actm = G == 64 ? m : ballot64((m & qbit) != 0);
+        uint64_t inl[8], ovl[8];
+        judge8(E, d, actm, inl, ovl);
         if (STATS) n_class += 8ull * __popcll(m), n_cslots += 64ull * 8;
+        // INSIDE children first: group 0's lanes (0 .. G - 1) hold the counts
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            if (__builtin_amdgcn_inverse_ballot_w64(inl[c] & gmask)) cnt += E.last[c] - E.first[c];
         uint32_t leaves = 0;  // combos of the OVERLAP non-empty leaf children (bit c)
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const uint32_t f = E.first[c], l = E.last[c];
-            if (g == 0 && in[c]) cnt += l - f;
-            const uint64_t ov = ballot64(ovl[c]) & gmask;
+            const uint64_t ov = ovl[c] & gmask;  // ballots: uniform already
             if (ov) {
                 const uint32_t gc = E.gc[c];
                 if (gc == 0) {
