@@ -344,9 +344,11 @@ template <int KERNEL, int ALGO, int PREC, int G, bool STATS>
 __device__ __attribute__((always_inline)) inline void count_tile(const SearchArgs &a, const uint32_t tile) {
     constexpr int S = 64 / G;
     constexpr bool kFp32 = PREC == OCT_PREC_FP32_RECHECK && (KERNEL == OCT_KERNEL_SPHERE || KERNEL == OCT_KERNEL_CIRCLE);
-    // real points per staging chunk: 64 FP32 or 32 FP64 points (leaves average ~30 points; a small
-    // chunk keeps the LDS per block low enough for 7 (FP32) or 6 (FP64) waves per SIMD)
-    constexpr int CAP = kFp32 ? kStageBytes / 32 : kStageBytes / 64;
+    // real points per staging chunk: 64 FP32 or 48 FP64 points (leaves average ~30 points; a small
+    // chunk keeps the LDS per block low enough for 7 (FP32) or 6 (FP64) waves per SIMD: 19.0 / 26.0 KB
+    // per 4-wave block). Both take two loads per lane; 96 FP32 points (three) ran 4 % slower, 48
+    // FP64 points instead of 32 (fewer chunks for the larger leaves) 4 % faster.
+    constexpr int CAP = kFp32 ? kStageBytes / 32 : 48;
     constexpr int SMX = S > 16 ? S : 16;                               // largest lane-group count of a leaf
     constexpr int PAD = kUnroll * SMX + SMX;                           // sentinel + row-padding slots after a chunk
     __shared__ uint32_t s_node[kWaves][kStack];
